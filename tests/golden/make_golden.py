@@ -1,0 +1,227 @@
+"""Write golden vectors from the reference's own compiled sources.
+
+Runs only in the development container (it needs the reference tree):
+    make -C oracle ref && python tests/golden/make_golden.py
+The driver oracle/_ref/ref_driver is built from the reference's
+include/xdg/geometry/plucker.h, face_common.h, src/geometry/closest.cpp and
+src/mesh_manager_interface.cpp (MeshMock from tests/mesh_mock.h).  Outputs
+are stored as .npz fixtures next to this script: inputs and the reference's
+outputs, nothing else.
+"""
+from __future__ import annotations
+
+import math
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from xdg_amd import geometry as G  # noqa: E402
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+INFTY = np.finfo(np.float64).max
+
+
+def run(mode, arr, width_out):
+    arr = np.ascontiguousarray(arr, dtype=np.float64)
+    with tempfile.TemporaryDirectory() as td:
+        fi = os.path.join(td, "in.bin")
+        fo = os.path.join(td, "out.bin")
+        arr.tofile(fi)
+        subprocess.run([DRIVER, mode, fi, fo], check=True)
+        out = np.fromfile(fo, dtype=np.float64)
+    return out.reshape(-1, width_out)
+
+
+def unit(v):
+    n = np.sqrt(v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1] + v[..., 2] * v[..., 2])
+    return v / n[..., None]
+
+
+def plucker_cases(rng):
+    cases = []
+
+    def add(tri, o, d, tmax=INFTY, tmin=0.0, use=0, ori=0):
+        cases.append(np.concatenate([np.ravel(tri), o, d,
+                                     [tmax, tmin, float(use), float(ori)]]))
+
+    # 1. random rays aimed inside random triangles (hits), plus misses
+    for _ in range(1500):
+        tri = rng.uniform(-10, 10, (3, 3))
+        o = rng.uniform(-20, 20, 3)
+        b = rng.dirichlet([1, 1, 1])
+        target = b @ tri
+        d = unit(target - o)
+        use = int(rng.integers(0, 2))
+        ori = int(rng.choice([-1, 1])) if use else 0
+        add(tri, o, d, use=use, ori=ori)
+        add(tri, o, unit(rng.normal(size=3)), use=use, ori=ori)
+    # 2. rays through edge points and vertices (zero-snap, watertight ties)
+    for _ in range(800):
+        tri = rng.uniform(-5, 5, (3, 3))
+        o = rng.uniform(-10, 10, 3)
+        a, b = rng.choice(3, 2, replace=False)
+        lam = rng.choice([0.5, rng.uniform(0, 1)])
+        target = tri[a] + lam * (tri[b] - tri[a])
+        add(tri, o, target - o)
+        add(tri, o, tri[a] - o)
+        # the neighbouring triangle across edge (a, b)
+        c = 3 - a - b
+        other = tri[a] + tri[b] - tri[c]
+        add(np.stack([tri[b], tri[a], other]), o, target - o)
+    # 3. integer grids and axis-aligned rays (exact arithmetic, MeshMock-like)
+    for _ in range(400):
+        tri = rng.integers(-6, 7, (3, 3)).astype(float)
+        o = rng.integers(-3, 4, 3).astype(float)
+        ax = rng.integers(0, 3)
+        d = np.zeros(3)
+        d[ax] = rng.choice([-1.0, 1.0])
+        add(tri, o, d)
+        add(tri, o, d, use=1, ori=1)
+    # 4. coplanar and grazing rays
+    for _ in range(300):
+        tri = rng.uniform(-5, 5, (3, 3))
+        e1, e2 = tri[1] - tri[0], tri[2] - tri[0]
+        o = tri[0] + rng.uniform(-2, 2) * e1 + rng.uniform(-2, 2) * e2
+        d = rng.uniform(-1, 1) * e1 + rng.uniform(-1, 1) * e2
+        add(tri, o, d)
+        n = np.cross(e1, e2)
+        add(tri, o + 1e-3 * unit(n), d + 1e-9 * n)
+    # 5. tiny and huge scales
+    for scale in (1e-8, 1e-3, 1e3, 1e8):
+        for _ in range(100):
+            tri = rng.uniform(-1, 1, (3, 3)) * scale
+            o = rng.uniform(-2, 2, 3) * scale
+            b = rng.dirichlet([1, 1, 1])
+            add(tri, o, unit(b @ tri - o))
+    # 6. distance limits: tmax exactly at, just below and above t
+    for _ in range(300):
+        tri = rng.uniform(-10, 10, (3, 3))
+        o = rng.uniform(-20, 20, 3)
+        d = unit(rng.dirichlet([1, 1, 1]) @ tri - o)
+        cases.append(np.concatenate([np.ravel(tri), o, d, [INFTY, 0, 0, 0]]))
+    return np.stack(cases)
+
+
+def closest_cases(rng):
+    cases = []
+    ref_tri = np.array([[-1.0, -1.0, 0.0], [2.0, 1.0, 0.0], [-2.0, 3.0, 0.0]])
+    ref_pts = [[0, 0, 0], [3, 3, 0], [-2.5, 3.5, 0], [-2.5, -1.5, 0],
+               [-2, -2, 0], [1, -0.5, 0], [10, 0, 0], [3, 1, 0], [3, 1, 10],
+               [-1, -1.5, 0], [-1, -1.5, 1]]
+    for p in ref_pts:
+        cases.append(np.concatenate([ref_tri.ravel(), p]))
+    for v in ref_tri:
+        cases.append(np.concatenate([ref_tri.ravel(), v]))
+    for _ in range(3000):
+        tri = rng.uniform(-5, 5, (3, 3))
+        p = rng.uniform(-12, 12, 3)
+        cases.append(np.concatenate([tri.ravel(), p]))
+    for _ in range(500):
+        tri = rng.integers(-4, 5, (3, 3)).astype(float)
+        p = rng.integers(-6, 7, 3).astype(float)
+        cases.append(np.concatenate([tri.ravel(), p]))
+    return np.stack(cases)
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"],
+                       check=True)
+    rng = np.random.default_rng(20240521)
+
+    pc = plucker_cases(rng)
+    po = run("plucker", pc, 2)
+    # also pin the edge test on the first edges of the plucker cases
+    ec = []
+    for c in pc[:2000]:
+        tri = c[:9].reshape(3, 3)
+        o, d = c[9:12], c[12:15]
+        rn = np.array([d[1] * o[2] - d[2] * o[1], d[2] * o[0] - d[0] * o[2],
+                       d[0] * o[1] - d[1] * o[0]])
+        ec.append(np.concatenate([tri[0], tri[1], d, rn]))
+        ec.append(np.concatenate([tri[1], tri[0], d, rn]))
+    ec = np.stack(ec)
+    eo = run("edge", ec, 1)
+
+    nc = np.concatenate([rng.uniform(-10, 10, (2000, 9)),
+                         rng.uniform(-1e-20, 1e-20, (50, 9)),
+                         rng.uniform(-1e20, 1e20, (50, 9))])
+    no = run("normal", nc, 3)
+
+    cc = closest_cases(rng)
+    co = run("closest", cc, 4)
+
+    np.savez_compressed(os.path.join(HERE, "golden_primitives.npz"),
+                        plucker_in=pc, plucker_out=po, edge_in=ec,
+                        edge_out=eo[:, 0], normal_in=nc, normal_out=no,
+                        closest_in=cc, closest_out=co)
+
+    # ---- MeshMock element walk -------------------------------------------
+    verts, tris, tets, adj = G.mesh_mock_arrays()
+    fo = G.MOCK_TET_FACES
+    next_in = []
+    for e in range(12):
+        tv = verts[tets[e]]
+        for _ in range(80):
+            b = rng.dirichlet([1, 1, 1, 1])
+            r = b @ tv
+            u = unit(rng.normal(size=3))
+            next_in.append(np.concatenate([[e], r, u]))
+        # on-face and on-vertex starts
+        for f in range(4):
+            fv = tv[fo[f]]
+            r = rng.dirichlet([1, 1, 1]) @ fv
+            next_in.append(np.concatenate([[e], r, unit(rng.normal(size=3))]))
+        next_in.append(np.concatenate([[e], tv[0], unit(rng.normal(size=3))]))
+    next_in = np.stack(next_in)
+    next_out = run("mock_next", next_in, 2)
+
+    tracks = run("mock_tracks", np.array([1000.0]), 6)
+    np.savez_compressed(os.path.join(HERE, "golden_mock_tracks.npz"),
+                        tracks=tracks)
+    face_ids = np.arange(12, dtype=np.float64)
+    fnorm = run("mock_face_normal", face_ids, 3)
+
+    # walk inputs: (start elem, start, u = normalize(end - start), distance)
+    # with the start element from the oracle's find_element (an input here)
+    import pyoracle as O  # test-infrastructure oracle
+    tm = O.TetMesh(G.mock_tetmesh())
+    starts = tracks[:, :3]
+    ends = tracks[:, 3:]
+    elems = tm.find_element(starts)
+    walk_in = []
+    for s, e, el in zip(starts, ends, elems):
+        u = e - s
+        dist = math.sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2])
+        ln = dist if dist >= G.MIN_RCP_INPUT else G.MIN_RCP_INPUT
+        u = np.array([u[0] / ln, u[1] / ln, u[2] / ln])
+        walk_in.append(np.concatenate([[el], s, u, [dist]]))
+    # the upper-right-corner walk of tests/test_tally_segments.cpp:27-36
+    r = np.array([-1.0, -2.0, -3.0])
+    ur = np.array([5.0, 6.0, 7.0])
+    u = ur - r
+    dist = math.sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2])
+    u = np.array([u[0] / dist, u[1] / dist, u[2] / dist])
+    walk_in.append(np.concatenate([[tm.find_element([r])[0]], r, u, [dist]]))
+    walk_in = np.stack(walk_in)
+    walk_out = run("mock_walk", walk_in, 129)
+
+    np.savez_compressed(os.path.join(HERE, "golden_mock_walk.npz"),
+                        next_in=next_in, next_out=next_out, walk_in=walk_in,
+                        walk_out=walk_out, face_normal=fnorm)
+    print("plucker", pc.shape, "hits", int(po[:, 0].sum()))
+    print("closest", cc.shape, "regions", np.bincount(co[:, 0].astype(int)))
+    print("mock next", next_in.shape, "walk", walk_in.shape,
+          "max segs", int(walk_out[:, 0].max()))
+
+
+if __name__ == "__main__":
+    main()
