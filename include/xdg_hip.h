@@ -1,0 +1,197 @@
+/*
+ * xdg_hip.h -- C-ABI of the MI355X geometry-query backend (libxdg_hip.so).
+ *
+ * This is the drop-in boundary beside XDG's Embree and GPRT backends.  Each
+ * entry point replaces one method of the reference's abstract RayTracer
+ * (include/xdg/ray_tracing_interface.h:16-147) or the facade callers on the
+ * hot path (src/xdg.cpp), batched: one call processes n queries.  A C++
+ * RayTracer-shaped class over this ABI is in include/xdg_hip/hip_ray_tracer.h.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  "host" arrays are read during the call;
+ *     "device" arrays are device pointers (hipMalloc'd or any allocator on
+ *     the same device), read/written asynchronously on `stream`
+ *     (NULL = the context's stream).  Results are ready after
+ *     xdg_hip_synchronize(ctx, stream).
+ *   - Every function returns an int status: XDG_HIP_OK (0) or a negative
+ *     code; xdg_hip_last_error(ctx) describes the last failure.  No C++
+ *     exception crosses the ABI.
+ *   - Calls on one context from several host threads are serialized.
+ *   - Types follow include/xdg/constants.h: ids are int32, ID_NONE = -1,
+ *     INFTY = DBL_MAX.  Tree ids start at 1 (ray_tracing_interface.cpp:10-18).
+ *   - Tie rule (the only intended difference from Embree's visit-order
+ *     dependence, triangle_intersect.cpp:78): among bitwise-equal distances
+ *     the smallest face id wins; find_element returns the lowest containing
+ *     element index.
+ */
+#ifndef XDG_HIP_H
+#define XDG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XDG_HIP_OK 0
+#define XDG_HIP_ERR_INVALID (-1)     /* bad argument */
+#define XDG_HIP_ERR_NOT_FOUND (-2)   /* unknown tree id (std::map::at throws in the reference) */
+#define XDG_HIP_ERR_HIP (-3)         /* HIP runtime error */
+#define XDG_HIP_ERR_NO_DEVICE (-4)   /* no usable GPU */
+#define XDG_HIP_ERR_CAPPED (-5)      /* a walk hit its step bound (results kept) */
+
+/* precision of box traversal */
+#define XDG_HIP_PLAIN 0 /* FP64 boxes + FP64 Pluecker */
+#define XDG_HIP_MIXED 1 /* outward-rounded FP32 boxes + FP64 Pluecker */
+
+/* HitOrientation (include/xdg/shared_enums.h:11-15) */
+#define XDG_HIP_ANY (-1)
+#define XDG_HIP_EXITING 0
+#define XDG_HIP_ENTERING 1
+
+#define XDG_HIP_MAX_EXCLUDE 8 /* exclusion slots per ray */
+
+typedef struct xdg_hip_ctx xdg_hip_ctx;
+
+typedef struct {
+  int64_t n_prims;   /* triangles or elements */
+  int64_t n_nodes;   /* BVH2 node slots */
+  int32_t depth;     /* max root-to-leaf depth */
+  int32_t root;      /* child-encoded root */
+  int64_t bytes;     /* device bytes held by the tree */
+  double bump;       /* box dilation used */
+  double o_limit;    /* mixed traversal is exact for |origin|_inf <= o_limit */
+} xdg_hip_tree_info;
+
+/* ---- lifecycle --------------------------------------------------------- */
+
+/* Replaces constructing EmbreeRayTracer (src/embree/ray_tracer.cpp:15-24)
+ * and RayTracer::init (ray_tracing_interface.h:22). */
+int xdg_hip_create(xdg_hip_ctx** ctx, int device);
+int xdg_hip_destroy(xdg_hip_ctx* ctx);
+const char* xdg_hip_last_error(const xdg_hip_ctx* ctx);
+/* Launch shape for the traversal kernels (block: 64..1024, multiple of 64;
+ * stack: LDS entries per lane, >= the deepest tree, <= 64). */
+int xdg_hip_set_launch(xdg_hip_ctx* ctx, int block, int stack);
+int xdg_hip_get_launch(xdg_hip_ctx* ctx, int* block, int* stack);
+int xdg_hip_synchronize(xdg_hip_ctx* ctx, void* stream);
+
+/* ---- tree construction (host arrays in, device LBVH out) ---------------- */
+
+/* RayTracer::create_surface_tree (ray_tracing_interface.h:51; Embree:
+ * ray_tracer.cpp:50-105).  One tree = all triangles of one volume, in the
+ * caller's order: tris[3*i..] index verts; face_ids[i] is the mesh face id
+ * returned by queries; surf_ids[i] the surface id; flip[i] = 1 when this
+ * volume is the surface's reverse-sense parent (triangle_intersect.cpp:84).
+ * bump: box dilation, normally max(1e-6 * volume box chord, 1e-3)
+ * (ray_tracing_interface.cpp:20-24). */
+int xdg_hip_build_surface_tree(xdg_hip_ctx* ctx, int64_t n_verts, const double* verts, int64_t n_tris,
+                               const int32_t* tris, const int32_t* face_ids, const int32_t* surf_ids,
+                               const uint8_t* flip, double bump, int32_t* tree_out);
+
+/* RayTracer::create_element_tree (ray_tracing_interface.h:63; Embree:
+ * ray_tracer.cpp:150-188) for tetrahedra.  tets: 4 vertex indices per
+ * element (element id = index); adj: neighbour across face f (-1 boundary);
+ * face_order: 12 ints, the local vertex triples of faces 0..3 (MOAB
+ * canonical {0,1,3},{1,2,3},{2,0,3},{0,2,1}, include/xdg/moab/
+ * direct_access.h:181-183). */
+int xdg_hip_build_element_tree(xdg_hip_ctx* ctx, int64_t n_verts, const double* verts, int64_t n_tets,
+                               const int32_t* tets, const int32_t* adj, const int32_t* face_order,
+                               int32_t* tree_out);
+
+/* Attach the implicit-complement surface tree and the boundary face ->
+ * element map (MeshManager::get_boundary_face_element) used by the
+ * re-entry branch of segments (src/xdg.cpp:164-185).  bface_elem is
+ * indexed by the face ids of the surface tree. */
+int xdg_hip_attach_boundary(xdg_hip_ctx* ctx, int32_t element_tree, int32_t surface_tree, int64_t n_faces,
+                            const int32_t* bface_elem);
+
+int xdg_hip_get_tree_info(xdg_hip_ctx* ctx, int32_t tree, int element, xdg_hip_tree_info* info);
+int xdg_hip_num_trees(xdg_hip_ctx* ctx, int32_t* surface_trees, int32_t* element_trees);
+
+/* ---- queries (device arrays) -------------------------------------------- */
+
+/* RayTracer::ray_fire (ray_tracing_interface.h:81-86; Embree
+ * ray_tracer.cpp:290-326).  tree_per_ray (device, nullable) overrides
+ * `tree` per ray.  tmax (device, nullable): dist_limit per ray (INFTY when
+ * NULL).  excl (device, nullable): k <= 8 face ids per ray (-1 = empty).
+ * Miss: t = INFTY, surf = face = -1.  The reference appends the hit face to
+ * the caller's exclusion vector: face_out carries it. */
+int xdg_hip_ray_fire(xdg_hip_ctx* ctx, int32_t tree, const int32_t* tree_per_ray, int64_t n, const double* org,
+                     const double* dir, const double* tmax, int orientation, const int32_t* excl, int k,
+                     int precision, double* t_out, int32_t* surf_out, int32_t* face_out, void* stream);
+
+/* RayTracer::point_in_volume (ray_tracing_interface.h:76-79; Embree
+ * ray_tracer.cpp:259-287).  dir NULL -> (1/sqrt 2, 1/sqrt 2, 0). */
+int xdg_hip_point_in_volume(xdg_hip_ctx* ctx, int32_t tree, int64_t n, const double* pts, const double* dir,
+                            const int32_t* excl, int k, int precision, uint8_t* inside_out, void* stream);
+
+/* XDG::find_volume (src/xdg.cpp:113-128): the first volume in the given
+ * order (skipping ipc_volume) whose point_in_volume is true, else
+ * ipc_volume.  vol_trees/vol_ids are host arrays of nvol entries. */
+int xdg_hip_find_volume(xdg_hip_ctx* ctx, int nvol, const int32_t* vol_trees, const int32_t* vol_ids,
+                        int32_t ipc_volume, int64_t n, const double* pts, const double* dir, int precision,
+                        int32_t* vol_out, void* stream);
+
+/* RayTracer::occluded (ray_tracing_interface.h:112-115; Embree
+ * ray_tracer.cpp:347-370).  dist_out (nullable): -INFTY on hit, INFTY on
+ * miss, as the reference's `dist` argument. */
+int xdg_hip_occluded(xdg_hip_ctx* ctx, int32_t tree, int64_t n, const double* org, const double* dir,
+                     int precision, uint8_t* hit_out, double* dist_out, void* stream);
+
+/* RayTracer::closest (ray_tracing_interface.h:109-110; Embree
+ * ray_tracer.cpp:328-345): distance and the nearest FACE id. */
+int xdg_hip_closest(xdg_hip_ctx* ctx, int32_t tree, int64_t n, const double* pts, double* dist_out,
+                    int32_t* face_out, void* stream);
+
+/* RayTracer::find_element(tree, point) (ray_tracing_interface.h:98-107;
+ * Embree ray_tracer.cpp:226-257). */
+int xdg_hip_find_element(xdg_hip_ctx* ctx, int32_t element_tree, int64_t n, const double* pts, int32_t* elem_out,
+                         void* stream);
+
+/* MeshManager::next_element (mesh_manager_interface.cpp:150-200), batched:
+ * no exit face -> next = -1, dist = INFTY (the reference indexes face -1). */
+int xdg_hip_next_element(xdg_hip_ctx* ctx, int32_t element_tree, int64_t n, const int32_t* elem, const double* r,
+                         const double* u, int32_t* next_out, double* dist_out, void* stream);
+
+/* XDG::segments(start, end) (src/xdg.cpp:142-198) over the element tree,
+ * fused with the per-element track-length tally of tally_segments
+ * (tools/tally_segments.h:51-83).  Per segment: count of (element, length)
+ * entries (zero-length entries kept, mesh_manager_interface.cpp:125) and a
+ * 64-bit sequence hash (h = 0x6a09e667f3bcc909; per entry
+ * h = mix(h ^ elem); h = mix(h ^ bits(len)); mix = splitmix64 finaliser).
+ * tally_fx (device, nullable, one per element): += llrint(len * 2^40).
+ * seq_off (device, nullable, n+1 prefix offsets) with seq_elem/seq_len:
+ * explicit sequences.  Returns XDG_HIP_ERR_CAPPED after synchronizing if a
+ * walk hit its step bound. */
+int xdg_hip_tally_segments(xdg_hip_ctx* ctx, int32_t element_tree, int64_t n, const double* p0, const double* p1,
+                           int64_t* tally_fx, int32_t* count_out, uint64_t* hash_out, const int64_t* seq_off,
+                           int32_t* seq_elem, double* seq_len, void* stream);
+
+/* Synchronize the device, return and clear the status word (bit 0: a walk of
+ * xdg_hip_tally_segments hit its step bound since the last call). */
+int xdg_hip_status(xdg_hip_ctx* ctx, uint32_t* flags);
+
+/* Evaluate one device geometry primitive per item, for parity checks
+ * against the reference's own functions (device in/out arrays):
+ *   0 plucker_ray_tri_intersect (plucker.h:64-161): 19 doubles
+ *     (v0 v1 v2 o d tmax tmin use_orientation orientation) -> (hit, t)
+ *   1 face_normal (mesh_manager_interface.cpp:216-220): 9 -> 3
+ *   2 closest_location_on_triangle (closest.cpp:54-183): 12 -> 3
+ *   3 plucker_tet_containment_test (tetrahedron_contain.cpp:11-43):
+ *     15 (p v0 v1 v2 v3) -> 1
+ *   4 sqrt: 1 -> 1;  5 division a/b: 2 -> 1 */
+int xdg_hip_eval_primitive(xdg_hip_ctx* ctx, int kind, int64_t n, const double* in, double* out, void* stream);
+
+/* ---- device memory helpers for callers without their own allocator ----- */
+int xdg_hip_malloc(xdg_hip_ctx* ctx, size_t bytes, void** ptr);
+int xdg_hip_free(xdg_hip_ctx* ctx, void* ptr);
+int xdg_hip_memcpy_htod(xdg_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int xdg_hip_memcpy_dtoh(xdg_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XDG_HIP_H */

@@ -1,0 +1,346 @@
+"""HIP kernels vs the CPU oracle and the reference's golden vectors.
+
+Bit-exact bar: hit distances compared as uint64 bit patterns, ids and flags
+exactly.  Every case calls through the C-ABI (libxdg_hip.so).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from xdg_amd import geometry as G
+from xdg_amd import philox as P
+
+pytestmark = pytest.mark.gpu
+
+INFTY = np.finfo(np.float64).max
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(torch_mod):
+    from xdg_amd import capi
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def dev(torch, a, dtype=None):
+    a = np.ascontiguousarray(a)
+    t = torch.from_numpy(a)
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to("cuda").contiguous()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def build_tree(ctx, model, vol):
+    v9, f, s, fl = model.tree_arrays(vol)
+    n = v9.shape[0]
+    verts = v9.reshape(-1, 3)
+    tris = np.arange(3 * n, dtype=np.int32).reshape(-1, 3)
+    return ctx.build_surface_tree(verts, tris, f, s, fl, model.bump(vol))
+
+
+# ---------------------------------------------------------------------------
+# primitives vs the reference's compiled functions
+# ---------------------------------------------------------------------------
+
+def test_primitives_golden(ctx, golden_dir):
+    g = np.load(os.path.join(golden_dir, "golden_primitives.npz"))
+    out = ctx.eval_primitive(0, g["plucker_in"], 2)
+    ref = g["plucker_out"]
+    assert np.array_equal(out[:, 0], ref[:, 0])
+    assert np.array_equal(bits(out[:, 1]), bits(ref[:, 1]))
+    out = ctx.eval_primitive(1, g["normal_in"], 3)
+    assert np.array_equal(bits(out), bits(g["normal_out"]))
+    out = ctx.eval_primitive(2, g["closest_in"], 3)
+    assert np.array_equal(bits(out), bits(g["closest_out"][:, 1:]))
+
+
+def test_sqrt_div_correctly_rounded(ctx):
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(0, 1, 200000),
+                        np.exp(rng.uniform(-700, 700, 200000)),
+                        rng.integers(1, 1 << 40, 100000).astype(np.float64),
+                        [0.0, 1.0, 2.0, 4.0, 1e-310, 5e-324, INFTY]])
+    out = ctx.eval_primitive(4, x.reshape(-1, 1), 1)[:, 0]
+    assert np.array_equal(bits(out), bits(np.sqrt(x)))
+    a = np.exp(rng.uniform(-300, 300, 300000)) * rng.choice([-1, 1], 300000)
+    b = np.exp(rng.uniform(-300, 300, 300000)) * rng.choice([-1, 1], 300000)
+    out = ctx.eval_primitive(5, np.stack([a, b], 1), 1)[:, 0]
+    assert np.array_equal(bits(out), bits(a / b))
+
+
+def test_tet_containment_reference_cases(ctx):
+    # tests/test_tet_containment.cpp:14-95
+    t1 = [0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1]
+    t2 = [1, 1, 1, 2, 1, 1, 1, 2, 1, 1, 1, 2]
+    cases = [([0.1, 0.1, 0.1], t1, 1), ([2, 2, 2], t1, 0), ([0, 0, 0.5], t1, 1),
+             ([-0.5, -0.5, 0], t1, 0), ([0.3, 0.3, 0.1], t1, 1),
+             ([0.3, 0.1, 0.3], t1, 1), ([0.1, 0.3, 0.3], t1, 1),
+             ([0.3, 0.3, 0.3], t1, 1), ([0.3, 0.3, -0.1], t1, 0),
+             ([0.3, -0.1, 0.3], t1, 0), ([-0.1, 0.3, 0.3], t1, 0),
+             ([0.4, 0.4, 0.4], t1, 0), ([1.2, 1.2, 1.2], t2, 1),
+             ([0, 0, 0], t2, 0), ([1, 1, 1.5], t2, 1), ([1.3, 1.3, 1.1], t2, 1),
+             ([1.3, 1.3, 0.9], t2, 0), ([0.1, 0.1, 0.1], t2, 0),
+             ([1.2, 1.2, 1.2], t1, 0)]
+    inp = np.array([p + t for p, t, _ in cases], dtype=np.float64)
+    out = ctx.eval_primitive(3, inp, 1)[:, 0]
+    assert list(out.astype(int)) == [e for _, _, e in cases]
+    # and the oracle agrees case by case
+    for (p, t, e) in cases:
+        assert O.tet_contains(p, t)[0] == bool(e)
+
+
+# ---------------------------------------------------------------------------
+# MeshMock known answers (tests/test_ray_fire.cpp, test_point_in_volume.cpp,
+# test_occluded.cpp, test_closest.cpp, test_find_element.cpp)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_mock_ray_fire(ctx, torch_mod, prec):
+    torch = torch_mod
+    m = G.mesh_mock_model()
+    tree = build_tree(ctx, m, 0)
+    o = np.zeros((6, 3))
+    d = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1],
+                  [0, 0, -1]], float)
+    t, s, f = ctx.ray_fire(tree, dev(torch, o), dev(torch, d), precision=prec)
+    assert np.allclose(host(t), [5, 2, 6, 3, 7, 4], atol=1e-6)
+    t, s, f = ctx.ray_fire(tree, dev(torch, [[-10.0, 0, 0], [10.0, 0, 0]]),
+                           dev(torch, [[1.0, 0, 0], [-1.0, 0, 0]]),
+                           precision=prec)
+    assert np.allclose(host(t), [15, 12], atol=1e-6)
+    t, s, f = ctx.ray_fire(tree, dev(torch, [[-10.0, 0, 0], [10.0, 0, 0]]),
+                           dev(torch, [[1.0, 0, 0], [-1.0, 0, 0]]),
+                           orientation=1, precision=prec)
+    assert np.allclose(host(t), [8, 5], atol=1e-6)
+    t, s, f = ctx.ray_fire(tree, dev(torch, np.zeros((2, 3))),
+                           dev(torch, [[1.0, 0, 0], [1.0, 0, 0]]),
+                           tmax=dev(torch, [4.5, 5.1]), precision=prec)
+    assert host(s)[0] == -1 and host(s)[1] != -1
+    assert abs(host(t)[1] - 5.0) < 1e-6
+    t, s, f = ctx.ray_fire(tree, dev(torch, np.zeros((1, 3))),
+                           dev(torch, [[1.0, 0, 0]]), precision=prec)
+    ex = dev(torch, host(f).reshape(1, 1).astype(np.int32))
+    t2, s2, f2 = ctx.ray_fire(tree, dev(torch, np.zeros((1, 3))),
+                              dev(torch, [[1.0, 0, 0]]), excl=ex,
+                              precision=prec)
+    assert host(s2)[0] == -1
+
+
+def test_mock_piv_occluded_closest(ctx, torch_mod):
+    torch = torch_mod
+    m = G.mesh_mock_model()
+    tree = build_tree(ctx, m, 0)
+    pts = [[0, 0, 0], [0, 0, 1000], [4 - 1e-6, 0, 0], [5.001, 0, 0]]
+    r = host(ctx.point_in_volume(tree, dev(torch, np.array(pts, float))))
+    assert list(r) == [1, 0, 1, 0]
+    r = host(ctx.point_in_volume(tree, dev(torch, [[5.0, 0, 0], [5.1, 0, 0], [5.1, 0, 0]]),
+                                 dirs=dev(torch, [[1.0, 0, 0], [1.0, 0, 0], [-1.0, 0, 0]])))
+    assert list(r) == [1, 0, 0]
+    hit, dist = ctx.occluded(tree, dev(torch, [[-100.0, 0, 0]] * 2),
+                             dev(torch, [[1.0, 0, 0], [-1.0, 0, 0]]))
+    assert list(host(hit)) == [1, 0]
+    assert host(dist)[0] == -INFTY and host(dist)[1] == INFTY
+    d, f = ctx.closest(tree, dev(torch, [[0.0, 0, 0], [4.0, 0, 0], [10.0, 0, 0]]))
+    assert np.allclose(host(d), [2, 1, 5], atol=1e-6)
+    # test_closest.cpp:46-56: 1000 points along x through the box centre
+    rng = np.random.default_rng(3)
+    p = np.tile([1.5, 1.5, 1.5], (1000, 1))
+    p[:, 0] = rng.uniform(-10, 10, 1000)
+    d, f = ctx.closest(tree, dev(torch, p))
+    exp = np.where(p[:, 0] < 1.5, np.abs(p[:, 0] + 2.0), np.abs(p[:, 0] - 5.0))
+    assert np.allclose(host(d), exp, atol=1e-6)
+
+
+def test_mock_elements(ctx, torch_mod, golden_dir):
+    torch = torch_mod
+    tm = G.mock_tetmesh()
+    et = ctx.build_element_tree(tm.verts, tm.tets, tm.adj, tm.face_order)
+    e = host(ctx.find_element(et, dev(torch, [[0.0, 0, 0], [10.0, 10, 10]])))
+    assert e[0] != -1 and e[1] == -1
+    g = np.load(os.path.join(golden_dir, "golden_mock_walk.npz"))
+    cin, cout = g["next_in"], g["next_out"]
+    nx, d = ctx.next_element(et, dev(torch, cin[:, 0].astype(np.int32)),
+                             dev(torch, cin[:, 1:4]), dev(torch, cin[:, 4:7]))
+    nx, d = host(nx), host(d)
+    ok = cout[:, 1] != INFTY
+    assert np.array_equal(nx[ok], cout[ok, 0].astype(np.int32))
+    assert np.array_equal(bits(d), bits(cout[:, 1]))
+
+
+# ---------------------------------------------------------------------------
+# seeded synthetic workloads vs the oracle
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def sphere5(ctx):
+    m = G.single_sphere(5, 10.0)
+    return m, build_tree(ctx, m, 1), O.Tree.from_model(m, 1)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("orientation", [0, 1, -1])
+def test_sphere_ray_fire_bitexact(ctx, torch_mod, sphere5, prec, orientation):
+    torch = torch_mod
+    m, tree, ot = sphere5
+    n = 65536
+    org = P.ball_points(1, P.STREAM_POS, n, 9.0)
+    dirs = P.iso_dirs(1, P.STREAM_DIR, n)
+    if orientation == 1:  # entering hits need origins outside
+        org = org * 2.5
+    t, s, f = ctx.ray_fire(tree, dev(torch, org), dev(torch, dirs),
+                           orientation=orientation, precision=prec)
+    rt, rs, rf = ot.ray_fire(org, dirs, orientation=orientation)
+    assert np.array_equal(host(s), rs)
+    assert np.array_equal(host(f), rf)
+    assert np.array_equal(bits(host(t)), bits(rt))
+
+
+def test_sphere_limits_and_exclusions(ctx, torch_mod, sphere5):
+    torch = torch_mod
+    m, tree, ot = sphere5
+    n = 20000
+    org = P.ball_points(11, P.STREAM_POS, n, 9.0)
+    dirs = P.iso_dirs(11, P.STREAM_DIR, n)
+    rt, rs, rf = ot.ray_fire(org, dirs)
+    # dist_limit exactly at / just below the hit distance
+    tm = rt.copy()
+    tm[::2] = np.nextafter(rt[::2], 0)
+    excl = np.full((n, 2), -1, np.int32)
+    excl[::3, 0] = rf[::3]
+    for prec in (0, 1):
+        t, s, f = ctx.ray_fire(tree, dev(torch, org), dev(torch, dirs),
+                               tmax=dev(torch, tm), excl=dev(torch, excl),
+                               precision=prec)
+        et, es, ef = ot.ray_fire(org, dirs, tmax=tm, excl=excl)
+        assert np.array_equal(host(s), es)
+        assert np.array_equal(host(f), ef)
+        assert np.array_equal(bits(host(t)), bits(et))
+
+
+def test_sphere_occluded_closest_piv(ctx, torch_mod, sphere5):
+    torch = torch_mod
+    m, tree, ot = sphere5
+    n = 20000
+    org = P.box_points(12, P.STREAM_POS, n, [-15] * 3, [15] * 3)
+    dirs = P.iso_dirs(12, P.STREAM_DIR, n)
+    for prec in (0, 1):
+        hit, dist = ctx.occluded(tree, dev(torch, org), dev(torch, dirs),
+                                 precision=prec)
+        rh, rd = ot.occluded(org, dirs)
+        assert np.array_equal(host(hit).astype(bool), rh)
+        assert np.array_equal(bits(host(dist)), bits(rd))
+        inside = ctx.point_in_volume(tree, dev(torch, org), dev(torch, dirs),
+                                     precision=prec)
+        assert np.array_equal(host(inside).astype(bool),
+                              ot.point_in_volume(org, dirs))
+    d, f = ctx.closest(tree, dev(torch, org[:4096]))
+    rd, rf = ot.closest(org[:4096])
+    assert np.array_equal(host(f), rf)
+    assert np.array_equal(bits(host(d)), bits(rd))
+
+
+def test_nested_find_volume_and_fire(ctx, torch_mod):
+    torch = torch_mod
+    m = G.nested_spheres(4)
+    vols = [v for v in m.volumes]
+    trees = [build_tree(ctx, m, v) for v in vols]
+    otrees = [O.Tree.from_model(m, v) for v in vols]
+    n = 100000
+    pts = P.box_points(2, P.STREAM_POS, n, [-16] * 3, [16] * 3)
+    dirs = P.iso_dirs(2, P.STREAM_DIR, n)
+    ref_vol = O.find_volume(otrees, vols, m.ipc, pts, dirs)
+    # analytic classification away from the surfaces
+    r = np.sqrt((pts ** 2).sum(1))
+    far = (np.abs(r - 5) > 0.1) & (np.abs(r - 10) > 0.1) & (np.abs(r - 15) > 0.2)
+    analytic = np.select([r < 5, r < 10, r < 15], [1, 2, 3], 4)
+    assert np.array_equal(ref_vol[far], analytic[far])
+    for prec in (0, 1):
+        vol = host(ctx.find_volume(trees, vols, m.ipc, dev(torch, pts),
+                                   dev(torch, dirs), precision=prec))
+        assert np.array_equal(vol, ref_vol)
+        tpr = np.array([trees[vols.index(v)] for v in vol], np.int32)
+        t, s, f = ctx.ray_fire(0, dev(torch, pts), dev(torch, dirs),
+                               tree_per_ray=dev(torch, tpr), precision=prec)
+        t, s, f = host(t), host(s), host(f)
+        for v in vols:
+            sel = vol == v
+            if not sel.any():
+                continue
+            rt, rs, rf = otrees[vols.index(v)].ray_fire(pts[sel], dirs[sel])
+            assert np.array_equal(s[sel], rs)
+            assert np.array_equal(f[sel], rf)
+            assert np.array_equal(bits(t[sel]), bits(rt))
+
+
+def _segments_setup(ctx, mesh):
+    et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
+    bm = mesh.boundary_model()
+    st = build_tree(ctx, bm, bm.ipc)
+    ctx.attach_boundary(et, st, mesh.bface_elem)
+    return et
+
+
+def test_kuhn_find_element_and_segments(ctx, torch_mod):
+    torch = torch_mod
+    mesh = G.kuhn_cube(16, 0.2, seed=4)
+    et = _segments_setup(ctx, mesh)
+    om = O.TetMesh(mesh)
+    n = 20000
+    p0 = P.box_points(4, P.STREAM_POS, n, [0] * 3, [1] * 3)
+    p1 = P.box_points(4, P.STREAM_DIR, n, [0] * 3, [1] * 3)
+    e = host(ctx.find_element(et, dev(torch, p0)))
+    assert np.array_equal(e, om.find_element(p0))
+    tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device="cuda")
+    cnt, hsh = ctx.tally_segments(et, dev(torch, p0), dev(torch, p1), tally=tally)
+    rc, rh, rt = om.segments(p0, p1, tally=True)
+    assert np.array_equal(host(cnt), rc)
+    assert np.array_equal(host(hsh).view(np.uint64), rh)
+    assert np.array_equal(host(tally), rt)
+    assert ctx.status() == 0
+    # track length conservation (tests/test_tracks.cpp:48,121)
+    chord = np.sqrt(((p1 - p0) ** 2).sum(1))
+    total = host(tally).sum() / 2.0 ** 40
+    assert abs(total - chord.sum()) < 1e-6 * chord.sum()
+
+
+def test_mock_segments_sequences(ctx, torch_mod, golden_dir):
+    torch = torch_mod
+    mesh = G.mock_tetmesh()
+    et = _segments_setup(ctx, mesh)
+    om = O.TetMesh(mesh)
+    tracks = np.load(os.path.join(golden_dir, "golden_mock_tracks.npz"))["tracks"]
+    p0, p1 = tracks[:, :3], tracks[:, 3:]
+    cap = np.full(len(p0), 64, np.int64)
+    rc, rh, (off, se, sl) = om.segments(p0, p1, sequences=cap)
+    seq_off = dev(torch, off)
+    seq_e = torch.zeros(int(off[-1]), dtype=torch.int32, device="cuda")
+    seq_l = torch.zeros(int(off[-1]), dtype=torch.float64, device="cuda")
+    cnt, hsh = ctx.tally_segments(et, dev(torch, p0), dev(torch, p1),
+                                  seq_off=seq_off, seq_elem=seq_e, seq_len=seq_l)
+    assert np.array_equal(host(cnt), rc)
+    assert np.array_equal(host(hsh).view(np.uint64), rh)
+    ge, gl = host(seq_e), host(seq_l)
+    for i in range(len(p0)):
+        a, b = off[i], off[i] + rc[i]
+        assert np.array_equal(ge[a:b], se[a:b])
+        assert np.array_equal(bits(gl[a:b]), bits(sl[a:b]))

@@ -1,0 +1,327 @@
+"""ctypes binding of libxdg_hip.so (the C-ABI in include/xdg_hip.h).
+
+torch is imported first so that this process uses a single HIP runtime
+(torch's); tensors passed to the query functions must live on the context's
+device.  The library is built in-tree by __graft_entry__.build()
+(xdg_amd/csrc/Makefile) and is loaded from xdg_amd/lib; a missing library
+raises: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+import torch  # noqa: F401  (must precede loading libxdg_hip.so)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libxdg_hip.so")
+
+OK = 0
+ERR_INVALID = -1
+ERR_NOT_FOUND = -2
+ERR_HIP = -3
+ERR_NO_DEVICE = -4
+ERR_CAPPED = -5
+
+PLAIN = 0
+MIXED = 1
+ANY = -1
+EXITING = 0
+ENTERING = 1
+MAX_EXCLUDE = 8
+
+INFTY = np.finfo(np.float64).max
+
+_vp = C.c_void_p
+_i32 = C.c_int32
+_i64 = C.c_int64
+
+
+class XdgHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"xdg_hip error {code}: {msg}")
+        self.code = code
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [("n_prims", C.c_int64), ("n_nodes", C.c_int64),
+                ("depth", C.c_int32), ("root", C.c_int32),
+                ("bytes", C.c_int64), ("bump", C.c_double),
+                ("o_limit", C.c_double)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(make -C xdg_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "xdg_hip_create": [C.POINTER(_vp), C.c_int],
+            "xdg_hip_destroy": [_vp],
+            "xdg_hip_set_launch": [_vp, C.c_int, C.c_int],
+            "xdg_hip_get_launch": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+            "xdg_hip_synchronize": [_vp, _vp],
+            "xdg_hip_build_surface_tree": [_vp, _i64, _vp, _i64, _vp, _vp, _vp,
+                                           _vp, C.c_double, C.POINTER(_i32)],
+            "xdg_hip_build_element_tree": [_vp, _i64, _vp, _i64, _vp, _vp, _vp,
+                                           C.POINTER(_i32)],
+            "xdg_hip_attach_boundary": [_vp, _i32, _i32, _i64, _vp],
+            "xdg_hip_get_tree_info": [_vp, _i32, C.c_int, C.POINTER(TreeInfo)],
+            "xdg_hip_num_trees": [_vp, C.POINTER(_i32), C.POINTER(_i32)],
+            "xdg_hip_ray_fire": [_vp, _i32, _vp, _i64, _vp, _vp, _vp, C.c_int,
+                                 _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp],
+            "xdg_hip_point_in_volume": [_vp, _i32, _i64, _vp, _vp, _vp, C.c_int,
+                                        C.c_int, _vp, _vp],
+            "xdg_hip_find_volume": [_vp, C.c_int, _vp, _vp, _i32, _i64, _vp,
+                                    _vp, C.c_int, _vp, _vp],
+            "xdg_hip_occluded": [_vp, _i32, _i64, _vp, _vp, C.c_int, _vp, _vp,
+                                 _vp],
+            "xdg_hip_closest": [_vp, _i32, _i64, _vp, _vp, _vp, _vp],
+            "xdg_hip_find_element": [_vp, _i32, _i64, _vp, _vp, _vp],
+            "xdg_hip_next_element": [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp,
+                                     _vp],
+            "xdg_hip_tally_segments": [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp,
+                                       _vp, _vp, _vp, _vp],
+            "xdg_hip_status": [_vp, C.POINTER(C.c_uint32)],
+            "xdg_hip_eval_primitive": [_vp, C.c_int, _i64, _vp, _vp, _vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.xdg_hip_last_error.argtypes = [_vp]
+        L.xdg_hip_last_error.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _ptr(x):
+    """Device pointer of a torch tensor (or None)."""
+    if x is None:
+        return None
+    if isinstance(x, torch.Tensor):
+        if not x.is_cuda:
+            raise ValueError("query arrays must be device tensors")
+        if not x.is_contiguous():
+            raise ValueError("query arrays must be contiguous")
+        return _vp(x.data_ptr())
+    raise TypeError(f"expected a torch tensor, got {type(x)}")
+
+
+def _ordered(fn):
+    """Without an explicit stream a query runs on the context's stream:
+    first let torch's current stream finish producing the inputs."""
+    def wrapper(self, *args, **kwargs):
+        if kwargs.get("stream") is None:
+            torch.cuda.current_stream(self.dev).synchronize()
+        return fn(self, *args, **kwargs)
+    wrapper.__name__ = fn.__name__
+    wrapper.__doc__ = fn.__doc__
+    return wrapper
+
+
+def _host(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data_as(_vp)
+
+
+class Context:
+    """Owns one xdg_hip context on one device."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+        h = _vp()
+        rc = lib().xdg_hip_create(C.byref(h), device)
+        if rc != OK:
+            raise XdgHipError(rc, "xdg_hip_create failed (no usable GPU?)")
+        self.h = h
+        self.dev = torch.device("cuda", device)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().xdg_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            msg = lib().xdg_hip_last_error(self.h)
+            raise XdgHipError(rc, msg.decode() if msg else "")
+        return rc
+
+    def _done(self, rc, stream):
+        """Check a query's status; without an explicit stream the call ran on
+        the context's own stream, so wait for it before torch reads."""
+        self._check(rc)
+        if stream is None:
+            self._check(lib().xdg_hip_synchronize(self.h, None))
+
+    # ---- configuration ---------------------------------------------------
+    def set_launch(self, block, stack):
+        self._check(lib().xdg_hip_set_launch(self.h, block, stack))
+
+    def get_launch(self):
+        b, s = C.c_int(), C.c_int()
+        self._check(lib().xdg_hip_get_launch(self.h, C.byref(b), C.byref(s)))
+        return b.value, s.value
+
+    def synchronize(self, stream=None):
+        self._check(lib().xdg_hip_synchronize(self.h, stream))
+
+    # ---- builds ------------------------------------------------------------
+    def build_surface_tree(self, verts, tris, face_ids, surf_ids, flip, bump):
+        v, vp = _host(verts, np.float64)
+        t, tp = _host(tris, np.int32)
+        f, fp = _host(face_ids, np.int32)
+        s, sp = _host(surf_ids, np.int32)
+        fl, flp = _host(flip, np.uint8)
+        out = _i32()
+        self._check(lib().xdg_hip_build_surface_tree(
+            self.h, v.size // 3, vp, t.size // 3, tp, fp, sp, flp, float(bump),
+            C.byref(out)))
+        return out.value
+
+    def build_element_tree(self, verts, tets, adj, face_order):
+        v, vp = _host(verts, np.float64)
+        t, tp = _host(tets, np.int32)
+        a, ap = _host(adj, np.int32)
+        fo, fop = _host(face_order, np.int32)
+        out = _i32()
+        self._check(lib().xdg_hip_build_element_tree(
+            self.h, v.size // 3, vp, t.size // 4, tp, ap, fop, C.byref(out)))
+        return out.value
+
+    def attach_boundary(self, element_tree, surface_tree, bface_elem):
+        b, bp = _host(bface_elem, np.int32)
+        self._check(lib().xdg_hip_attach_boundary(self.h, element_tree,
+                                                  surface_tree, b.size, bp))
+
+    def tree_info(self, tree, element=False):
+        info = TreeInfo()
+        self._check(lib().xdg_hip_get_tree_info(self.h, tree, int(element),
+                                                C.byref(info)))
+        return {k: getattr(info, k) for k, _ in TreeInfo._fields_}
+
+    def num_trees(self):
+        s, e = _i32(), _i32()
+        self._check(lib().xdg_hip_num_trees(self.h, C.byref(s), C.byref(e)))
+        return s.value, e.value
+
+    # ---- queries (device tensors) ------------------------------------------
+    @_ordered
+    def ray_fire(self, tree, org, dirs, tmax=None, orientation=EXITING,
+                 excl=None, precision=PLAIN, tree_per_ray=None, out=None,
+                 stream=None):
+        n = org.shape[0]
+        if out is None:
+            out = (torch.empty(n, dtype=torch.float64, device=org.device),
+                   torch.empty(n, dtype=torch.int32, device=org.device),
+                   torch.empty(n, dtype=torch.int32, device=org.device))
+        t, s, f = out
+        k = 0 if excl is None else excl.shape[1]
+        self._done(lib().xdg_hip_ray_fire(
+            self.h, tree, _ptr(tree_per_ray), n, _ptr(org), _ptr(dirs),
+            _ptr(tmax), orientation, _ptr(excl), k, precision, _ptr(t),
+            _ptr(s), _ptr(f), stream), stream)
+        return t, s, f
+
+    @_ordered
+    def point_in_volume(self, tree, pts, dirs=None, excl=None, precision=PLAIN,
+                        stream=None):
+        n = pts.shape[0]
+        out = torch.empty(n, dtype=torch.uint8, device=pts.device)
+        k = 0 if excl is None else excl.shape[1]
+        self._done(lib().xdg_hip_point_in_volume(
+            self.h, tree, n, _ptr(pts), _ptr(dirs), _ptr(excl), k, precision,
+            _ptr(out), stream), stream)
+        return out
+
+    @_ordered
+    def find_volume(self, vol_trees, vol_ids, ipc, pts, dirs=None,
+                    precision=PLAIN, out=None, stream=None):
+        n = pts.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device=pts.device)
+        vt, vtp = _host(vol_trees, np.int32)
+        vi, vip = _host(vol_ids, np.int32)
+        self._done(lib().xdg_hip_find_volume(
+            self.h, vt.size, vtp, vip, ipc, n, _ptr(pts), _ptr(dirs), precision,
+            _ptr(out), stream), stream)
+        return out
+
+    @_ordered
+    def occluded(self, tree, org, dirs, precision=PLAIN, stream=None):
+        n = org.shape[0]
+        hit = torch.empty(n, dtype=torch.uint8, device=org.device)
+        dist = torch.empty(n, dtype=torch.float64, device=org.device)
+        self._done(lib().xdg_hip_occluded(self.h, tree, n, _ptr(org),
+                                           _ptr(dirs), precision, _ptr(hit),
+                                           _ptr(dist), stream), stream)
+        return hit, dist
+
+    @_ordered
+    def closest(self, tree, pts, stream=None):
+        n = pts.shape[0]
+        dist = torch.empty(n, dtype=torch.float64, device=pts.device)
+        face = torch.empty(n, dtype=torch.int32, device=pts.device)
+        self._done(lib().xdg_hip_closest(self.h, tree, n, _ptr(pts),
+                                          _ptr(dist), _ptr(face), stream), stream)
+        return dist, face
+
+    @_ordered
+    def find_element(self, etree, pts, stream=None):
+        n = pts.shape[0]
+        out = torch.empty(n, dtype=torch.int32, device=pts.device)
+        self._done(lib().xdg_hip_find_element(self.h, etree, n, _ptr(pts),
+                                               _ptr(out), stream), stream)
+        return out
+
+    @_ordered
+    def next_element(self, etree, elem, r, u, stream=None):
+        n = elem.shape[0]
+        nx = torch.empty(n, dtype=torch.int32, device=r.device)
+        d = torch.empty(n, dtype=torch.float64, device=r.device)
+        self._done(lib().xdg_hip_next_element(self.h, etree, n, _ptr(elem),
+                                               _ptr(r), _ptr(u), _ptr(nx),
+                                               _ptr(d), stream), stream)
+        return nx, d
+
+    @_ordered
+    def tally_segments(self, etree, p0, p1, tally=None, seq_off=None,
+                       seq_elem=None, seq_len=None, out=None, stream=None):
+        n = p0.shape[0]
+        if out is None:
+            out = (torch.empty(n, dtype=torch.int32, device=p0.device),
+                   torch.empty(n, dtype=torch.int64, device=p0.device))
+        cnt, hsh = out
+        self._done(lib().xdg_hip_tally_segments(
+            self.h, etree, n, _ptr(p0), _ptr(p1), _ptr(tally), _ptr(cnt),
+            _ptr(hsh), _ptr(seq_off), _ptr(seq_elem), _ptr(seq_len), stream), stream)
+        return cnt, hsh
+
+    def status(self):
+        f = C.c_uint32()
+        self._check(lib().xdg_hip_status(self.h, C.byref(f)))
+        return f.value
+
+    def eval_primitive(self, kind, inputs, width_out, stream=None):
+        inp = torch.as_tensor(np.ascontiguousarray(inputs, dtype=np.float64),
+                              device=self.dev).contiguous()
+        n = inp.shape[0]
+        out = torch.zeros((n, width_out), dtype=torch.float64, device=self.dev)
+        self._check(lib().xdg_hip_eval_primitive(self.h, kind, n, _ptr(inp),
+                                                 _ptr(out), stream))
+        self.synchronize(stream)
+        return out.cpu().numpy()

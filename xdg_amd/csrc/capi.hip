@@ -1,0 +1,712 @@
+// capi.hip -- extern "C" boundary of libxdg_hip.so (declared in
+// include/xdg_hip.h): context, tree registry, argument validation, launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "xdg_hip.h"
+
+using namespace xdgh;
+using namespace xdgd;
+
+namespace {
+
+struct SurfTreeHost {
+  SurfTreeDev dev{};
+  std::vector<DevBuf> bufs;
+  int32_t min_face = 0, max_face = -1;
+  int64_t n_nodes = 0;
+  int64_t bytes = 0;
+};
+
+struct ElemTreeHost {
+  ElemTreeDev dev{};
+  std::vector<DevBuf> bufs;
+  int64_t n_nodes = 0;
+  int64_t bytes = 0;
+  int64_t n_bfaces = 0;
+};
+
+struct InvalidArg {
+  std::string msg;
+};
+struct NotFound {
+  std::string msg;
+};
+
+int env_int(const char* name, int dflt)
+{
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  return std::atoi(v);
+}
+
+int64_t bufs_bytes(const std::vector<DevBuf>& b)
+{
+  int64_t s = 0;
+  for (auto& x : b) s += (int64_t)x.bytes;
+  return s;
+}
+
+}  // namespace
+
+struct xdg_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::string err;
+  std::vector<SurfTreeHost> surf;  // tree id = index + 1
+  std::vector<ElemTreeHost> elem;
+  SurfTreeDev* d_surf = nullptr;
+  size_t d_surf_cap = 0;
+  ElemTreeDev* d_elem = nullptr;
+  size_t d_elem_cap = 0;
+  int32_t* d_small = nullptr;  // find_volume tree/id lists
+  size_t small_cap = 0;
+  std::vector<int32_t> small_cache;
+  uint32_t* d_status = nullptr;
+  LaunchCfg cfg;
+  int leaf_max = 4;
+  int max_surf_depth = 1;
+  int max_elem_depth = 1;
+};
+
+namespace {
+
+void sync_tables(xdg_hip_ctx* c)
+{
+  XDG_HIP_CHECK(hipDeviceSynchronize());
+  size_t ns = c->surf.size() + 1;
+  if (ns > c->d_surf_cap) {
+    if (c->d_surf) XDG_HIP_CHECK(hipFree(c->d_surf));
+    c->d_surf_cap = std::max<size_t>(ns * 2, 16);
+    XDG_HIP_CHECK(hipMalloc(&c->d_surf, c->d_surf_cap * sizeof(SurfTreeDev)));
+  }
+  std::vector<SurfTreeDev> hs(ns);
+  std::memset((void*)hs.data(), 0, ns * sizeof(SurfTreeDev));
+  for (size_t i = 0; i < c->surf.size(); ++i) hs[i + 1] = c->surf[i].dev;
+  XDG_HIP_CHECK(hipMemcpy(c->d_surf, hs.data(), ns * sizeof(SurfTreeDev), hipMemcpyHostToDevice));
+
+  size_t ne = c->elem.size() + 1;
+  if (ne > c->d_elem_cap) {
+    if (c->d_elem) XDG_HIP_CHECK(hipFree(c->d_elem));
+    c->d_elem_cap = std::max<size_t>(ne * 2, 16);
+    XDG_HIP_CHECK(hipMalloc(&c->d_elem, c->d_elem_cap * sizeof(ElemTreeDev)));
+  }
+  std::vector<ElemTreeDev> he(ne);
+  std::memset((void*)he.data(), 0, ne * sizeof(ElemTreeDev));
+  for (size_t i = 0; i < c->elem.size(); ++i) he[i + 1] = c->elem[i].dev;
+  XDG_HIP_CHECK(hipMemcpy(c->d_elem, he.data(), ne * sizeof(ElemTreeDev), hipMemcpyHostToDevice));
+}
+
+void require(bool ok, const char* msg)
+{
+  if (!ok) throw InvalidArg{msg};
+}
+
+const SurfTreeHost& surf_tree(xdg_hip_ctx* c, int32_t tree)
+{
+  if (tree < 1 || (size_t)tree > c->surf.size())
+    throw NotFound{"surface tree " + std::to_string(tree) + " is not registered"};
+  return c->surf[tree - 1];
+}
+
+const ElemTreeHost& elem_tree(xdg_hip_ctx* c, int32_t tree)
+{
+  if (tree < 1 || (size_t)tree > c->elem.size())
+    throw NotFound{"element tree " + std::to_string(tree) + " is not registered"};
+  return c->elem[tree - 1];
+}
+
+hipStream_t pick(xdg_hip_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+
+LaunchCfg cfg_for(xdg_hip_ctx* c, int depth)
+{
+  LaunchCfg k = c->cfg;
+  k.stack = std::max(k.stack, std::max(depth, 1));
+  if (k.stack > 64) throw InvalidArg{"tree deeper than the 64-entry traversal stack"};
+  // keep the LDS stack within 64 KiB per workgroup
+  while ((size_t)k.stack * k.block * 4 > 65536 && k.block > 64) k.block /= 2;
+  return k;
+}
+
+template <class F>
+int guarded(xdg_hip_ctx* c, F&& f)
+{
+  if (!c) return XDG_HIP_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  try {
+    XDG_HIP_CHECK(hipSetDevice(c->device));
+    return f();
+  } catch (const InvalidArg& e) {
+    c->err = e.msg;
+    return XDG_HIP_ERR_INVALID;
+  } catch (const NotFound& e) {
+    c->err = e.msg;
+    return XDG_HIP_ERR_NOT_FOUND;
+  } catch (const HipError& e) {
+    c->err = e.msg;
+    return XDG_HIP_ERR_HIP;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return XDG_HIP_ERR_HIP;
+  } catch (...) {
+    c->err = "unknown error";
+    return XDG_HIP_ERR_HIP;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int xdg_hip_create(xdg_hip_ctx** out, int device)
+{
+  if (!out) return XDG_HIP_ERR_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return XDG_HIP_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return XDG_HIP_ERR_NO_DEVICE;
+  auto* c = new xdg_hip_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->d_status, sizeof(uint32_t) * 4) != hipSuccess) {
+    delete c;
+    return XDG_HIP_ERR_HIP;
+  }
+  c->cfg.block = env_int("XDG_HIP_BLOCK", 256);
+  c->cfg.stack = env_int("XDG_HIP_STACK", 1);
+  c->leaf_max = env_int("XDG_HIP_LEAF", 4);
+  try {
+    sync_tables(c);
+  } catch (...) {
+    delete c;
+    return XDG_HIP_ERR_HIP;
+  }
+  *out = c;
+  return XDG_HIP_OK;
+}
+
+int xdg_hip_destroy(xdg_hip_ctx* c)
+{
+  if (!c) return XDG_HIP_OK;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    for (auto& t : c->surf) dev_free_all(t.bufs);
+    for (auto& t : c->elem) dev_free_all(t.bufs);
+    if (c->d_surf) (void)hipFree(c->d_surf);
+    if (c->d_elem) (void)hipFree(c->d_elem);
+    if (c->d_small) (void)hipFree(c->d_small);
+    if (c->d_status) (void)hipFree(c->d_status);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+  }
+  delete c;
+  return XDG_HIP_OK;
+}
+
+const char* xdg_hip_last_error(const xdg_hip_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int xdg_hip_set_launch(xdg_hip_ctx* c, int block, int stack)
+{
+  return guarded(c, [&]() {
+    require(block >= 64 && block <= 1024 && block % 64 == 0, "block must be a multiple of 64 in [64, 1024]");
+    require(stack >= 1 && stack <= 64, "stack must be in [1, 64]");
+    c->cfg.block = block;
+    c->cfg.stack = stack;
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_get_launch(xdg_hip_ctx* c, int* block, int* stack)
+{
+  return guarded(c, [&]() {
+    if (block) *block = c->cfg.block;
+    if (stack) *stack = c->cfg.stack;
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_synchronize(xdg_hip_ctx* c, void* stream)
+{
+  return guarded(c, [&]() {
+    XDG_HIP_CHECK(hipStreamSynchronize(pick(c, stream)));
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* verts, int64_t n_tris,
+                               const int32_t* tris, const int32_t* face_ids, const int32_t* surf_ids,
+                               const uint8_t* flip, double bump, int32_t* tree_out)
+{
+  return guarded(c, [&]() {
+    require(tree_out != nullptr, "tree_out is null");
+    require(n_tris >= 0 && n_verts >= 0, "negative size");
+    require(n_tris == 0 || (verts && tris && face_ids && surf_ids && flip), "null input array");
+    require(n_tris < (1LL << 28), "too many triangles for one tree");
+    require(std::isfinite(bump) && bump >= 0.0, "bump must be finite and >= 0");
+    std::vector<double> v9((size_t)n_tris * 9);
+    for (int64_t i = 0; i < n_tris; ++i) {
+      for (int j = 0; j < 3; ++j) {
+        int32_t vi = tris[3 * i + j];
+        require(vi >= 0 && vi < n_verts, "triangle vertex index out of range");
+        for (int k = 0; k < 3; ++k) v9[9 * i + 3 * j + k] = verts[3 * (int64_t)vi + k];
+      }
+      require(surf_ids[i] >= 0 && surf_ids[i] < (1 << 30), "surface id out of range [0, 2^30)");
+    }
+    int32_t min_face = 0, max_face = -1;
+    if (n_tris > 0) {
+      min_face = *std::min_element(face_ids, face_ids + n_tris);
+      max_face = *std::max_element(face_ids, face_ids + n_tris);
+    }
+    hipStream_t s = c->stream;
+    SurfTreeHost t;
+    std::vector<DevBuf> tmp;
+    try {
+      size_t nt = (size_t)std::max<int64_t>(n_tris, 1);
+      double* d_v9 = (double*)dev_alloc(nt * 9 * sizeof(double), tmp);
+      int32_t* d_face = (int32_t*)dev_alloc(nt * sizeof(int32_t), tmp);
+      int32_t* d_surf = (int32_t*)dev_alloc(nt * sizeof(int32_t), tmp);
+      uint8_t* d_flip = (uint8_t*)dev_alloc(nt, tmp);
+      double* d_box = (double*)dev_alloc(nt * 6 * sizeof(double), tmp);
+      if (n_tris > 0) {
+        XDG_HIP_CHECK(hipMemcpyAsync(d_v9, v9.data(), (size_t)n_tris * 9 * sizeof(double), hipMemcpyHostToDevice, s));
+        XDG_HIP_CHECK(hipMemcpyAsync(d_face, face_ids, (size_t)n_tris * 4, hipMemcpyHostToDevice, s));
+        XDG_HIP_CHECK(hipMemcpyAsync(d_surf, surf_ids, (size_t)n_tris * 4, hipMemcpyHostToDevice, s));
+        XDG_HIP_CHECK(hipMemcpyAsync(d_flip, flip, (size_t)n_tris, hipMemcpyHostToDevice, s));
+      }
+      launch_tri_boxes(d_v9, n_tris, bump, d_box, s);
+      LbvhOut lb;
+      lbvh_build(d_box, n_tris, c->leaf_max, s, t.bufs, lb);
+      TriRec* d_tris = (TriRec*)dev_alloc(nt * sizeof(TriRec), t.bufs);
+      launch_write_tris(d_v9, d_face, d_surf, d_flip, lb.order, n_tris, d_tris, s);
+      XDG_HIP_CHECK(hipGetLastError());
+      XDG_HIP_CHECK(hipStreamSynchronize(s));
+      t.dev.nodes_d = lb.nodes_d;
+      t.dev.nodes_f = lb.nodes_f;
+      t.dev.tris = d_tris;
+      t.dev.root = lb.root;
+      t.dev.n_prims = (int32_t)n_tris;
+      t.dev.depth = lb.depth;
+      t.dev.bump = bump;
+      double lim = bump * 1048576.0 - lb.max_abs;  // bump * 2^20 - max |coordinate|
+      t.dev.o_limit = lim > 0.0 ? (float)lim : 0.0f;
+      if ((double)t.dev.o_limit > lim) t.dev.o_limit = nextafterf(t.dev.o_limit, 0.0f);
+      t.n_nodes = lb.n_nodes;
+      t.bytes = bufs_bytes(t.bufs);
+      t.min_face = min_face;
+      t.max_face = max_face;
+    } catch (...) {
+      dev_free_all(tmp);
+      dev_free_all(t.bufs);
+      throw;
+    }
+    dev_free_all(tmp);
+    c->surf.push_back(std::move(t));
+    c->max_surf_depth = std::max(c->max_surf_depth, c->surf.back().dev.depth);
+    try {
+      sync_tables(c);
+    } catch (...) {
+      dev_free_all(c->surf.back().bufs);
+      c->surf.pop_back();
+      throw;
+    }
+    *tree_out = (int32_t)c->surf.size();
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_build_element_tree(xdg_hip_ctx* c, int64_t n_verts, const double* verts, int64_t n_tets,
+                               const int32_t* tets, const int32_t* adj, const int32_t* face_order,
+                               int32_t* tree_out)
+{
+  return guarded(c, [&]() {
+    require(tree_out != nullptr, "tree_out is null");
+    require(n_tets >= 0 && n_verts >= 0, "negative size");
+    require(n_tets == 0 || (verts && tets && adj && face_order), "null input array");
+    require(n_tets < (1LL << 28), "too many elements for one tree");
+    if (face_order) {
+      for (int f = 0; f < 12; ++f) require(face_order[f] >= 0 && face_order[f] < 4, "face_order entries must be 0..3");
+    }
+    std::vector<ElemRec> recs((size_t)n_tets);
+    for (int64_t e = 0; e < n_tets; ++e) {
+      ElemRec& r = recs[e];
+      for (int j = 0; j < 4; ++j) {
+        int32_t vi = tets[4 * e + j];
+        require(vi >= 0 && vi < n_verts, "tet vertex index out of range");
+        r.conn[j] = vi;
+        for (int k = 0; k < 3; ++k) r.v[3 * j + k] = verts[3 * (int64_t)vi + k];
+        int32_t a = adj[4 * e + j];
+        require(a >= -1 && a < n_tets, "adjacency out of range");
+        r.adj[j] = a;
+      }
+    }
+    hipStream_t s = c->stream;
+    ElemTreeHost t;
+    std::vector<DevBuf> tmp;
+    try {
+      size_t ne = (size_t)std::max<int64_t>(n_tets, 1);
+      ElemRec* d_elems = (ElemRec*)dev_alloc(ne * sizeof(ElemRec), t.bufs);
+      double* d_box = (double*)dev_alloc(ne * 6 * sizeof(double), tmp);
+      if (n_tets > 0)
+        XDG_HIP_CHECK(hipMemcpyAsync(d_elems, recs.data(), (size_t)n_tets * sizeof(ElemRec), hipMemcpyHostToDevice, s));
+      launch_elem_boxes(d_elems, n_tets, d_box, s);
+      LbvhOut lb;
+      lbvh_build(d_box, n_tets, c->leaf_max, s, t.bufs, lb);
+      XDG_HIP_CHECK(hipGetLastError());
+      XDG_HIP_CHECK(hipStreamSynchronize(s));
+      t.dev.nodes_d = lb.nodes_d;
+      t.dev.nodes_f = lb.nodes_f;
+      t.dev.order = lb.order;
+      t.dev.elems = d_elems;
+      t.dev.root = lb.root;
+      t.dev.n_prims = (int32_t)n_tets;
+      t.dev.depth = lb.depth;
+      for (int f = 0; f < 12; ++f) t.dev.face_order[f] = face_order ? face_order[f] : 0;
+      t.dev.ipc_tree = -1;
+      t.dev.bface_elem = nullptr;
+      t.n_nodes = lb.n_nodes;
+      t.bytes = bufs_bytes(t.bufs);
+    } catch (...) {
+      dev_free_all(tmp);
+      dev_free_all(t.bufs);
+      throw;
+    }
+    dev_free_all(tmp);
+    c->elem.push_back(std::move(t));
+    c->max_elem_depth = std::max(c->max_elem_depth, c->elem.back().dev.depth);
+    try {
+      sync_tables(c);
+    } catch (...) {
+      dev_free_all(c->elem.back().bufs);
+      c->elem.pop_back();
+      throw;
+    }
+    *tree_out = (int32_t)c->elem.size();
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_attach_boundary(xdg_hip_ctx* c, int32_t element_tree, int32_t surface_tree, int64_t n_faces,
+                            const int32_t* bface_elem)
+{
+  return guarded(c, [&]() {
+    elem_tree(c, element_tree);
+    const SurfTreeHost& st = surf_tree(c, surface_tree);
+    require(n_faces >= 0 && (n_faces == 0 || bface_elem), "null bface_elem");
+    require(st.dev.n_prims == 0 || (st.min_face >= 0 && st.max_face < n_faces),
+            "surface tree face ids must index bface_elem");
+    ElemTreeHost& et = c->elem[element_tree - 1];
+    for (int64_t i = 0; i < n_faces; ++i)
+      require(bface_elem[i] >= -1 && bface_elem[i] < et.dev.n_prims, "bface_elem out of range");
+    int32_t* d = (int32_t*)dev_alloc(sizeof(int32_t) * std::max<int64_t>(n_faces, 1), et.bufs);
+    if (n_faces > 0) XDG_HIP_CHECK(hipMemcpy(d, bface_elem, sizeof(int32_t) * n_faces, hipMemcpyHostToDevice));
+    et.dev.ipc_tree = surface_tree;
+    et.dev.bface_elem = d;
+    et.n_bfaces = n_faces;
+    et.bytes = bufs_bytes(et.bufs);
+    sync_tables(c);
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_get_tree_info(xdg_hip_ctx* c, int32_t tree, int element, xdg_hip_tree_info* info)
+{
+  return guarded(c, [&]() {
+    require(info != nullptr, "info is null");
+    if (element) {
+      const ElemTreeHost& t = elem_tree(c, tree);
+      info->n_prims = t.dev.n_prims;
+      info->n_nodes = t.n_nodes;
+      info->depth = t.dev.depth;
+      info->root = t.dev.root;
+      info->bytes = t.bytes;
+      info->bump = 0.0;
+      info->o_limit = 0.0;
+    } else {
+      const SurfTreeHost& t = surf_tree(c, tree);
+      info->n_prims = t.dev.n_prims;
+      info->n_nodes = t.n_nodes;
+      info->depth = t.dev.depth;
+      info->root = t.dev.root;
+      info->bytes = t.bytes;
+      info->bump = t.dev.bump;
+      info->o_limit = t.dev.o_limit;
+    }
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_num_trees(xdg_hip_ctx* c, int32_t* surface_trees, int32_t* element_trees)
+{
+  return guarded(c, [&]() {
+    if (surface_trees) *surface_trees = (int32_t)c->surf.size();
+    if (element_trees) *element_trees = (int32_t)c->elem.size();
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_ray_fire(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per_ray, int64_t n, const double* org,
+                     const double* dir, const double* tmax, int orientation, const int32_t* excl, int k,
+                     int precision, double* t_out, int32_t* surf_out, int32_t* face_out, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    require(orientation >= -1 && orientation <= 1, "unknown orientation");
+    require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 8]");
+    int depth = c->max_surf_depth;
+    if (!tree_per_ray) depth = surf_tree(c, tree).dev.depth;
+    if (n == 0) return XDG_HIP_OK;
+    require(org && dir && t_out && surf_out, "null array");
+    RayFireArgs a{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir, tmax, orientation,
+                  excl, excl ? k : 0, t_out, surf_out, face_out};
+    launch_ray_fire(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_point_in_volume(xdg_hip_ctx* c, int32_t tree, int64_t n, const double* pts, const double* dir,
+                            const int32_t* excl, int k, int precision, uint8_t* inside_out, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 8]");
+    const SurfTreeHost& t = surf_tree(c, tree);
+    if (n == 0) return XDG_HIP_OK;
+    require(pts && inside_out, "null array");
+    PivArgs a{};
+    a.table = c->d_surf;
+    a.ntrees = (int32_t)c->surf.size();
+    a.tree = tree;
+    a.n = n;
+    a.pts = pts;
+    a.dir = dir;
+    // ray_tracer.cpp:268: {1. / std::sqrt(2.0), 1 / std::sqrt(2.0), 0.0}
+    a.dflt[0] = 1. / std::sqrt(2.0);
+    a.dflt[1] = 1 / std::sqrt(2.0);
+    a.dflt[2] = 0.0;
+    a.excl = excl;
+    a.k = excl ? k : 0;
+    a.inside = inside_out;
+    launch_point_in_volume(a, precision == XDG_HIP_MIXED, cfg_for(c, t.dev.depth), pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_find_volume(xdg_hip_ctx* c, int nvol, const int32_t* vol_trees, const int32_t* vol_ids,
+                        int32_t ipc_volume, int64_t n, const double* pts, const double* dir, int precision,
+                        int32_t* vol_out, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0 && nvol >= 0, "negative size");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    require(nvol == 0 || (vol_trees && vol_ids), "null volume lists");
+    int depth = 1;
+    for (int v = 0; v < nvol; ++v) {
+      if (vol_ids[v] == ipc_volume) continue;
+      depth = std::max(depth, surf_tree(c, vol_trees[v]).dev.depth);
+    }
+    if (n == 0) return XDG_HIP_OK;
+    require(pts && vol_out, "null array");
+    std::vector<int32_t> lists(vol_trees, vol_trees + nvol);
+    lists.insert(lists.end(), vol_ids, vol_ids + nvol);
+    hipStream_t s = pick(c, stream);
+    if (lists != c->small_cache) {
+      XDG_HIP_CHECK(hipDeviceSynchronize());
+      if (lists.size() > c->small_cap) {
+        if (c->d_small) XDG_HIP_CHECK(hipFree(c->d_small));
+        c->d_small = nullptr;
+        c->small_cap = std::max<size_t>(lists.size(), 64);
+        XDG_HIP_CHECK(hipMalloc(&c->d_small, c->small_cap * sizeof(int32_t)));
+      }
+      if (!lists.empty())
+        XDG_HIP_CHECK(hipMemcpy(c->d_small, lists.data(), lists.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      c->small_cache = lists;
+    }
+    FindVolumeArgs a{};
+    a.table = c->d_surf;
+    a.ntrees = (int32_t)c->surf.size();
+    a.nvol = nvol;
+    a.vol_trees = c->d_small;
+    a.vol_ids = c->d_small + nvol;
+    a.ipc = ipc_volume;
+    a.n = n;
+    a.pts = pts;
+    a.dir = dir;
+    a.dflt[0] = 1. / std::sqrt(2.0);
+    a.dflt[1] = 1 / std::sqrt(2.0);
+    a.dflt[2] = 0.0;
+    a.vol_out = vol_out;
+    launch_find_volume(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), s);
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_occluded(xdg_hip_ctx* c, int32_t tree, int64_t n, const double* org, const double* dir,
+                     int precision, uint8_t* hit_out, double* dist_out, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    const SurfTreeHost& t = surf_tree(c, tree);
+    if (n == 0) return XDG_HIP_OK;
+    require(org && dir && hit_out, "null array");
+    OccludedArgs a{c->d_surf, (int32_t)c->surf.size(), tree, n, org, dir, hit_out, dist_out};
+    launch_occluded(a, precision == XDG_HIP_MIXED, cfg_for(c, t.dev.depth), pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_closest(xdg_hip_ctx* c, int32_t tree, int64_t n, const double* pts, double* dist_out,
+                    int32_t* face_out, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    const SurfTreeHost& t = surf_tree(c, tree);
+    if (n == 0) return XDG_HIP_OK;
+    require(pts && dist_out && face_out, "null array");
+    ClosestArgs a{c->d_surf, (int32_t)c->surf.size(), tree, n, pts, dist_out, face_out};
+    launch_closest(a, cfg_for(c, t.dev.depth), pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_find_element(xdg_hip_ctx* c, int32_t element_tree, int64_t n, const double* pts, int32_t* elem_out,
+                         void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    const ElemTreeHost& t = elem_tree(c, element_tree);
+    if (n == 0) return XDG_HIP_OK;
+    require(pts && elem_out, "null array");
+    FindElementArgs a{c->d_elem, (int32_t)c->elem.size(), element_tree, n, pts, elem_out};
+    launch_find_element(a, cfg_for(c, t.dev.depth), pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_next_element(xdg_hip_ctx* c, int32_t element_tree, int64_t n, const int32_t* elem, const double* r,
+                         const double* u, int32_t* next_out, double* dist_out, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    elem_tree(c, element_tree);
+    if (n == 0) return XDG_HIP_OK;
+    require(elem && r && u && next_out && dist_out, "null array");
+    NextElementArgs a{c->d_elem, (int32_t)c->elem.size(), element_tree, n, elem, r, u, next_out, dist_out};
+    launch_next_element(a, pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_tally_segments(xdg_hip_ctx* c, int32_t element_tree, int64_t n, const double* p0, const double* p1,
+                           int64_t* tally_fx, int32_t* count_out, uint64_t* hash_out, const int64_t* seq_off,
+                           int32_t* seq_elem, double* seq_len, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    const ElemTreeHost& t = elem_tree(c, element_tree);
+    int depth = t.dev.depth;
+    if (t.dev.ipc_tree >= 1) depth = std::max(depth, surf_tree(c, t.dev.ipc_tree).dev.depth);
+    if (n == 0) return XDG_HIP_OK;
+    require(p0 && p1 && count_out && hash_out, "null array");
+    require(!seq_off || (seq_elem && seq_len), "seq_off given without seq_elem/seq_len");
+    SegmentsArgs a{};
+    a.etable = c->d_elem;
+    a.nelem_trees = (int32_t)c->elem.size();
+    a.stable = c->d_surf;
+    a.nsurf_trees = (int32_t)c->surf.size();
+    a.tree = element_tree;
+    a.n = n;
+    a.p0 = p0;
+    a.p1 = p1;
+    a.tally_fx = (unsigned long long*)tally_fx;
+    a.count = count_out;
+    a.hash = hash_out;
+    a.seq_off = seq_off;
+    a.seq_elem = seq_elem;
+    a.seq_len = seq_len;
+    a.status = c->d_status;
+    launch_segments(a, face_order_kind(t.dev.face_order), cfg_for(c, depth), pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_status(xdg_hip_ctx* c, uint32_t* flags)
+{
+  return guarded(c, [&]() {
+    require(flags != nullptr, "flags is null");
+    XDG_HIP_CHECK(hipDeviceSynchronize());
+    XDG_HIP_CHECK(hipMemcpy(flags, c->d_status, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const uint32_t zero = 0;
+    XDG_HIP_CHECK(hipMemcpy(c->d_status, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_eval_primitive(xdg_hip_ctx* c, int kind, int64_t n, const double* in, double* out, void* stream)
+{
+  return guarded(c, [&]() {
+    require(kind >= 0 && kind <= 5, "unknown primitive kind");
+    require(n >= 0, "negative n");
+    if (n == 0) return XDG_HIP_OK;
+    require(in && out, "null array");
+    launch_eval(kind, n, in, out, pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_malloc(xdg_hip_ctx* c, size_t bytes, void** ptr)
+{
+  return guarded(c, [&]() {
+    require(ptr != nullptr, "ptr is null");
+    XDG_HIP_CHECK(hipMalloc(ptr, bytes ? bytes : 16));
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_free(xdg_hip_ctx* c, void* ptr)
+{
+  return guarded(c, [&]() {
+    if (ptr) XDG_HIP_CHECK(hipFree(ptr));
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_memcpy_htod(xdg_hip_ctx* c, void* dst, const void* src, size_t bytes)
+{
+  return guarded(c, [&]() {
+    if (bytes) XDG_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_memcpy_dtoh(xdg_hip_ctx* c, void* dst, const void* src, size_t bytes)
+{
+  return guarded(c, [&]() {
+    if (bytes) XDG_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return XDG_HIP_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,382 @@
+// element_query.hip -- tetrahedral element location and the element walk.
+//
+// Reference semantics:
+//   find_element   EmbreeRayTracer::find_element src/embree/ray_tracer.cpp:226-257
+//                  + TetrahedronOcclusionFunc src/tetrahedron_contain.cpp:94-114
+//                  (first containing tet in Embree order; here: the lowest
+//                  element index whose containment test passes)
+//   next_element   MeshManager::next_element src/mesh_manager_interface.cpp:150-200
+//   walk_elements  src/mesh_manager_interface.cpp:107-136
+//   segments       XDG::segments src/xdg.cpp:142-198 (including the
+//                  implicit-complement re-entry branch :164-185)
+//
+// One segment per lane.  The walker reads one 128-byte ElemRec per step
+// (vertices + adjacency) and evaluates the four oriented Pluecker face tests
+// exactly as next_element does.  Per-element track lengths are accumulated
+// as fixed-point integers (len * 2^40, round-to-nearest-even) with 64-bit
+// atomics, so the tally is independent of the order in which lanes add.
+#include "internal.h"
+#include "surface_traverse.h"
+
+namespace xdgh {
+
+using namespace xdgd;
+
+namespace {
+
+constexpr int MAX_WALK_STEPS = 1 << 16;
+constexpr int MAX_SEG_LOOPS = 64;
+constexpr unsigned long long HASH_SEED = 0x6a09e667f3bcc909ULL;
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x)
+{
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ULL;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebULL;
+  x ^= x >> 31;
+  return x;
+}
+
+__device__ __forceinline__ float f32_dn(double x)
+{
+  float f = (float)x;
+  if ((double)f > x) f = nextafterf(f, -INFINITY);
+  return f;
+}
+
+__device__ __forceinline__ bool in_box_f(const float* b, float plo_x, float plo_y, float plo_z, float phi_x,
+                                         float phi_y, float phi_z)
+{
+  // conservative: p in [lo, hi] (FP64) implies round_up(p) >= lo_f and
+  // round_down(p) <= hi_f for outward-rounded lo_f <= lo, hi_f >= hi
+  return phi_x >= b[0] && phi_y >= b[1] && phi_z >= b[2] && plo_x <= b[3] && plo_y <= b[4] && plo_z <= b[5];
+}
+
+__device__ __forceinline__ bool elem_contains(const ElemRec& e, const D3& p)
+{
+  return tet_contains(p, d3(e.v[0], e.v[1], e.v[2]), d3(e.v[3], e.v[4], e.v[5]), d3(e.v[6], e.v[7], e.v[8]),
+                      d3(e.v[9], e.v[10], e.v[11]));
+}
+
+// lowest containing element index, -1 if none
+__device__ int32_t find_element_dev(const ElemTreeDev& T, const D3& p, int32_t* stk, int stride, int cap)
+{
+  if (T.n_prims <= 0) return ID_NONE;
+  const float lx = f32_dn(p.x), ly = f32_dn(p.y), lz = f32_dn(p.z);
+  const float hx = f32_up(p.x), hy = f32_up(p.y), hz = f32_up(p.z);
+  int32_t best = INT32_MAX;
+  int32_t node = T.root;
+  int sp = 0;
+  while (true) {
+    if (node >= 0) {
+      const NodeF nd = T.nodes_f[node];
+      const bool h0 = in_box_f(nd.b0, lx, ly, lz, hx, hy, hz);
+      const bool h1 = in_box_f(nd.b1, lx, ly, lz, hx, hy, hz);
+      if (h0 && h1) {
+        if (sp < cap) stk[sp++ * stride] = nd.c1;
+        node = nd.c0;
+      } else if (h0) {
+        node = nd.c0;
+      } else if (h1) {
+        node = nd.c1;
+      } else {
+        if (sp == 0) break;
+        node = stk[--sp * stride];
+      }
+      continue;
+    }
+    const int32_t first = leaf_first(node);
+    const int32_t cnt = leaf_count(node);
+    for (int32_t j = 0; j < cnt; ++j) {
+      const int32_t e = T.order[first + j];
+      if (e >= best) continue;
+      if (elem_contains(T.elems[e], p)) best = e;
+    }
+    if (sp == 0) break;
+    node = stk[--sp * stride];
+  }
+  return best == INT32_MAX ? ID_NONE : best;
+}
+
+// Tet face orders (local vertex triples).  FO_MOAB: MOAB canonical
+// (include/xdg/moab/direct_access.h:181-183); FO_MOCK: MeshMock
+// (tests/mesh_mock.h:108-115); FO_TABLE: any other order, read from the tree.
+enum FaceOrder { FO_MOAB = 0, FO_MOCK = 1, FO_TABLE = 2 };
+
+template <int FO>
+__device__ __forceinline__ int fo_at(const ElemTreeDev& T, int f, int j)
+{
+  constexpr int moab[12] = {0, 1, 3, 1, 2, 3, 2, 0, 3, 0, 2, 1};
+  constexpr int mock[12] = {0, 1, 2, 0, 2, 3, 0, 3, 1, 1, 3, 2};
+  if (FO == FO_MOAB) return moab[3 * f + j];
+  if (FO == FO_MOCK) return mock[3 * f + j];
+  return T.face_order[3 * f + j];
+}
+
+template <int FO>
+__device__ __forceinline__ D3 face_vertex(const ElemTreeDev& T, const ElemRec& E, const D3* v, int f, int j)
+{
+  if (FO == FO_TABLE) {
+    const int k = T.face_order[3 * f + j];
+    return d3(E.v[3 * k], E.v[3 * k + 1], E.v[3 * k + 2]);
+  }
+  return v[fo_at<FO>(T, f, j)];
+}
+
+// MeshManager::next_element (mesh_manager_interface.cpp:150-200)
+template <int FO>
+__device__ __forceinline__ void next_element_dev(const ElemTreeDev& T, int32_t elem, const D3& r, const D3& u,
+                                                 int idx, double u_i, int32_t& next, double& dist)
+{
+  const ElemRec& E = T.elems[elem];
+  D3 v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = d3(E.v[3 * j], E.v[3 * j + 1], E.v[3 * j + 2]);
+  const D3 rb = cross(u, r);
+  const double r_i = pick3(r.x, r.y, r.z, idx);
+  int idx_out = ID_NONE;
+  double min_dist = INFTY;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const D3 a = face_vertex<FO>(T, E, v, f, 0);
+    const D3 b = face_vertex<FO>(T, E, v, f, 1);
+    const D3 c = face_vertex<FO>(T, E, v, f, 2);
+    double t;
+    double df = INFTY;
+    if (plucker_oriented(a, b, c, u, rb, idx, r_i, u_i, 1.0, INFTY, 0.0, t)) df = (0.0 < t) ? t : 0.0;
+    if (df < min_dist) {
+      min_dist = df;
+      idx_out = f;
+    }
+  }
+  next = idx_out == ID_NONE ? ID_NONE : E.adj[idx_out];
+  dist = min_dist;
+}
+
+__global__ void k_find_element(FindElementArgs a, int cap)
+{
+  extern __shared__ int32_t lds[];
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  int32_t out = ID_NONE;
+  if (a.tree >= 1 && a.tree <= a.nelem_trees) {
+    const D3 p = d3(a.pts[3 * i], a.pts[3 * i + 1], a.pts[3 * i + 2]);
+    out = find_element_dev(a.etable[a.tree], p, lds + threadIdx.x, blockDim.x, cap);
+  }
+  a.elem_out[i] = out;
+}
+
+__global__ void k_next_element(NextElementArgs a)
+{
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  int32_t nx = ID_NONE;
+  double d = INFTY;
+  if (a.tree >= 1 && a.tree <= a.nelem_trees) {
+    const ElemTreeDev& T = a.etable[a.tree];
+    const int32_t e = a.elem[i];
+    if (e >= 0 && e < T.n_prims) {
+      const D3 r = d3(a.r[3 * i], a.r[3 * i + 1], a.r[3 * i + 2]);
+      const D3 u = d3(a.u[3 * i], a.u[3 * i + 1], a.u[3 * i + 2]);
+      const int idx = major_axis(u);
+      next_element_dev<FO_TABLE>(T, e, r, u, idx, pick3(u.x, u.y, u.z, idx), nx, d);
+    }
+  }
+  a.next_out[i] = nx;
+  a.dist_out[i] = d;
+}
+
+struct Sink {
+  int32_t count;
+  unsigned long long hash;
+  int64_t cap;
+  int32_t* se;
+  double* sl;
+  unsigned long long* tally;
+};
+
+__device__ __forceinline__ void sink_push(Sink& s, int32_t e, double len)
+{
+  if (s.count < s.cap) {
+    s.se[s.count] = e;
+    s.sl[s.count] = len;
+  }
+  s.count++;
+  s.hash = mix64(s.hash ^ (unsigned long long)(uint32_t)e);
+  s.hash = mix64(s.hash ^ (unsigned long long)__double_as_longlong(len));
+  if (s.tally) {
+    long long q = __double2ll_rn(len * 1099511627776.0);  // len * 2^40, exact scaling
+    atomicAdd(s.tally + e, (unsigned long long)q);
+  }
+}
+
+template <int FO>
+__global__ void k_segments(SegmentsArgs a, int cap)
+{
+  extern __shared__ int32_t lds[];
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  Sink s;
+  s.count = 0;
+  s.hash = HASH_SEED;
+  s.tally = a.tally_fx;
+  if (a.seq_off) {
+    s.se = a.seq_elem + a.seq_off[i];
+    s.sl = a.seq_len + a.seq_off[i];
+    s.cap = a.seq_off[i + 1] - a.seq_off[i];
+  } else {
+    s.se = nullptr;
+    s.sl = nullptr;
+    s.cap = 0;
+  }
+  if (a.tree < 1 || a.tree > a.nelem_trees) {
+    a.count[i] = 0;
+    a.hash[i] = s.hash;
+    return;
+  }
+  const ElemTreeDev& T = a.etable[a.tree];
+  int32_t* stk = lds + threadIdx.x;
+  const int stride = blockDim.x;
+  // xdg.cpp:150-153
+  D3 r = d3(a.p0[3 * i], a.p0[3 * i + 1], a.p0[3 * i + 2]);
+  D3 u = sub(d3(a.p1[3 * i], a.p1[3 * i + 1], a.p1[3 * i + 2]), r);
+  double distance = sqrt(u.x * u.x + u.y * u.y + u.z * u.z);
+  u.x = u.x / distance;
+  u.y = u.y / distance;
+  u.z = u.z / distance;
+  const int idx = major_axis(u);
+  const double u_i = pick3(u.x, u.y, u.z, idx);
+  int32_t last = ID_NONE;
+  int steps = 0, loops = 0;
+  bool capped = false;
+  while (distance > 0) {
+    if (++loops > MAX_SEG_LOOPS || steps >= MAX_WALK_STEPS) {
+      capped = true;
+      break;
+    }
+    int32_t cur = find_element_dev(T, r, stk, stride, cap);
+    if (s.count > 0 && cur == last) cur = ID_NONE;
+    if (cur == ID_NONE) {
+      // xdg.cpp:164-185: fire against the implicit complement
+      if (T.ipc_tree < 1 || T.ipc_tree > a.nsurf_trees) break;
+      RayD ray;
+      ray.o = d3(r.x + TINY_BIT * u.x, r.y + TINY_BIT * u.y, r.z + TINY_BIT * u.z);
+      ray.d = u;
+      ray.rb = cross(ray.d, ray.o);
+      ray.idx = idx;
+      ray.oi = pick3(ray.o.x, ray.o.y, ray.o.z, idx);
+      ray.di = u_i;
+      HitState h;
+      h.t = INFTY;
+      h.face = INT32_MAX;
+      h.surf = ID_NONE;
+      h.inside = false;
+      trace<RAYFIRE, false>(a.stable[T.ipc_tree], ray, ORIENT_EXITING, nullptr, 0, stk, stride, cap, h);
+      if (h.surf == ID_NONE || h.t > distance) break;
+      const double hit_dist = h.t + TINY_BIT;
+      r.x = r.x + hit_dist * u.x;
+      r.y = r.y + hit_dist * u.y;
+      r.z = r.z + hit_dist * u.z;
+      distance -= hit_dist;
+      const int32_t adjacent = T.bface_elem ? T.bface_elem[h.face] : ID_NONE;
+      if (adjacent == ID_NONE) break;
+      cur = adjacent;
+    }
+    // walk_elements (mesh_manager_interface.cpp:107-136) from cur
+    D3 rr = r;
+    double dist_w = distance;
+    double sum = 0.0;
+    while (dist_w > 0) {
+      int32_t next;
+      double ex;
+      next_element_dev<FO>(T, cur, rr, u, idx, u_i, next, ex);
+      const double len = (dist_w < ex) ? dist_w : ex;
+      dist_w -= len;
+      sink_push(s, cur, len);
+      last = cur;
+      sum = sum + len;
+      rr.x = rr.x + len * u.x;
+      rr.y = rr.y + len * u.y;
+      rr.z = rr.z + len * u.z;
+      cur = next;
+      if (cur == ID_NONE) break;
+      if (++steps >= MAX_WALK_STEPS) break;
+    }
+    r.x = r.x + sum * u.x;
+    r.y = r.y + sum * u.y;
+    r.z = r.z + sum * u.z;
+    distance -= sum;
+  }
+  a.count[i] = s.count;
+  a.hash[i] = s.hash;
+  if (capped && a.status) atomicOr(a.status, 1u);
+}
+
+__global__ void k_elem_boxes(const ElemRec* __restrict__ elems, int64_t n, double* box6)
+{
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const ElemRec& e = elems[i];
+  // BoundingBox::from_points + dilation (bbox.h:85-102): chord * 1e-6
+  double lo[3] = {INFTY, INFTY, INFTY}, hi[3] = {-INFTY, -INFTY, -INFTY};
+  for (int j = 0; j < 4; ++j)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fmin(lo[k], e.v[3 * j + k]);
+      hi[k] = fmax(hi[k], e.v[3 * j + k]);
+    }
+  const double wx = hi[0] - lo[0], wy = hi[1] - lo[1], wz = hi[2] - lo[2];
+  const double bump = sqrt(wx * wx + wy * wy + wz * wz) * 1e-6;
+  for (int k = 0; k < 3; ++k) {
+    box6[6 * i + k] = lo[k] - bump;
+    box6[6 * i + 3 + k] = hi[k] + bump;
+  }
+}
+
+inline dim3 grid1(int64_t n, int blk) { return dim3((unsigned)((n + blk - 1) / blk)); }
+inline size_t lds_bytes(const LaunchCfg& c) { return sizeof(int32_t) * (size_t)c.stack * c.block; }
+
+}  // namespace
+
+void launch_find_element(const FindElementArgs& a, const LaunchCfg& c, hipStream_t s)
+{
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(k_find_element, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+}
+
+void launch_next_element(const NextElementArgs& a, hipStream_t s)
+{
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(k_next_element, grid1(a.n, 256), dim3(256), 0, s, a);
+}
+
+void launch_segments(const SegmentsArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s)
+{
+  if (a.n <= 0) return;
+  if (face_order_kind == FO_MOAB)
+    hipLaunchKernelGGL(k_segments<FO_MOAB>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+  else if (face_order_kind == FO_MOCK)
+    hipLaunchKernelGGL(k_segments<FO_MOCK>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+  else
+    hipLaunchKernelGGL(k_segments<FO_TABLE>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+}
+
+int face_order_kind(const int32_t* fo)
+{
+  static const int32_t moab[12] = {0, 1, 3, 1, 2, 3, 2, 0, 3, 0, 2, 1};
+  static const int32_t mock[12] = {0, 1, 2, 0, 2, 3, 0, 3, 1, 1, 3, 2};
+  bool is_moab = true, is_mock = true;
+  for (int i = 0; i < 12; ++i) {
+    is_moab = is_moab && fo[i] == moab[i];
+    is_mock = is_mock && fo[i] == mock[i];
+  }
+  return is_moab ? FO_MOAB : (is_mock ? FO_MOCK : FO_TABLE);
+}
+
+void launch_elem_boxes(const ElemRec* elems, int64_t n, double* box6, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_elem_boxes, grid1(n, 256), dim3(256), 0, s, elems, n, box6);
+}
+
+}  // namespace xdgh

@@ -1,0 +1,182 @@
+// internal.h -- host-side declarations shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "xdg_device.h"
+
+namespace xdgh {
+
+struct HipError {
+  std::string msg;
+};
+
+#define XDG_HIP_CHECK(expr)                                                              \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      throw ::xdgh::HipError{std::string(#expr) + ": " + hipGetErrorString(_e)};         \
+  } while (0)
+
+// Owned device allocation (freed by the owner).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+void* dev_alloc(size_t bytes, std::vector<DevBuf>& owned);
+void dev_free_all(std::vector<DevBuf>& owned);
+
+// Result of one LBVH build over n primitive boxes.
+struct LbvhOut {
+  xdgd::NodeD* nodes_d = nullptr;  // n-1 slots (unreachable slots left unused)
+  xdgd::NodeF* nodes_f = nullptr;
+  int32_t* order = nullptr;        // leaf-ordered primitive indices (n)
+  int32_t root = 0;
+  int32_t depth = 0;
+  int64_t n_nodes = 0;
+  double max_abs = 0.0;            // max |coordinate| over all boxes
+};
+
+// box6: device array of n boxes (lo.xyz, hi.xyz), already dilated.
+void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std::vector<DevBuf>& owned,
+                LbvhOut& out);
+
+// ---- surface queries (surface_query.hip) ---------------------------------
+struct RayFireArgs {
+  const xdgd::SurfTreeDev* table;
+  int32_t ntrees;        // table holds ids 1..ntrees
+  int32_t tree;          // used when tree_per_ray == nullptr
+  const int32_t* tree_per_ray;
+  int64_t n;
+  const double* org;
+  const double* dir;
+  const double* tmax;    // nullable
+  int orientation;
+  const int32_t* excl;   // nullable, k per ray
+  int k;
+  double* t_out;
+  int32_t* surf_out;
+  int32_t* face_out;
+};
+
+struct LaunchCfg {
+  int block = 256;
+  int stack = 64;  // LDS stack entries per lane
+};
+
+void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
+
+struct PivArgs {
+  const xdgd::SurfTreeDev* table;
+  int32_t ntrees;
+  int32_t tree;
+  int64_t n;
+  const double* pts;
+  const double* dir;  // nullable -> default direction
+  double dflt[3];
+  const int32_t* excl;
+  int k;
+  uint8_t* inside;
+};
+void launch_point_in_volume(const PivArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
+
+struct FindVolumeArgs {
+  const xdgd::SurfTreeDev* table;
+  int32_t ntrees;
+  int nvol;
+  const int32_t* vol_trees;  // device, nvol
+  const int32_t* vol_ids;    // device, nvol
+  int32_t ipc;
+  int64_t n;
+  const double* pts;
+  const double* dir;  // nullable -> default direction
+  double dflt[3];
+  int32_t* vol_out;
+};
+void launch_find_volume(const FindVolumeArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
+
+struct OccludedArgs {
+  const xdgd::SurfTreeDev* table;
+  int32_t ntrees;
+  int32_t tree;
+  int64_t n;
+  const double* org;
+  const double* dir;
+  uint8_t* hit;
+  double* dist;  // nullable
+};
+void launch_occluded(const OccludedArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
+
+struct ClosestArgs {
+  const xdgd::SurfTreeDev* table;
+  int32_t ntrees;
+  int32_t tree;
+  int64_t n;
+  const double* pts;
+  double* dist;
+  int32_t* face;
+};
+void launch_closest(const ClosestArgs& a, const LaunchCfg& c, hipStream_t s);
+
+// ---- element queries (element_query.hip) ---------------------------------
+struct FindElementArgs {
+  const xdgd::ElemTreeDev* etable;
+  int32_t nelem_trees;
+  int32_t tree;
+  int64_t n;
+  const double* pts;
+  int32_t* elem_out;
+};
+void launch_find_element(const FindElementArgs& a, const LaunchCfg& c, hipStream_t s);
+
+struct NextElementArgs {
+  const xdgd::ElemTreeDev* etable;
+  int32_t nelem_trees;
+  int32_t tree;
+  int64_t n;
+  const int32_t* elem;
+  const double* r;
+  const double* u;
+  int32_t* next_out;
+  double* dist_out;
+};
+void launch_next_element(const NextElementArgs& a, hipStream_t s);
+
+struct SegmentsArgs {
+  const xdgd::ElemTreeDev* etable;
+  int32_t nelem_trees;
+  const xdgd::SurfTreeDev* stable;
+  int32_t nsurf_trees;
+  int32_t tree;
+  int64_t n;
+  const double* p0;
+  const double* p1;
+  unsigned long long* tally_fx;  // nullable, per element
+  int32_t* count;
+  uint64_t* hash;
+  const int64_t* seq_off;  // nullable
+  int32_t* seq_elem;
+  double* seq_len;
+  uint32_t* status;  // device word: bit0 = step cap hit
+};
+void launch_segments(const SegmentsArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s);
+// 0 = MOAB canonical, 1 = MeshMock, 2 = other (read from the tree)
+int face_order_kind(const int32_t* face_order);
+
+// device evaluation of one geometry primitive per item (parity checks)
+void launch_eval(int kind, int64_t n, const double* in, double* out, hipStream_t s);
+
+// element prim boxes (per-element dilation, tetrahedron_contain.cpp:47-63)
+void launch_elem_boxes(const xdgd::ElemRec* elems, int64_t n, double* box6, hipStream_t s);
+// triangle prim boxes (+- bump, triangle_intersect.cpp:36-50)
+void launch_tri_boxes(const double* verts9, int64_t n, double bump, double* box6, hipStream_t s);
+// leaf-ordered triangle records
+void launch_write_tris(const double* verts9, const int32_t* face, const int32_t* surf, const uint8_t* flip,
+                       const int32_t* order, int64_t n, xdgd::TriRec* out, hipStream_t s);
+
+}  // namespace xdgh

@@ -1,0 +1,227 @@
+// surface_traverse.h -- BVH traversal core for surface-tree queries.
+//
+// Shared by the surface query kernels and the tet walker (which fires rays
+// against the implicit complement, src/xdg.cpp:164-185).  See
+// surface_query.hip for the semantics and the tie rule.
+#pragma once
+
+#include "xdg_device.h"
+
+namespace xdgd {
+
+enum Mode { RAYFIRE = 0, PIV = 1, OCCL = 2 };
+
+struct RayD {
+  D3 o, d, rb;
+  int idx;
+  double oi, di;  // origin / direction on the major axis
+};
+
+struct BoxRayD {
+  double ix, iy, iz, nx, ny, nz;  // inv dir, -o * inv
+};
+struct BoxRayF {
+  float ix, iy, iz, nx, ny, nz;
+};
+
+__device__ __forceinline__ double safe_inv_d(double d)
+{
+  const double eps = 1e-300;
+  return 1.0 / (fabs(d) > eps ? d : copysign(eps, d));
+}
+__device__ __forceinline__ float safe_inv_f(float d)
+{
+  const float eps = 1e-30f;
+  return 1.0f / (fabsf(d) > eps ? d : copysignf(eps, d));
+}
+
+__device__ __forceinline__ float f32_up(double x)
+{
+  float f = (float)x;
+  if ((double)f < x) f = nextafterf(f, INFINITY);
+  return f;
+}
+
+// slab test of one box; returns entry distance in tn
+__device__ __forceinline__ bool slab_d(const double* b, const BoxRayD& r, double tmax, double& tn)
+{
+  double x0 = fma(b[0], r.ix, r.nx), x1 = fma(b[3], r.ix, r.nx);
+  double y0 = fma(b[1], r.iy, r.ny), y1 = fma(b[4], r.iy, r.ny);
+  double z0 = fma(b[2], r.iz, r.nz), z1 = fma(b[5], r.iz, r.nz);
+  double a = fmax(fmax(fmin(x0, x1), fmin(y0, y1)), fmax(fmin(z0, z1), 0.0));
+  double c = fmin(fmin(fmax(x0, x1), fmax(y0, y1)), fmin(fmax(z0, z1), tmax));
+  tn = a;
+  return a <= c;
+}
+
+__device__ __forceinline__ bool slab_f(const float* b, const BoxRayF& r, float tmax, float& tn)
+{
+  float x0 = fmaf(b[0], r.ix, r.nx), x1 = fmaf(b[3], r.ix, r.nx);
+  float y0 = fmaf(b[1], r.iy, r.ny), y1 = fmaf(b[4], r.iy, r.ny);
+  float z0 = fmaf(b[2], r.iz, r.nz), z1 = fmaf(b[5], r.iz, r.nz);
+  float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+  float c = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+  tn = a;
+  return a <= c;
+}
+
+struct HitState {
+  double t;
+  int32_t face;
+  int32_t surf;
+  bool inside;  // PIV: dot(dir, sense-flipped normal) > 0 of the committed hit
+};
+
+// Leaf triangle for RAYFIRE / PIV: TriangleIntersectionFunc with the tie rule.
+template <int MODE>
+__device__ __forceinline__ void tri_closest_hit(const TriRec* __restrict__ tris, int32_t p, const RayD& ray,
+                                                int orientation, const int32_t* ex, int k, HitState& h)
+{
+  const TriRec& r = tris[p];
+  const D3 v0 = d3(r.v[0], r.v[1], r.v[2]);
+  const D3 v1 = d3(r.v[3], r.v[4], r.v[5]);
+  const D3 v2 = d3(r.v[6], r.v[7], r.v[8]);
+  double t;
+  if (!plucker_any(v0, v1, v2, ray.d, ray.rb, ray.idx, ray.oi, ray.di, h.t, 0.0, t)) return;
+  const int32_t face = r.face;
+  if (t == h.t && face >= h.face) return;  // tie rule
+  const int32_t s2 = r.surf2;
+  D3 n = face_normal(v0, v1, v2);
+  if (s2 & 1) n = neg(n);
+  if (MODE == RAYFIRE && orientation != ORIENT_ANY) {
+    const double dp = dot(ray.d, n);
+    if (orientation == ORIENT_EXITING && dp < 0.0) return;
+    if (orientation == ORIENT_ENTERING && dp >= 0.0) return;
+  }
+  for (int j = 0; j < k; ++j)
+    if (ex[j] == face) return;
+  h.t = t;
+  h.face = face;
+  h.surf = s2 >> 1;
+  if (MODE == PIV) h.inside = dot(ray.d, n) > 0.0;
+}
+
+template <int MODE>
+__device__ __forceinline__ bool tri_any_hit(const TriRec* __restrict__ tris, int32_t p, const RayD& ray)
+{
+  const TriRec& r = tris[p];
+  const D3 v0 = d3(r.v[0], r.v[1], r.v[2]);
+  const D3 v1 = d3(r.v[3], r.v[4], r.v[5]);
+  const D3 v2 = d3(r.v[6], r.v[7], r.v[8]);
+  double t;
+  return plucker_any(v0, v1, v2, ray.d, ray.rb, ray.idx, ray.oi, ray.di, INFTY, 0.0, t);
+}
+
+// Generic traversal.  Returns true for OCCL when any hit is found.
+template <int MODE, bool MIXED>
+__device__ __forceinline__ bool traverse(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
+                                         int k, int32_t* stk, int stride, int depth_cap, HitState& h)
+{
+  if (T.n_prims <= 0) return false;
+  BoxRayD bd;
+  BoxRayF bf;
+  if (MIXED) {
+    float ox = (float)ray.o.x, oy = (float)ray.o.y, oz = (float)ray.o.z;
+    bf.ix = safe_inv_f((float)ray.d.x);
+    bf.iy = safe_inv_f((float)ray.d.y);
+    bf.iz = safe_inv_f((float)ray.d.z);
+    bf.nx = -ox * bf.ix;
+    bf.ny = -oy * bf.iy;
+    bf.nz = -oz * bf.iz;
+  } else {
+    bd.ix = safe_inv_d(ray.d.x);
+    bd.iy = safe_inv_d(ray.d.y);
+    bd.iz = safe_inv_d(ray.d.z);
+    bd.nx = -ray.o.x * bd.ix;
+    bd.ny = -ray.o.y * bd.iy;
+    bd.nz = -ray.o.z * bd.iz;
+  }
+  float tf = MODE == OCCL ? INFINITY : f32_up(h.t);
+  int32_t node = T.root;
+  int sp = 0;
+  while (true) {
+    if (node >= 0) {
+      int32_t c0, c1;
+      bool h0, h1;
+      bool swap;
+      if (MIXED) {
+        const NodeF nd = T.nodes_f[node];
+        float t0, t1;
+        h0 = slab_f(nd.b0, bf, tf, t0);
+        h1 = slab_f(nd.b1, bf, tf, t1);
+        c0 = nd.c0;
+        c1 = nd.c1;
+        swap = t1 < t0;
+      } else {
+        const NodeD nd = T.nodes_d[node];
+        double t0, t1;
+        double tm = MODE == OCCL ? INFTY : h.t;
+        h0 = slab_d(nd.b0, bd, tm, t0);
+        h1 = slab_d(nd.b1, bd, tm, t1);
+        c0 = nd.c0;
+        c1 = nd.c1;
+        swap = t1 < t0;
+      }
+      if (h0 && h1) {
+        if (swap) {
+          int32_t tmp = c0;
+          c0 = c1;
+          c1 = tmp;
+        }
+        if (sp < depth_cap) stk[sp++ * stride] = c1;
+        node = c0;
+      } else if (h0) {
+        node = c0;
+      } else if (h1) {
+        node = c1;
+      } else {
+        if (sp == 0) break;
+        node = stk[--sp * stride];
+      }
+      continue;
+    }
+    const int32_t first = leaf_first(node);
+    const int32_t cnt = leaf_count(node);
+    for (int32_t j = 0; j < cnt; ++j) {
+      if (MODE == OCCL) {
+        if (tri_any_hit<MODE>(T.tris, first + j, ray)) return true;
+      } else {
+        tri_closest_hit<MODE>(T.tris, first + j, ray, orientation, ex, k, h);
+      }
+    }
+    if (MODE != OCCL) tf = f32_up(h.t);
+    if (sp == 0) break;
+    node = stk[--sp * stride];
+  }
+  return false;
+}
+
+__device__ __forceinline__ RayD make_ray(const double* o, const double* d)
+{
+  RayD r;
+  r.o = d3(o[0], o[1], o[2]);
+  r.d = d3(d[0], d[1], d[2]);
+  r.rb = cross(r.d, r.o);  // plucker.h:75
+  r.idx = major_axis(r.d);
+  r.oi = o[r.idx];
+  r.di = d[r.idx];
+  return r;
+}
+
+__device__ __forceinline__ bool mixed_safe(const SurfTreeDev& T, const RayD& r)
+{
+  double m = fmax(fmax(fabs(r.o.x), fabs(r.o.y)), fabs(r.o.z));
+  return m <= (double)T.o_limit;
+}
+
+template <int MODE, bool MIXED>
+__device__ __forceinline__ bool trace(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
+                                      int k, int32_t* stk, int stride, int cap, HitState& h)
+{
+  if (MIXED) {
+    if (mixed_safe(T, ray)) return traverse<MODE, true>(T, ray, orientation, ex, k, stk, stride, cap, h);
+  }
+  return traverse<MODE, false>(T, ray, orientation, ex, k, stk, stride, cap, h);
+}
+
+}  // namespace xdgd

@@ -1,0 +1,349 @@
+// xdg_device.h -- device-side geometry primitives and data layouts.
+//
+// Every FP64 expression keeps the reference's operation order (the whole
+// library is compiled with -ffp-contract=off, so no FMA is formed implicitly)
+// and therefore rounds bit-identically to the reference C++ code:
+//   Vec3da dot/cross/normalize   include/xdg/vec3da.h:58-90
+//   plucker_edge_test            include/xdg/geometry/plucker.h:45-62
+//   plucker_ray_tri_intersect    include/xdg/geometry/plucker.h:64-161
+//   face_normal                  src/mesh_manager_interface.cpp:216-220
+//   closest_location_on_triangle src/geometry/closest.cpp:6-183
+//   tetrahedron containment      src/tetrahedron_contain.cpp:11-43
+// Explicit fma()/fmaf() appear only in conservative box tests, whose result
+// decides which candidates are examined, never a returned value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xdgd {
+
+constexpr double INFTY = 1.7976931348623157e308;  // constants.h:15 (DBL_MAX)
+constexpr double PLUCKER_ZERO_TOL = 20.0 * 2.220446049250313080847e-16;
+constexpr double TINY_BIT = 1e-10;                // constants.h TINY_BIT
+// constants.h:80: float(FLT_MIN * 1E5), promoted to double when compared
+constexpr float MIN_RCP_INPUT = 0x1p-126f * 1E5;
+constexpr int ID_NONE = -1;
+
+// HitOrientation (include/xdg/shared_enums.h:11-15)
+constexpr int ORIENT_ANY = -1;
+constexpr int ORIENT_EXITING = 0;
+constexpr int ORIENT_ENTERING = 1;
+
+struct D3 {
+  double x, y, z;
+};
+
+__device__ __forceinline__ D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+__device__ __forceinline__ D3 sub(const D3& a, const D3& b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ double dot(const D3& a, const D3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 cross(const D3& a, const D3& b)
+{
+  return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double pick3(double x, double y, double z, int i)
+{
+  return i == 0 ? x : (i == 1 ? y : z);
+}
+__device__ __forceinline__ D3 neg(const D3& a) { return D3{-a.x, -a.y, -a.z}; }
+
+// Vec3da::normalize (vec3da.h:61-66)
+__device__ __forceinline__ D3 normalize(D3 a)
+{
+  double len = sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+  len = len < (double)MIN_RCP_INPUT ? (double)MIN_RCP_INPUT : len;
+  a.x /= len;
+  a.y /= len;
+  a.z /= len;
+  return a;
+}
+
+// first() plucker.h:35-43
+__device__ __forceinline__ bool first(const D3& a, const D3& b)
+{
+  if (a.x < b.x) return true;
+  if (a.x > b.x) return false;
+  if (a.y < b.y) return true;
+  if (a.y > b.y) return false;
+  return a.z < b.z;
+}
+
+// plucker_edge_test plucker.h:45-62
+__device__ __forceinline__ double edge_test(const D3& va, const D3& vb, const D3& ray, const D3& rn)
+{
+  double pip;
+  if (first(va, vb)) {
+    const D3 e = sub(vb, va);
+    const D3 en = cross(e, va);
+    pip = dot(ray, en) + dot(rn, e);
+  } else {
+    const D3 e = sub(va, vb);
+    const D3 en = cross(e, vb);
+    pip = dot(ray, en) + dot(rn, e);
+    pip = -pip;
+  }
+  if (PLUCKER_ZERO_TOL > fabs(pip)) pip = 0.0;
+  return pip;
+}
+
+// largest-|dir| axis, first wins ties (plucker.h:137-144)
+__device__ __forceinline__ int major_axis(const D3& d)
+{
+  int idx = 0;
+  double m = 0.0;
+  if (fabs(d.x) > m) { idx = 0; m = fabs(d.x); }
+  if (fabs(d.y) > m) { idx = 1; m = fabs(d.y); }
+  if (fabs(d.z) > m) { idx = 2; }
+  return idx;
+}
+
+// Distance from the three Pluecker coordinates (plucker.h:123-160).
+// Returns false for the coplanar / barycentric / range rejections.  o_i and
+// d_i are the ray's origin and direction on the major axis idx.  The three
+// axis candidates are formed and one is selected, so no value is indexed
+// dynamically (which would force the vertices into scratch memory).
+__device__ __forceinline__ bool plucker_finish(double c0, double c1, double c2, const D3& v0, const D3& v1,
+                                               const D3& v2, double o_i, double d_i, int idx, double tmax,
+                                               double tmin, double& t)
+{
+  if (0.0 == c0 && 0.0 == c1 && 0.0 == c2) return false;
+  const double inverse_sum = 1.0 / (c0 + c1 + c2);
+  const double w0 = c0 * inverse_sum;
+  const double w1 = c1 * inverse_sum;
+  const double w2 = c2 * inverse_sum;
+  const double ix_x = w0 * v2.x + w1 * v0.x + w2 * v1.x;
+  const double ix_y = w0 * v2.y + w1 * v0.y + w2 * v1.y;
+  const double ix_z = w0 * v2.z + w1 * v0.z + w2 * v1.z;
+  const double ix = idx == 0 ? ix_x : (idx == 1 ? ix_y : ix_z);
+  double dist = (ix - o_i) / d_i;
+  if (w2 < 0.0 || w0 < 0.0 || (w2 + w0) > 1.0) dist = -1.0;
+  if (dist < tmin || dist > tmax) return false;
+  t = dist;
+  return true;
+}
+
+// plucker_ray_tri_intersect, unoriented mode (useOrientation = false), with
+// the ray moment rb = cross(d, o), the major axis and its components
+// precomputed per ray.
+__device__ __forceinline__ bool plucker_any(const D3& v0, const D3& v1, const D3& v2, const D3& d, const D3& rb,
+                                            int idx, double o_i, double d_i, double tmax, double tmin, double& t)
+{
+  const double c0 = edge_test(v0, v1, d, rb);
+  const double c1 = edge_test(v1, v2, d, rb);
+  if ((0.0 < c0 && 0.0 > c1) || (0.0 > c0 && 0.0 < c1)) return false;
+  const double c2 = edge_test(v2, v0, d, rb);
+  if ((0.0 < c1 && 0.0 > c2) || (0.0 > c1 && 0.0 < c2) || (0.0 < c0 && 0.0 > c2) || (0.0 > c0 && 0.0 < c2))
+    return false;
+  return plucker_finish(c0, c1, c2, v0, v1, v2, o_i, d_i, idx, tmax, tmin, t);
+}
+
+// plucker_ray_tri_intersect, oriented mode (useOrientation = true)
+__device__ __forceinline__ bool plucker_oriented(const D3& v0, const D3& v1, const D3& v2, const D3& d,
+                                                 const D3& rb, int idx, double o_i, double d_i, double ori,
+                                                 double tmax, double tmin, double& t)
+{
+  const double c0 = edge_test(v0, v1, d, rb);
+  if (ori * c0 > 0) return false;
+  const double c1 = edge_test(v1, v2, d, rb);
+  if (ori * c1 > 0) return false;
+  const double c2 = edge_test(v2, v0, d, rb);
+  if (ori * c2 > 0) return false;
+  return plucker_finish(c0, c1, c2, v0, v1, v2, o_i, d_i, idx, tmax, tmin, t);
+}
+
+// MeshManager::face_normal (mesh_manager_interface.cpp:216-220)
+__device__ __forceinline__ D3 face_normal(const D3& v0, const D3& v1, const D3& v2)
+{
+  return normalize(cross(sub(v1, v0), sub(v2, v0)));
+}
+
+// closest_location_on_triangle (src/geometry/closest.cpp:54-183)
+__device__ __forceinline__ D3 axpy(const D3& a, double f, const D3& b)
+{
+  return D3{a.x + f * b.x, a.y + f * b.y, a.z + f * b.z};
+}
+__device__ __forceinline__ D3 axmy(const D3& a, double f, const D3& b)
+{
+  return D3{a.x - f * b.x, a.y - f * b.y, a.z - f * b.z};
+}
+__device__ __forceinline__ D3 lin2(double f, const D3& a, double g, const D3& b)
+{
+  return D3{f * a.x + g * b.x, f * a.y + g * b.y, f * a.z + g * b.z};
+}
+
+__device__ inline D3 closest_on_triangle(const D3& v0, const D3& v1, const D3& v2, const D3& p)
+{
+  const D3 sv = sub(v1, v0);
+  const D3 tv = sub(v2, v0);
+  const D3 pv = sub(v0, p);
+  const double ss = dot(sv, sv);
+  const double st = dot(sv, tv);
+  const double tt = dot(tv, tv);
+  const double sp = dot(sv, pv);
+  const double tp = dot(tv, pv);
+  const double det = ss * tt - st * st;
+  double s = st * tp - tt * sp;
+  double t = st * sp - ss * tp;
+  if (s + t < det) {
+    if (s < 0) {
+      if (t < 0) {  // region 4
+        if (sp < 0) {
+          if (-sp > ss) return v1;
+          return axmy(v0, (sp / ss), sv);
+        } else if (tp < 0) {
+          if (-tp > tt) return v2;
+          return axmy(v0, (tp / tt), tv);
+        }
+        return v0;
+      }
+      // region 3
+      if (tp >= 0) return v0;
+      if (-tp >= tt) return v2;
+      return axmy(v0, (tp / tt), tv);
+    } else if (t < 0) {  // region 5
+      if (sp >= 0.0) return v0;
+      if (-sp >= ss) return v1;
+      return axmy(v0, (sp / ss), sv);
+    }
+    // region 0
+    const double inv_det = 1.0 / det;
+    s *= inv_det;
+    t *= inv_det;
+    return axpy(axpy(v0, s, sv), t, tv);
+  }
+  if (s < 0) {  // region 2
+    s = st + sp;
+    t = tt + tp;
+    if (t > s) {
+      const double num = t - s;
+      const double den = ss - 2 * st + tt;
+      if (num > den) return v1;
+      const double q = num / den;
+      return lin2(q, v1, (1 - q), v2);
+    } else if (t <= 0) {
+      return v2;
+    } else if (tp >= 0) {
+      return v0;
+    }
+    return axmy(v0, (tp / tt), tv);
+  } else if (t < 0) {  // region 6
+    t = st + tp;
+    s = ss + sp;
+    if (s > t) {
+      const double num = t - s;
+      const double den = tt - 2 * st + ss;
+      if (num > den) return v2;
+      t = num / den;
+      s = 1 - t;
+      return lin2(s, v1, t, v2);
+    } else if (s <= 0) {
+      return v1;
+    } else if (sp >= 0) {
+      return v0;
+    }
+    return axmy(v0, (sp / ss), sv);
+  }
+  // region 1
+  const double num = tt + tp - st - sp;
+  if (num <= 0) return v2;
+  const double den = ss - 2 * st + tt;
+  if (num >= den) return v1;
+  const double q = num / den;
+  return lin2(q, v1, (1 - q), v2);
+}
+
+// plucker_tet_containment_test (src/tetrahedron_contain.cpp:11-43):
+// lambda123 = inverse(T) * (p - v0), T = [v1-v0, v2-v0, v3-v0] (columns);
+// inverse = adjugate / det (element-wise); lambda0 = 1 - (l1 + l2 + l3).
+__device__ __forceinline__ bool tet_contains(const D3& p, const D3& v0, const D3& v1, const D3& v2, const D3& v3)
+{
+  const D3 c0 = sub(v1, v0);
+  const D3 c1 = sub(v2, v0);
+  const D3 c2 = sub(v3, v0);
+  const D3 r = sub(p, v0);
+  const D3 a0 = D3{c1.y * c2.z - c2.y * c1.z, c2.y * c0.z - c0.y * c2.z, c0.y * c1.z - c1.y * c0.z};
+  const D3 a1 = D3{c1.z * c2.x - c2.z * c1.x, c2.z * c0.x - c0.z * c2.x, c0.z * c1.x - c1.z * c0.x};
+  const D3 a2 = D3{c1.x * c2.y - c2.x * c1.y, c2.x * c0.y - c0.x * c2.y, c0.x * c1.y - c1.x * c0.y};
+  const double det = c0.x * a0.x + c0.y * a1.x + c0.z * a2.x;
+  const D3 i0 = D3{a0.x / det, a0.y / det, a0.z / det};
+  const D3 i1 = D3{a1.x / det, a1.y / det, a1.z / det};
+  const D3 i2 = D3{a2.x / det, a2.y / det, a2.z / det};
+  const double l1 = i0.x * r.x + i1.x * r.y + i2.x * r.z;
+  const double l2 = i0.y * r.x + i1.y * r.y + i2.y * r.z;
+  const double l3 = i0.z * r.x + i1.z * r.y + i2.z * r.z;
+  const double l0 = 1.0 - (l1 + l2 + l3);
+  const double hi = 1.0 + PLUCKER_ZERO_TOL;
+  if (l0 < -PLUCKER_ZERO_TOL || l0 > hi) return false;
+  if (l1 < -PLUCKER_ZERO_TOL || l1 > hi) return false;
+  if (l2 < -PLUCKER_ZERO_TOL || l2 > hi) return false;
+  if (l3 < -PLUCKER_ZERO_TOL || l3 > hi) return false;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// data layouts in HBM
+// ---------------------------------------------------------------------------
+
+// BVH2 node with both children's boxes inline; one node fetch yields both
+// child boxes.  Child encoding: c >= 0 internal node index; c < 0 leaf,
+// ~c = (first << 3) | (count - 1) over the leaf-ordered primitive array.
+struct alignas(16) NodeD {  // plain FP64 layout, 112 B
+  double b0[6];             // child 0: lo.x lo.y lo.z hi.x hi.y hi.z
+  double b1[6];             // child 1
+  int32_t c0, c1, pad0, pad1;
+};
+
+struct alignas(16) NodeF {  // mixed layout, 64 B: outward-rounded FP32 boxes
+  float b0[6];
+  float b1[6];
+  int32_t c0, c1, pad0, pad1;
+};
+
+// Triangle record in leaf order, 80 B: FP64 vertices + ids.
+struct alignas(16) TriRec {
+  double v[9];
+  int32_t face;   // mesh face id (the primitive id returned by queries)
+  int32_t surf2;  // (surface id << 1) | reverse-sense flag for this tree
+};
+
+// Tetrahedron record indexed by element index, 128 B.
+struct alignas(16) ElemRec {
+  double v[12];     // vertices in connectivity order
+  int32_t adj[4];   // neighbour across face i, -1 on the boundary
+  int32_t conn[4];  // global vertex ids
+};
+
+__host__ __device__ __forceinline__ int32_t leaf_code(int32_t first, int32_t count)
+{
+  return ~((first << 3) | (count - 1));
+}
+__host__ __device__ __forceinline__ int32_t leaf_first(int32_t code) { return (~code) >> 3; }
+__host__ __device__ __forceinline__ int32_t leaf_count(int32_t code) { return ((~code) & 7) + 1; }
+
+// Per-tree descriptor table entry (device-resident, indexed by tree id).
+struct SurfTreeDev {
+  const NodeD* nodes_d;
+  const NodeF* nodes_f;
+  const TriRec* tris;
+  int32_t root;      // child-encoded root
+  int32_t n_prims;
+  float o_limit;     // mixed traversal is conservative for |o|_inf <= o_limit
+  int32_t depth;     // max root-to-leaf depth
+  double bump;
+};
+
+struct ElemTreeDev {
+  const NodeD* nodes_d;
+  const NodeF* nodes_f;
+  const int32_t* order;  // leaf-ordered element indices
+  const ElemRec* elems;
+  int32_t root;
+  int32_t n_prims;
+  int32_t depth;
+  int32_t face_order[12];
+  // implicit-complement re-entry (XDG::segments, xdg.cpp:164-185)
+  int32_t ipc_tree;          // surface tree id, -1 if none
+  const int32_t* bface_elem; // boundary face id -> element
+};
+
+}  // namespace xdgd
