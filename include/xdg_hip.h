@@ -122,6 +122,15 @@ int xdg_hip_ray_fire(xdg_hip_ctx* ctx, int32_t tree, const int32_t* tree_per_ray
                      const double* dir, const double* tmax, int orientation, const int32_t* excl, int k,
                      int precision, double* t_out, int32_t* surf_out, int32_t* face_out, void* stream);
 
+/* Instrumented xdg_hip_ray_fire (same traversal, no outputs) for roofline
+ * accounting; synchronous.  counters (host, 8): rays, BVH nodes visited,
+ * triangle tests, Pluecker edge tests, tests reaching the distance
+ * computation, normal evaluations, hits, mixed-precision fallbacks to FP64
+ * boxes.  No counterpart in the reference (Embree exposes no counters). */
+int xdg_hip_ray_fire_stats(xdg_hip_ctx* ctx, int32_t tree, const int32_t* tree_per_ray, int64_t n,
+                           const double* org, const double* dir, const double* tmax, int orientation,
+                           const int32_t* excl, int k, int precision, uint64_t* counters);
+
 /* RayTracer::point_in_volume (ray_tracing_interface.h:76-79; Embree
  * ray_tracer.cpp:259-287).  dir NULL -> (1/sqrt 2, 1/sqrt 2, 0). */
 int xdg_hip_point_in_volume(xdg_hip_ctx* ctx, int32_t tree, int64_t n, const double* pts, const double* dir,

@@ -77,6 +77,8 @@ def lib():
             "xdg_hip_num_trees": [_vp, C.POINTER(_i32), C.POINTER(_i32)],
             "xdg_hip_ray_fire": [_vp, _i32, _vp, _i64, _vp, _vp, _vp, C.c_int,
                                  _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp],
+            "xdg_hip_ray_fire_stats": [_vp, _i32, _vp, _i64, _vp, _vp, _vp,
+                                       C.c_int, _vp, C.c_int, C.c_int, _vp],
             "xdg_hip_point_in_volume": [_vp, _i32, _i64, _vp, _vp, _vp, C.c_int,
                                         C.c_int, _vp, _vp],
             "xdg_hip_find_volume": [_vp, C.c_int, _vp, _vp, _i32, _i64, _vp,
@@ -236,6 +238,21 @@ class Context:
             _ptr(tmax), orientation, _ptr(excl), k, precision, _ptr(t),
             _ptr(s), _ptr(f), stream), stream)
         return t, s, f
+
+    @_ordered
+    def ray_fire_stats(self, tree, org, dirs, tmax=None, orientation=EXITING,
+                       excl=None, precision=PLAIN, tree_per_ray=None,
+                       stream=None):
+        """Work counters of the instrumented traversal (synchronous)."""
+        n = org.shape[0]
+        cnt = (C.c_uint64 * 8)()
+        k = 0 if excl is None else excl.shape[1]
+        self._check(lib().xdg_hip_ray_fire_stats(
+            self.h, tree, _ptr(tree_per_ray), n, _ptr(org), _ptr(dirs),
+            _ptr(tmax), orientation, _ptr(excl), k, precision, cnt))
+        keys = ["rays", "nodes", "tri_tests", "edge_tests", "finishes",
+                "normals", "hits", "fallbacks"]
+        return dict(zip(keys, [int(x) for x in cnt]))
 
     @_ordered
     def point_in_volume(self, tree, pts, dirs=None, excl=None, precision=PLAIN,

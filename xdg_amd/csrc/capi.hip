@@ -476,6 +476,40 @@ int xdg_hip_ray_fire(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per_ray, 
   });
 }
 
+int xdg_hip_ray_fire_stats(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per_ray, int64_t n, const double* org,
+                           const double* dir, const double* tmax, int orientation, const int32_t* excl, int k,
+                           int precision, uint64_t* counters)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    require(counters != nullptr, "counters is null");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    require(orientation >= -1 && orientation <= 1, "unknown orientation");
+    require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 8]");
+    int depth = c->max_surf_depth;
+    if (!tree_per_ray) depth = surf_tree(c, tree).dev.depth;
+    for (int j = 0; j < 8; ++j) counters[j] = 0;
+    if (n == 0) return XDG_HIP_OK;
+    require(org && dir, "null array");
+    std::vector<DevBuf> tmp;
+    try {
+      auto* d_cnt = (unsigned long long*)dev_alloc(8 * sizeof(unsigned long long), tmp);
+      XDG_HIP_CHECK(hipMemcpy(d_cnt, counters, 8 * sizeof(uint64_t), hipMemcpyHostToDevice));
+      RayFireArgs a{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir, tmax, orientation,
+                    excl, excl ? k : 0, nullptr, nullptr, nullptr};
+      launch_ray_fire_stats(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), d_cnt, c->stream);
+      XDG_HIP_CHECK(hipGetLastError());
+      XDG_HIP_CHECK(hipStreamSynchronize(c->stream));
+      XDG_HIP_CHECK(hipMemcpy(counters, d_cnt, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    } catch (...) {
+      dev_free_all(tmp);
+      throw;
+    }
+    dev_free_all(tmp);
+    return XDG_HIP_OK;
+  });
+}
+
 int xdg_hip_point_in_volume(xdg_hip_ctx* c, int32_t tree, int64_t n, const double* pts, const double* dir,
                             const int32_t* excl, int k, int precision, uint8_t* inside_out, void* stream)
 {

@@ -70,6 +70,9 @@ struct LaunchCfg {
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
+// counters[8] (device, zeroed): rays, V, F, E, C, N, hits, mixed fallbacks
+void launch_ray_fire_stats(const RayFireArgs& a, bool mixed, const LaunchCfg& c, unsigned long long* counters,
+                           hipStream_t s);
 
 struct PivArgs {
   const xdgd::SurfTreeDev* table;

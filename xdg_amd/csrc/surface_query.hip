@@ -68,6 +68,38 @@ __global__ void k_ray_fire(RayFireArgs a, int cap)
   }
 }
 
+// Instrumented ray_fire: identical traversal, plus per-ray work counters
+// summed into counters[0..7] = rays, V, F, E, C, N, hits, mixed-fallbacks.
+template <bool MIXED>
+__global__ void k_ray_fire_stats(RayFireArgs a, int cap, unsigned long long* counters)
+{
+  extern __shared__ int32_t lds[];
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  int32_t tree = a.tree_per_ray ? a.tree_per_ray[i] : a.tree;
+  if (tree < 1 || tree > a.ntrees) return;
+  const SurfTreeDev& T = a.table[tree];
+  RayD ray = make_ray(a.org + 3 * i, a.dir + 3 * i);
+  int32_t ex[MAXK];
+  int k = a.excl ? (a.k < MAXK ? a.k : MAXK) : 0;
+  for (int j = 0; j < k; ++j) ex[j] = a.excl[(int64_t)a.k * i + j];
+  HitState h;
+  h.t = a.tmax ? a.tmax[i] : INFTY;
+  h.face = INT32_MAX;
+  h.surf = ID_NONE;
+  h.inside = false;
+  TraceStats st{0, 0, 0, 0, 0};
+  trace<RAYFIRE, MIXED, true>(T, ray, a.orientation, ex, k, lds + threadIdx.x, blockDim.x, cap, h, &st);
+  atomicAdd(counters + 0, 1ULL);
+  atomicAdd(counters + 1, (unsigned long long)st.V);
+  atomicAdd(counters + 2, (unsigned long long)st.F);
+  atomicAdd(counters + 3, (unsigned long long)st.E);
+  atomicAdd(counters + 4, (unsigned long long)st.C);
+  atomicAdd(counters + 5, (unsigned long long)st.N);
+  if (h.surf != ID_NONE) atomicAdd(counters + 6, 1ULL);
+  if (MIXED && !mixed_safe(T, ray)) atomicAdd(counters + 7, 1ULL);
+}
+
 template <bool MIXED>
 __global__ void k_piv(PivArgs a, int cap)
 {
@@ -228,6 +260,18 @@ void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipSt
     hipLaunchKernelGGL(k_ray_fire<true>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
   else
     hipLaunchKernelGGL(k_ray_fire<false>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+}
+
+void launch_ray_fire_stats(const RayFireArgs& a, bool mixed, const LaunchCfg& c, unsigned long long* counters,
+                           hipStream_t s)
+{
+  if (a.n <= 0) return;
+  if (mixed)
+    hipLaunchKernelGGL(k_ray_fire_stats<true>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack,
+                       counters);
+  else
+    hipLaunchKernelGGL(k_ray_fire_stats<false>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack,
+                       counters);
 }
 
 void launch_point_in_volume(const PivArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)

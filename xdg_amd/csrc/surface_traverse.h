@@ -72,20 +72,49 @@ struct HitState {
   bool inside;  // PIV: dot(dir, sense-flipped normal) > 0 of the committed hit
 };
 
+// Work counters of the instrumented traversal (roofline accounting): V node
+// visits, F triangle tests, E Pluecker edge tests, C tests reaching the
+// distance computation, N normal evaluations.
+struct TraceStats {
+  unsigned int V, F, E, C, N;
+};
+
+// Counts the edge tests plucker_any performs (same control flow).
+__device__ __forceinline__ void plucker_count(const D3& v0, const D3& v1, const D3& v2, const D3& d, const D3& rb,
+                                              TraceStats& st)
+{
+  const double c0 = edge_test(v0, v1, d, rb);
+  const double c1 = edge_test(v1, v2, d, rb);
+  st.E += 2;
+  if ((0.0 < c0 && 0.0 > c1) || (0.0 > c0 && 0.0 < c1)) return;
+  const double c2 = edge_test(v2, v0, d, rb);
+  st.E += 1;
+  if ((0.0 < c1 && 0.0 > c2) || (0.0 > c1 && 0.0 < c2) || (0.0 < c0 && 0.0 > c2) || (0.0 > c0 && 0.0 < c2))
+    return;
+  if (0.0 == c0 && 0.0 == c1 && 0.0 == c2) return;
+  st.C += 1;
+}
+
 // Leaf triangle for RAYFIRE / PIV: TriangleIntersectionFunc with the tie rule.
-template <int MODE>
+template <int MODE, bool STATS = false>
 __device__ __forceinline__ void tri_closest_hit(const TriRec* __restrict__ tris, int32_t p, const RayD& ray,
-                                                int orientation, const int32_t* ex, int k, HitState& h)
+                                                int orientation, const int32_t* ex, int k, HitState& h,
+                                                TraceStats* st = nullptr)
 {
   const TriRec& r = tris[p];
   const D3 v0 = d3(r.v[0], r.v[1], r.v[2]);
   const D3 v1 = d3(r.v[3], r.v[4], r.v[5]);
   const D3 v2 = d3(r.v[6], r.v[7], r.v[8]);
   double t;
+  if (STATS) {
+    st->F += 1;
+    plucker_count(v0, v1, v2, ray.d, ray.rb, *st);
+  }
   if (!plucker_any(v0, v1, v2, ray.d, ray.rb, ray.idx, ray.oi, ray.di, h.t, 0.0, t)) return;
   const int32_t face = r.face;
   if (t == h.t && face >= h.face) return;  // tie rule
   const int32_t s2 = r.surf2;
+  if (STATS) st->N += 1;
   D3 n = face_normal(v0, v1, v2);
   if (s2 & 1) n = neg(n);
   if (MODE == RAYFIRE && orientation != ORIENT_ANY) {
@@ -113,9 +142,10 @@ __device__ __forceinline__ bool tri_any_hit(const TriRec* __restrict__ tris, int
 }
 
 // Generic traversal.  Returns true for OCCL when any hit is found.
-template <int MODE, bool MIXED>
+template <int MODE, bool MIXED, bool STATS = false>
 __device__ __forceinline__ bool traverse(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
-                                         int k, int32_t* stk, int stride, int depth_cap, HitState& h)
+                                         int k, int32_t* stk, int stride, int depth_cap, HitState& h,
+                                         TraceStats* st = nullptr)
 {
   if (T.n_prims <= 0) return false;
   BoxRayD bd;
@@ -144,6 +174,7 @@ __device__ __forceinline__ bool traverse(const SurfTreeDev& T, const RayD& ray, 
       int32_t c0, c1;
       bool h0, h1;
       bool swap;
+      if (STATS) st->V += 1;
       if (MIXED) {
         const NodeF nd = T.nodes_f[node];
         float t0, t1;
@@ -186,7 +217,7 @@ __device__ __forceinline__ bool traverse(const SurfTreeDev& T, const RayD& ray, 
       if (MODE == OCCL) {
         if (tri_any_hit<MODE>(T.tris, first + j, ray)) return true;
       } else {
-        tri_closest_hit<MODE>(T.tris, first + j, ray, orientation, ex, k, h);
+        tri_closest_hit<MODE, STATS>(T.tris, first + j, ray, orientation, ex, k, h, st);
       }
     }
     if (MODE != OCCL) tf = f32_up(h.t);
@@ -214,14 +245,15 @@ __device__ __forceinline__ bool mixed_safe(const SurfTreeDev& T, const RayD& r)
   return m <= (double)T.o_limit;
 }
 
-template <int MODE, bool MIXED>
+template <int MODE, bool MIXED, bool STATS = false>
 __device__ __forceinline__ bool trace(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
-                                      int k, int32_t* stk, int stride, int cap, HitState& h)
+                                      int k, int32_t* stk, int stride, int cap, HitState& h,
+                                      TraceStats* st = nullptr)
 {
   if (MIXED) {
-    if (mixed_safe(T, ray)) return traverse<MODE, true>(T, ray, orientation, ex, k, stk, stride, cap, h);
+    if (mixed_safe(T, ray)) return traverse<MODE, true, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
   }
-  return traverse<MODE, false>(T, ray, orientation, ex, k, stk, stride, cap, h);
+  return traverse<MODE, false, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
 }
 
 }  // namespace xdgd
