@@ -48,6 +48,8 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=2_000_000)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-tet", action="store_true")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="also run a 256 MiB device copy (scale for PMC byte counters)")
     return p.parse_args()
 
 
@@ -90,6 +92,15 @@ def main():
     dev = torch.device("cuda", 0)
     stream = torch.cuda.Stream(device=dev)
     sh = capi._vp(stream.cuda_stream)
+
+    if args.copy_probe:
+        # a copy of known size: 256 MiB read + 256 MiB written per launch
+        src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---- geometry and trees ------------------------------------------------
     t0 = time.perf_counter()

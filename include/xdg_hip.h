@@ -62,6 +62,9 @@ typedef struct {
   int64_t bytes;     /* device bytes held by the tree */
   double bump;       /* box dilation used */
   double o_limit;    /* mixed traversal is exact for |origin|_inf <= o_limit */
+  int64_t n_nodes4;  /* 4-wide nodes (surface trees) */
+  int32_t stack4;    /* traversal stack entries the 4-wide tree can need */
+  int32_t reserved;
 } xdg_hip_tree_info;
 
 /* ---- lifecycle --------------------------------------------------------- */
@@ -75,6 +78,13 @@ const char* xdg_hip_last_error(const xdg_hip_ctx* ctx);
  * stack: LDS entries per lane, >= the deepest tree, <= 64). */
 int xdg_hip_set_launch(xdg_hip_ctx* ctx, int block, int stack);
 int xdg_hip_get_launch(xdg_hip_ctx* ctx, int* block, int* stack);
+/* Traversal variant: wide != 0 uses the 4-wide tree (default), else the
+ * binary LBVH; defer != 0 (default) traces far-origin mixed ray_fire rays in
+ * a separate FP64 pass; persistent selects, per precision (bit 0 plain,
+ * bit 1 mixed; default 1), ray_fire on a device-resident grid whose lanes
+ * claim new rays from a counter as they finish.  Results are identical
+ * either way. */
+int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent);
 int xdg_hip_synchronize(xdg_hip_ctx* ctx, void* stream);
 
 /* ---- tree construction (host arrays in, device LBVH out) ---------------- */
@@ -123,10 +133,13 @@ int xdg_hip_ray_fire(xdg_hip_ctx* ctx, int32_t tree, const int32_t* tree_per_ray
                      int precision, double* t_out, int32_t* surf_out, int32_t* face_out, void* stream);
 
 /* Instrumented xdg_hip_ray_fire (same traversal, no outputs) for roofline
- * accounting; synchronous.  counters (host, 8): rays, BVH nodes visited,
- * triangle tests, Pluecker edge tests, tests reaching the distance
- * computation, normal evaluations, hits, mixed-precision fallbacks to FP64
- * boxes.  No counterpart in the reference (Embree exposes no counters). */
+ * accounting; synchronous.  counters (host, XDG_HIP_NUM_COUNTERS): rays, BVH
+ * nodes visited, triangle tests, Pluecker edge tests, tests reaching the
+ * distance computation, normal evaluations, hits, mixed-precision fallbacks
+ * to FP64 boxes, node-loop iterations executed per wave (summed over waves),
+ * triangle iterations executed per wave.  No counterpart in the reference
+ * (Embree exposes no counters). */
+#define XDG_HIP_NUM_COUNTERS 10
 int xdg_hip_ray_fire_stats(xdg_hip_ctx* ctx, int32_t tree, const int32_t* tree_per_ray, int64_t n,
                            const double* org, const double* dir, const double* tmax, int orientation,
                            const int32_t* excl, int k, int precision, uint64_t* counters);

@@ -197,22 +197,86 @@ def sphere5(ctx):
     return m, build_tree(ctx, m, 1), O.Tree.from_model(m, 1)
 
 
+# traversal variants: (wide, persistent)
+VARIANTS = [(True, True), (True, False), (False, False)]
+
+
+@pytest.mark.parametrize("wide,persist", VARIANTS)
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("orientation", [0, 1, -1])
-def test_sphere_ray_fire_bitexact(ctx, torch_mod, sphere5, prec, orientation):
+def test_sphere_ray_fire_bitexact(ctx, torch_mod, sphere5, prec, orientation, wide, persist):
     torch = torch_mod
     m, tree, ot = sphere5
+    ctx.set_traversal(wide=wide, defer=True, persistent=persist)
     n = 65536
     org = P.ball_points(1, P.STREAM_POS, n, 9.0)
     dirs = P.iso_dirs(1, P.STREAM_DIR, n)
     if orientation == 1:  # entering hits need origins outside
         org = org * 2.5
-    t, s, f = ctx.ray_fire(tree, dev(torch, org), dev(torch, dirs),
-                           orientation=orientation, precision=prec)
+    try:
+        t, s, f = ctx.ray_fire(tree, dev(torch, org), dev(torch, dirs),
+                               orientation=orientation, precision=prec)
+    finally:
+        ctx.set_traversal()
     rt, rs, rf = ot.ray_fire(org, dirs, orientation=orientation)
     assert np.array_equal(host(s), rs)
     assert np.array_equal(host(f), rf)
     assert np.array_equal(bits(host(t)), bits(rt))
+
+
+@pytest.mark.parametrize("wide", [True, False])
+def test_nested_piv_occluded_variants(ctx, torch_mod, wide):
+    """Binary and 4-wide traversal, plain and mixed, give identical
+    point_in_volume / occluded answers (nested spheres, subdivision 3)."""
+    torch = torch_mod
+    m = G.nested_spheres(3)
+    ctx.set_traversal(wide=wide, defer=True)
+    try:
+        n = 20000
+        pts = P.box_points(21, P.STREAM_POS, n, [-16] * 3, [16] * 3)
+        dirs = P.iso_dirs(21, P.STREAM_DIR, n)
+        for v in m.volumes:
+            tree = build_tree(ctx, m, v)
+            ot = O.Tree.from_model(m, v)
+            ref_in = ot.point_in_volume(pts, dirs)
+            ref_hit, _ = ot.occluded(pts, dirs)
+            for prec in (0, 1):
+                got = host(ctx.point_in_volume(tree, dev(torch, pts), dev(torch, dirs), precision=prec))
+                assert np.array_equal(got.astype(bool), ref_in)
+                hit, _ = ctx.occluded(tree, dev(torch, pts), dev(torch, dirs), precision=prec)
+                assert np.array_equal(host(hit).astype(bool), ref_hit)
+    finally:
+        ctx.set_traversal(wide=True, defer=True)
+
+
+@pytest.mark.parametrize("persist", [True, False])
+def test_mixed_far_origins_use_fp64_pass(ctx, torch_mod, sphere5, persist):
+    """Origins beyond the FP32-rounding limit of the tree (|o| >> o_limit)
+    go through the deferred FP64 pass of the mixed kernel; results must be
+    identical to the oracle, mixed with near origins in one batch."""
+    torch = torch_mod
+    m, tree, ot = sphere5
+    ctx.set_traversal(persistent=persist)
+    lim = ctx.tree_info(tree)["o_limit"]
+    assert 100.0 < lim < 1e7
+    n = 8192
+    dirs = P.iso_dirs(13, P.STREAM_DIR, n)
+    org = P.ball_points(13, P.STREAM_POS, n, 9.0)
+    far = np.arange(n) % 3 == 0
+    org[far] = -dirs[far] * (4.0 * lim) + org[far] * 0.01
+    for orientation in (-1, 1):
+        rt, rs, rf = ot.ray_fire(org, dirs, orientation=orientation)
+        assert (rs[far] >= 0).mean() > 0.9
+        for prec in (0, 1):
+            t, s, f = ctx.ray_fire(tree, dev(torch, org), dev(torch, dirs),
+                                   orientation=orientation, precision=prec)
+            assert np.array_equal(host(s), rs)
+            assert np.array_equal(host(f), rf)
+            assert np.array_equal(bits(host(t)), bits(rt))
+        st = ctx.ray_fire_stats(tree, dev(torch, org), dev(torch, dirs),
+                                orientation=orientation, precision=1)
+        assert st["fallbacks"] == int(far.sum())
+    ctx.set_traversal()
 
 
 def test_sphere_limits_and_exclusions(ctx, torch_mod, sphere5):
@@ -227,11 +291,15 @@ def test_sphere_limits_and_exclusions(ctx, torch_mod, sphere5):
     tm[::2] = np.nextafter(rt[::2], 0)
     excl = np.full((n, 2), -1, np.int32)
     excl[::3, 0] = rf[::3]
-    for prec in (0, 1):
-        t, s, f = ctx.ray_fire(tree, dev(torch, org), dev(torch, dirs),
-                               tmax=dev(torch, tm), excl=dev(torch, excl),
-                               precision=prec)
-        et, es, ef = ot.ray_fire(org, dirs, tmax=tm, excl=excl)
+    et, es, ef = ot.ray_fire(org, dirs, tmax=tm, excl=excl)
+    for prec, persist in [(0, True), (1, True), (0, False), (1, False)]:
+        ctx.set_traversal(persistent=persist)
+        try:
+            t, s, f = ctx.ray_fire(tree, dev(torch, org), dev(torch, dirs),
+                                   tmax=dev(torch, tm), excl=dev(torch, excl),
+                                   precision=prec)
+        finally:
+            ctx.set_traversal()
         assert np.array_equal(host(s), es)
         assert np.array_equal(host(f), ef)
         assert np.array_equal(bits(host(t)), bits(et))

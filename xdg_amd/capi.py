@@ -26,6 +26,7 @@ ERR_CAPPED = -5
 
 PLAIN = 0
 MIXED = 1
+PERSIST_DEFAULT = 1  # persistent ray_fire for the plain kernel only
 ANY = -1
 EXITING = 0
 ENTERING = 1
@@ -48,7 +49,8 @@ class TreeInfo(C.Structure):
     _fields_ = [("n_prims", C.c_int64), ("n_nodes", C.c_int64),
                 ("depth", C.c_int32), ("root", C.c_int32),
                 ("bytes", C.c_int64), ("bump", C.c_double),
-                ("o_limit", C.c_double)]
+                ("o_limit", C.c_double), ("n_nodes4", C.c_int64),
+                ("stack4", C.c_int32), ("reserved", C.c_int32)]
 
 
 _LIB = None
@@ -66,6 +68,7 @@ def lib():
             "xdg_hip_create": [C.POINTER(_vp), C.c_int],
             "xdg_hip_destroy": [_vp],
             "xdg_hip_set_launch": [_vp, C.c_int, C.c_int],
+            "xdg_hip_set_traversal": [_vp, C.c_int, C.c_int, C.c_int],
             "xdg_hip_get_launch": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "xdg_hip_synchronize": [_vp, _vp],
             "xdg_hip_build_surface_tree": [_vp, _i64, _vp, _i64, _vp, _vp, _vp,
@@ -174,6 +177,17 @@ class Context:
     def set_launch(self, block, stack):
         self._check(lib().xdg_hip_set_launch(self.h, block, stack))
 
+    def set_traversal(self, wide=True, defer=True, persistent=None):
+        """persistent: None for the default (plain only), True / False for
+        both precisions, or the raw mask (bit 0 plain, bit 1 mixed)."""
+        if persistent is None:
+            mask = PERSIST_DEFAULT
+        elif persistent is True or persistent is False:
+            mask = 3 if persistent else 0
+        else:
+            mask = int(persistent)
+        self._check(lib().xdg_hip_set_traversal(self.h, int(wide), int(defer), mask))
+
     def get_launch(self):
         b, s = C.c_int(), C.c_int()
         self._check(lib().xdg_hip_get_launch(self.h, C.byref(b), C.byref(s)))
@@ -245,13 +259,14 @@ class Context:
                        stream=None):
         """Work counters of the instrumented traversal (synchronous)."""
         n = org.shape[0]
-        cnt = (C.c_uint64 * 8)()
+        cnt = (C.c_uint64 * 10)()
         k = 0 if excl is None else excl.shape[1]
         self._check(lib().xdg_hip_ray_fire_stats(
             self.h, tree, _ptr(tree_per_ray), n, _ptr(org), _ptr(dirs),
             _ptr(tmax), orientation, _ptr(excl), k, precision, cnt))
         keys = ["rays", "nodes", "tri_tests", "edge_tests", "finishes",
-                "normals", "hits", "fallbacks"]
+                "normals", "hits", "fallbacks", "wave_node_iters",
+                "wave_tri_iters"]
         return dict(zip(keys, [int(x) for x in cnt]))
 
     @_ordered

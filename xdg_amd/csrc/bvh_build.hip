@@ -447,3 +447,177 @@ void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std:
 }
 
 }  // namespace xdgh
+
+// ---------------------------------------------------------------------------
+// BVH4 collapse (host): each 4-wide node takes a binary node's two children
+// and repeatedly opens the internal child with the largest surface area until
+// it holds four children (or only leaves remain).  Nodes are emitted
+// breadth-first so that the children of a node are contiguous.  Leaf codes
+// are unchanged, so the BVH4 addresses the same leaf-ordered primitives.
+// ---------------------------------------------------------------------------
+namespace xdgh {
+
+namespace {
+
+struct Cand {
+  int32_t code;
+  double box[6];
+  float fbox[6];
+};
+
+double half_area(const double* b)
+{
+  const double dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+  return dx * dy + dy * dz + dz * dx;
+}
+
+}  // namespace
+
+void bvh4_collapse(const LbvhOut& bin, int64_t n, std::vector<DevBuf>& owned, Bvh4Out& out)
+{
+  using namespace xdgd;
+  const int32_t EMPTY = INT32_MIN;
+  if (bin.root < 0 || bin.n_nodes == 0) {  // a single leaf: no nodes
+    out.root = bin.root;
+    out.stack = 0;
+    out.n_nodes = 0;
+    out.nodes_d = (Node4D*)dev_alloc(sizeof(Node4D), owned);
+    out.nodes_f = (Node4F*)dev_alloc(sizeof(Node4F), owned);
+    return;
+  }
+  std::vector<NodeD> bd((size_t)bin.n_nodes);
+  std::vector<NodeF> bf((size_t)bin.n_nodes);
+  XDG_HIP_CHECK(hipMemcpy(bd.data(), bin.nodes_d, sizeof(NodeD) * bin.n_nodes, hipMemcpyDeviceToHost));
+  XDG_HIP_CHECK(hipMemcpy(bf.data(), bin.nodes_f, sizeof(NodeF) * bin.n_nodes, hipMemcpyDeviceToHost));
+
+  std::vector<Node4D> od;
+  std::vector<Node4F> of;
+  od.reserve((size_t)bin.n_nodes / 2 + 1);
+  of.reserve((size_t)bin.n_nodes / 2 + 1);
+  // work item: binary node index -> BVH4 slot to fill, and the stack entries
+  // pushed above it on the worst path
+  struct Item {
+    int32_t bnode;
+    int32_t slot;
+    int32_t above;
+  };
+  std::vector<Item> queue;
+  od.emplace_back();
+  of.emplace_back();
+  queue.push_back({bin.root, 0, 0});
+  int32_t max_stack = 0;
+  for (size_t q = 0; q < queue.size(); ++q) {
+    const Item it = queue[q];
+    const NodeD& b = bd[it.bnode];
+    const NodeF& f = bf[it.bnode];
+    std::vector<Cand> cs(2);
+    cs[0].code = b.c0;
+    cs[1].code = b.c1;
+    for (int k = 0; k < 6; ++k) {
+      cs[0].box[k] = b.b0[k];
+      cs[1].box[k] = b.b1[k];
+      cs[0].fbox[k] = f.b0[k];
+      cs[1].fbox[k] = f.b1[k];
+    }
+    while (cs.size() < 4) {
+      int best = -1;
+      double area = -1.0;
+      for (int j = 0; j < (int)cs.size(); ++j)
+        if (cs[j].code >= 0 && half_area(cs[j].box) > area) {
+          area = half_area(cs[j].box);
+          best = j;
+        }
+      if (best < 0) break;
+      const NodeD& cb = bd[cs[best].code];
+      const NodeF& cf = bf[cs[best].code];
+      Cand a, c;
+      a.code = cb.c0;
+      c.code = cb.c1;
+      for (int k = 0; k < 6; ++k) {
+        a.box[k] = cb.b0[k];
+        c.box[k] = cb.b1[k];
+        a.fbox[k] = cf.b0[k];
+        c.fbox[k] = cf.b1[k];
+      }
+      cs[best] = a;
+      cs.push_back(c);
+    }
+    const int32_t nc = (int32_t)cs.size();
+    const int32_t above = it.above + (nc - 1);
+    max_stack = std::max(max_stack, above);
+    Node4D nd;
+    Node4F nf;
+    std::memset((void*)&nd, 0, sizeof nd);
+    std::memset((void*)&nf, 0, sizeof nf);
+    for (int j = 0; j < 4; ++j) {
+      if (j < nc) {
+        nd.lox[j] = cs[j].box[0];
+        nd.loy[j] = cs[j].box[1];
+        nd.loz[j] = cs[j].box[2];
+        nd.hix[j] = cs[j].box[3];
+        nd.hiy[j] = cs[j].box[4];
+        nd.hiz[j] = cs[j].box[5];
+        nf.lox[j] = cs[j].fbox[0];
+        nf.loy[j] = cs[j].fbox[1];
+        nf.loz[j] = cs[j].fbox[2];
+        nf.hix[j] = cs[j].fbox[3];
+        nf.hiy[j] = cs[j].fbox[4];
+        nf.hiz[j] = cs[j].fbox[5];
+      } else {
+        nd.lox[j] = nd.loy[j] = nd.loz[j] = 0.0;
+        nd.hix[j] = nd.hiy[j] = nd.hiz[j] = 0.0;
+        nf.lox[j] = nf.loy[j] = nf.loz[j] = 0.0f;
+        nf.hix[j] = nf.hiy[j] = nf.hiz[j] = 0.0f;
+        nd.child[j] = nf.child[j] = EMPTY;
+      }
+    }
+    for (int j = 0; j < nc; ++j) {
+      if (cs[j].code >= 0) {
+        const int32_t slot = (int32_t)od.size();
+        od.emplace_back();
+        of.emplace_back();
+        queue.push_back({cs[j].code, slot, above});
+        nd.child[j] = nf.child[j] = slot;
+      } else {
+        nd.child[j] = nf.child[j] = cs[j].code;
+      }
+    }
+    od[it.slot] = nd;
+    of[it.slot] = nf;
+  }
+  // structural check before any kernel reads the BVH4: child indices in
+  // range, every primitive covered exactly once
+  {
+    std::vector<uint8_t> seen((size_t)n, 0);
+    int64_t covered = 0;
+    for (const Node4D& nd : od) {
+      for (int j = 0; j < 4; ++j) {
+        const int32_t c = nd.child[j];
+        if (c == EMPTY) continue;
+        if (c >= 0) {
+          if ((size_t)c >= od.size()) throw HipError{"bvh4: child index out of range"};
+          continue;
+        }
+        const int64_t f = leaf_first(c), k = leaf_count(c);
+        if (f < 0 || f + k > n) throw HipError{"bvh4: leaf range out of bounds"};
+        for (int64_t p = f; p < f + k; ++p) {
+          if (seen[p]) throw HipError{"bvh4: primitive covered twice"};
+          seen[p] = 1;
+        }
+        covered += k;
+      }
+    }
+    if (covered != n) throw HipError{"bvh4: not every primitive is reachable"};
+  }
+  // the stack never holds more than the pushes along one root-to-leaf path
+  out.stack = std::max(max_stack, 1);
+  out.root = 0;
+  out.n_nodes = (int64_t)od.size();
+  out.nodes_d = (Node4D*)dev_alloc(sizeof(Node4D) * od.size(), owned);
+  out.nodes_f = (Node4F*)dev_alloc(sizeof(Node4F) * of.size(), owned);
+  XDG_HIP_CHECK(hipMemcpy(out.nodes_d, od.data(), sizeof(Node4D) * od.size(), hipMemcpyHostToDevice));
+  XDG_HIP_CHECK(hipMemcpy(out.nodes_f, of.data(), sizeof(Node4F) * of.size(), hipMemcpyHostToDevice));
+  (void)n;
+}
+
+}  // namespace xdgh

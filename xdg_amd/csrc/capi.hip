@@ -24,6 +24,7 @@ struct SurfTreeHost {
   std::vector<DevBuf> bufs;
   int32_t min_face = 0, max_face = -1;
   int64_t n_nodes = 0;
+  int64_t n_nodes4 = 0;
   int64_t bytes = 0;
 };
 
@@ -73,9 +74,14 @@ struct xdg_hip_ctx {
   size_t small_cap = 0;
   std::vector<int32_t> small_cache;
   uint32_t* d_status = nullptr;
+  int32_t* d_fb_list = nullptr;  // mixed ray_fire: rays deferred to the FP64 pass
+  size_t fb_cap = 0;
+  unsigned int* d_fb_count = nullptr;
+  unsigned long long* d_next = nullptr;  // persistent ray_fire: ray counter
   LaunchCfg cfg;
   int leaf_max = 4;
   int max_surf_depth = 1;
+  int max_surf_stack4 = 1;
   int max_elem_depth = 1;
 };
 
@@ -128,6 +134,14 @@ const ElemTreeHost& elem_tree(xdg_hip_ctx* c, int32_t tree)
 
 hipStream_t pick(xdg_hip_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
 
+// traversal stack entries a surface query needs: one tree, or every tree
+// (per-ray trees)
+int surf_need(xdg_hip_ctx* c, const SurfTreeHost* t)
+{
+  if (c->cfg.wide) return t ? t->dev.stack4 : c->max_surf_stack4;
+  return t ? t->dev.depth : c->max_surf_depth;
+}
+
 LaunchCfg cfg_for(xdg_hip_ctx* c, int depth)
 {
   LaunchCfg k = c->cfg;
@@ -178,13 +192,18 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   auto* c = new xdg_hip_ctx();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&c->d_status, sizeof(uint32_t) * 4) != hipSuccess) {
+      hipMalloc(&c->d_status, sizeof(uint32_t) * 4) != hipSuccess ||
+      hipMalloc(&c->d_fb_count, sizeof(unsigned int) * 4) != hipSuccess ||
+      hipMalloc(&c->d_next, sizeof(unsigned long long) * 2) != hipSuccess) {
     delete c;
     return XDG_HIP_ERR_HIP;
   }
   c->cfg.block = env_int("XDG_HIP_BLOCK", 256);
   c->cfg.stack = env_int("XDG_HIP_STACK", 1);
   c->leaf_max = env_int("XDG_HIP_LEAF", 4);
+  c->cfg.defer = env_int("XDG_HIP_DEFER", 1) != 0;
+  c->cfg.wide = env_int("XDG_HIP_WIDE", 1) != 0;
+  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 1) & 3;
   try {
     sync_tables(c);
   } catch (...) {
@@ -208,6 +227,9 @@ int xdg_hip_destroy(xdg_hip_ctx* c)
     if (c->d_elem) (void)hipFree(c->d_elem);
     if (c->d_small) (void)hipFree(c->d_small);
     if (c->d_status) (void)hipFree(c->d_status);
+    if (c->d_fb_list) (void)hipFree(c->d_fb_list);
+    if (c->d_fb_count) (void)hipFree(c->d_fb_count);
+    if (c->d_next) (void)hipFree(c->d_next);
     if (c->stream) (void)hipStreamDestroy(c->stream);
   }
   delete c;
@@ -223,6 +245,16 @@ int xdg_hip_set_launch(xdg_hip_ctx* c, int block, int stack)
     require(stack >= 1 && stack <= 64, "stack must be in [1, 64]");
     c->cfg.block = block;
     c->cfg.stack = stack;
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_set_traversal(xdg_hip_ctx* c, int wide, int defer, int persistent)
+{
+  return guarded(c, [&]() {
+    c->cfg.wide = wide != 0;
+    c->cfg.defer = defer != 0;
+    c->cfg.persistent = persistent & 3;
     return XDG_HIP_OK;
   });
 }
@@ -289,8 +321,18 @@ int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
       lbvh_build(d_box, n_tris, c->leaf_max, s, t.bufs, lb);
       TriRec* d_tris = (TriRec*)dev_alloc(nt * sizeof(TriRec), t.bufs);
       launch_write_tris(d_v9, d_face, d_surf, d_flip, lb.order, n_tris, d_tris, s);
+      double* d_nrm = (double*)dev_alloc(nt * 3 * sizeof(double), t.bufs);
+      launch_tri_normals(d_tris, n_tris, d_nrm, s);
+      t.dev.normals = d_nrm;
       XDG_HIP_CHECK(hipGetLastError());
       XDG_HIP_CHECK(hipStreamSynchronize(s));
+      Bvh4Out b4;
+      bvh4_collapse(lb, n_tris, t.bufs, b4);
+      t.dev.nodes4_d = b4.nodes_d;
+      t.dev.nodes4_f = b4.nodes_f;
+      t.dev.root4 = b4.root;
+      t.dev.stack4 = b4.stack;
+      t.n_nodes4 = b4.n_nodes;
       t.dev.nodes_d = lb.nodes_d;
       t.dev.nodes_f = lb.nodes_f;
       t.dev.tris = d_tris;
@@ -313,6 +355,7 @@ int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
     dev_free_all(tmp);
     c->surf.push_back(std::move(t));
     c->max_surf_depth = std::max(c->max_surf_depth, c->surf.back().dev.depth);
+    c->max_surf_stack4 = std::max(c->max_surf_stack4, c->surf.back().dev.stack4);
     try {
       sync_tables(c);
     } catch (...) {
@@ -441,6 +484,8 @@ int xdg_hip_get_tree_info(xdg_hip_ctx* c, int32_t tree, int element, xdg_hip_tre
       info->bytes = t.bytes;
       info->bump = t.dev.bump;
       info->o_limit = t.dev.o_limit;
+      info->n_nodes4 = t.n_nodes4;
+      info->stack4 = t.dev.stack4;
     }
     return XDG_HIP_OK;
   });
@@ -464,12 +509,24 @@ int xdg_hip_ray_fire(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per_ray, 
     require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
     require(orientation >= -1 && orientation <= 1, "unknown orientation");
     require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 8]");
-    int depth = c->max_surf_depth;
-    if (!tree_per_ray) depth = surf_tree(c, tree).dev.depth;
+    int depth = tree_per_ray ? surf_need(c, nullptr) : surf_need(c, &surf_tree(c, tree));
     if (n == 0) return XDG_HIP_OK;
     require(org && dir && t_out && surf_out, "null array");
     RayFireArgs a{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir, tmax, orientation,
-                  excl, excl ? k : 0, t_out, surf_out, face_out};
+                  excl, excl ? k : 0, t_out, surf_out, face_out, nullptr, nullptr};
+    if (precision == XDG_HIP_MIXED && c->cfg.defer) {
+      if ((size_t)n > c->fb_cap) {
+        XDG_HIP_CHECK(hipDeviceSynchronize());
+        if (c->d_fb_list) XDG_HIP_CHECK(hipFree(c->d_fb_list));
+        c->d_fb_list = nullptr;
+        c->fb_cap = 0;
+        XDG_HIP_CHECK(hipMalloc(&c->d_fb_list, sizeof(int32_t) * (size_t)n));
+        c->fb_cap = (size_t)n;
+      }
+      a.fb_list = c->d_fb_list;
+      a.fb_count = c->d_fb_count;
+    }
+    a.next_ray = c->d_next;
     launch_ray_fire(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
@@ -486,21 +543,20 @@ int xdg_hip_ray_fire_stats(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per
     require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
     require(orientation >= -1 && orientation <= 1, "unknown orientation");
     require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 8]");
-    int depth = c->max_surf_depth;
-    if (!tree_per_ray) depth = surf_tree(c, tree).dev.depth;
-    for (int j = 0; j < 8; ++j) counters[j] = 0;
+    int depth = tree_per_ray ? surf_need(c, nullptr) : surf_need(c, &surf_tree(c, tree));
+    for (int j = 0; j < XDG_HIP_NUM_COUNTERS; ++j) counters[j] = 0;
     if (n == 0) return XDG_HIP_OK;
     require(org && dir, "null array");
     std::vector<DevBuf> tmp;
     try {
-      auto* d_cnt = (unsigned long long*)dev_alloc(8 * sizeof(unsigned long long), tmp);
-      XDG_HIP_CHECK(hipMemcpy(d_cnt, counters, 8 * sizeof(uint64_t), hipMemcpyHostToDevice));
+      auto* d_cnt = (unsigned long long*)dev_alloc(XDG_HIP_NUM_COUNTERS * sizeof(unsigned long long), tmp);
+      XDG_HIP_CHECK(hipMemcpy(d_cnt, counters, XDG_HIP_NUM_COUNTERS * sizeof(uint64_t), hipMemcpyHostToDevice));
       RayFireArgs a{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir, tmax, orientation,
                     excl, excl ? k : 0, nullptr, nullptr, nullptr};
       launch_ray_fire_stats(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), d_cnt, c->stream);
       XDG_HIP_CHECK(hipGetLastError());
       XDG_HIP_CHECK(hipStreamSynchronize(c->stream));
-      XDG_HIP_CHECK(hipMemcpy(counters, d_cnt, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+      XDG_HIP_CHECK(hipMemcpy(counters, d_cnt, XDG_HIP_NUM_COUNTERS * sizeof(uint64_t), hipMemcpyDeviceToHost));
     } catch (...) {
       dev_free_all(tmp);
       throw;
@@ -534,7 +590,7 @@ int xdg_hip_point_in_volume(xdg_hip_ctx* c, int32_t tree, int64_t n, const doubl
     a.excl = excl;
     a.k = excl ? k : 0;
     a.inside = inside_out;
-    launch_point_in_volume(a, precision == XDG_HIP_MIXED, cfg_for(c, t.dev.depth), pick(c, stream));
+    launch_point_in_volume(a, precision == XDG_HIP_MIXED, cfg_for(c, surf_need(c, &t)), pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
   });
@@ -551,7 +607,7 @@ int xdg_hip_find_volume(xdg_hip_ctx* c, int nvol, const int32_t* vol_trees, cons
     int depth = 1;
     for (int v = 0; v < nvol; ++v) {
       if (vol_ids[v] == ipc_volume) continue;
-      depth = std::max(depth, surf_tree(c, vol_trees[v]).dev.depth);
+      depth = std::max(depth, surf_need(c, &surf_tree(c, vol_trees[v])));
     }
     if (n == 0) return XDG_HIP_OK;
     require(pts && vol_out, "null array");
@@ -600,7 +656,7 @@ int xdg_hip_occluded(xdg_hip_ctx* c, int32_t tree, int64_t n, const double* org,
     if (n == 0) return XDG_HIP_OK;
     require(org && dir && hit_out, "null array");
     OccludedArgs a{c->d_surf, (int32_t)c->surf.size(), tree, n, org, dir, hit_out, dist_out};
-    launch_occluded(a, precision == XDG_HIP_MIXED, cfg_for(c, t.dev.depth), pick(c, stream));
+    launch_occluded(a, precision == XDG_HIP_MIXED, cfg_for(c, surf_need(c, &t)), pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
   });

@@ -46,6 +46,16 @@ struct LbvhOut {
 void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std::vector<DevBuf>& owned,
                 LbvhOut& out);
 
+// 4-wide tree over the same leaves, collapsed from a binary LBVH.
+struct Bvh4Out {
+  xdgd::Node4D* nodes_d = nullptr;
+  xdgd::Node4F* nodes_f = nullptr;
+  int32_t root = 0;
+  int32_t stack = 0;  // max stack entries any traversal can need
+  int64_t n_nodes = 0;
+};
+void bvh4_collapse(const LbvhOut& bin, int64_t n, std::vector<DevBuf>& owned, Bvh4Out& out);
+
 // ---- surface queries (surface_query.hip) ---------------------------------
 struct RayFireArgs {
   const xdgd::SurfTreeDev* table;
@@ -62,15 +72,28 @@ struct RayFireArgs {
   double* t_out;
   int32_t* surf_out;
   int32_t* face_out;
+  // mixed precision only (nullable): rays whose origin lies beyond the tree's
+  // FP32-rounding limit are queued here and traced by a second FP64 pass,
+  // which keeps FP64 box state out of the mixed kernel's registers
+  int32_t* fb_list;
+  unsigned int* fb_count;
+  // persistent kernels (nullable): next unclaimed ray of the batch
+  unsigned long long* next_ray;
 };
 
 struct LaunchCfg {
   int block = 256;
-  int stack = 64;  // LDS stack entries per lane
+  int stack = 64;     // LDS stack entries per lane
+  bool defer = true;  // mixed ray_fire: FP64 pass for far-origin rays
+  bool wide = true;   // surface queries traverse the BVH4
+  // ray_fire on a resident grid with per-lane ray refill, per precision:
+  // bit 0 plain, bit 1 mixed
+  int persistent = 1;
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
-// counters[8] (device, zeroed): rays, V, F, E, C, N, hits, mixed fallbacks
+// counters[10] (device, zeroed): rays, V, F, E, C, N, hits, mixed fallbacks,
+// wave node iterations, wave triangle iterations
 void launch_ray_fire_stats(const RayFireArgs& a, bool mixed, const LaunchCfg& c, unsigned long long* counters,
                            hipStream_t s);
 
@@ -178,6 +201,8 @@ void launch_eval(int kind, int64_t n, const double* in, double* out, hipStream_t
 void launch_elem_boxes(const xdgd::ElemRec* elems, int64_t n, double* box6, hipStream_t s);
 // triangle prim boxes (+- bump, triangle_intersect.cpp:36-50)
 void launch_tri_boxes(const double* verts9, int64_t n, double bump, double* box6, hipStream_t s);
+// per-triangle face normals (leaf order, 3 doubles each)
+void launch_tri_normals(const xdgd::TriRec* tris, int64_t n, double* out, hipStream_t s);
 // leaf-ordered triangle records
 void launch_write_tris(const double* verts9, const int32_t* face, const int32_t* surf, const uint8_t* flip,
                        const int32_t* order, int64_t n, xdgd::TriRec* out, hipStream_t s);

@@ -14,13 +14,16 @@
 // one deterministic tie rule: among bitwise-equal distances the smallest
 // face id wins (the reference keeps the last candidate Embree visits).
 //
-// Traversal: one ray per lane, wave64; BVH2 nodes carry both child boxes;
-// near child first, far child pushed on a per-lane stack in LDS laid out
-// [slot][lane] (conflict-free).  PLAIN tests FP64 boxes; MIXED tests the
-// outward-rounded FP32 boxes with an FP32 ray and keeps FP64 only for the
-// Pluecker test, the distance and the normal/parity decision.  MIXED falls
-// back to the FP64 boxes for a ray whose origin lies beyond the tree's
-// conservative-rounding limit.
+// Traversal: one ray per lane, wave64, over the BVH4 (WIDE, default) or the
+// binary LBVH; near child first, others pushed on a per-lane stack in LDS
+// laid out [slot][lane] (conflict-free).  PLAIN tests FP64 boxes; MIXED
+// tests the outward-rounded FP32 boxes with an FP32 ray and keeps FP64 only
+// for the Pluecker test, the distance and the normal/parity decision.  A
+// mixed ray whose origin lies beyond the tree's conservative-rounding limit
+// is traced with FP64 boxes (for ray_fire: queued to a second pass so that
+// the mixed kernel carries no FP64 box state).
+#include <algorithm>
+
 #include "internal.h"
 #include "surface_traverse.h"
 
@@ -32,12 +35,11 @@ namespace {
 
 constexpr int MAXK = 8;
 
-template <bool MIXED>
-__global__ void k_ray_fire(RayFireArgs a, int cap)
+// One ray of ray_fire.  DEFER (mixed only): a ray beyond the FP32-rounding
+// limit is queued for the FP64 pass instead of traced here.
+template <bool MIXED, bool DEFER, bool WIDE>
+__device__ __forceinline__ void ray_fire_one(const RayFireArgs& a, int64_t i, int32_t* stk, int cap)
 {
-  extern __shared__ int32_t lds[];
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
   int32_t tree = a.tree_per_ray ? a.tree_per_ray[i] : a.tree;
   double* tout = a.t_out;
   if (tree < 1 || tree > a.ntrees) {
@@ -48,6 +50,11 @@ __global__ void k_ray_fire(RayFireArgs a, int cap)
   }
   const SurfTreeDev& T = a.table[tree];
   RayD ray = make_ray(a.org + 3 * i, a.dir + 3 * i);
+  if (DEFER && !mixed_safe(T, ray)) {
+    const unsigned int j = atomicAdd(a.fb_count, 1u);
+    a.fb_list[j] = (int32_t)i;
+    return;
+  }
   int32_t ex[MAXK];
   int k = a.excl ? (a.k < MAXK ? a.k : MAXK) : 0;
   for (int j = 0; j < k; ++j) ex[j] = a.excl[(int64_t)a.k * i + j];
@@ -56,7 +63,10 @@ __global__ void k_ray_fire(RayFireArgs a, int cap)
   h.face = INT32_MAX;
   h.surf = ID_NONE;
   h.inside = false;
-  trace<RAYFIRE, MIXED>(T, ray, a.orientation, ex, k, lds + threadIdx.x, blockDim.x, cap, h);
+  if (DEFER)
+    trace_f32<RAYFIRE, false, WIDE>(T, ray, a.orientation, ex, k, stk, blockDim.x, cap, h);
+  else
+    trace<RAYFIRE, MIXED, false, WIDE>(T, ray, a.orientation, ex, k, stk, blockDim.x, cap, h);
   if (h.surf == ID_NONE) {
     tout[i] = INFTY;
     a.surf_out[i] = ID_NONE;
@@ -68,9 +78,159 @@ __global__ void k_ray_fire(RayFireArgs a, int cap)
   }
 }
 
+template <bool MIXED, bool DEFER, bool WIDE>
+__global__ void k_ray_fire(RayFireArgs a, int cap)
+{
+  extern __shared__ int32_t lds[];
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  ray_fire_one<MIXED, DEFER, WIDE>(a, i, lds + threadIdx.x, cap);
+}
+
+// FP64 pass over the rays the mixed kernel queued (grid-stride; the count is
+// read on the device, so no host round trip separates the two passes).
+template <bool WIDE>
+__global__ void k_ray_fire_deferred(RayFireArgs a, int cap)
+{
+  extern __shared__ int32_t lds[];
+  const unsigned int cnt = *a.fb_count;
+  for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x)
+    ray_fire_one<false, false, WIDE>(a, a.fb_list[j], lds + threadIdx.x, cap);
+}
+
+__global__ void k_zero_u32(unsigned int* p) { *p = 0u; }
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+__device__ __forceinline__ void write_hit(const RayFireArgs& a, int64_t i, const HitState& h)
+{
+  if (h.surf == ID_NONE) {
+    a.t_out[i] = INFTY;
+    a.surf_out[i] = ID_NONE;
+    if (a.face_out) a.face_out[i] = ID_NONE;
+  } else {
+    a.t_out[i] = h.t;
+    a.surf_out[i] = h.surf;
+    if (a.face_out) a.face_out[i] = h.face;
+  }
+}
+
+// Persistent ray_fire over the BVH4.  A fixed grid of resident waves works
+// through the batch: a lane whose ray is finished writes its result and
+// takes the next unclaimed ray (one 64-bit atomic per wave per refill), so a
+// wave does not idle behind its slowest lane.  MIXED queues far-origin rays
+// for the FP64 pass (k_ray_fire_deferred).  Every lane leaves the loop once
+// the counter passes n, so the grid always drains.
+template <bool MIXED>
+__global__ void k_ray_fire_pt(RayFireArgs a, int cap, unsigned long long* next_ray)
+{
+  extern __shared__ int32_t lds[];
+  LdsStack S(lds + threadIdx.x, blockDim.x, cap);
+  const int lane = threadIdx.x & 63;
+  const int32_t DONE = CHILD_EMPTY;
+  int64_t ri = -1;
+  bool have = false;  // ri's result is pending
+  TreeView T{nullptr, nullptr, nullptr, nullptr};
+  RayD ray;
+  BoxRayF bf{0, 0, 0, 0, 0, 0};
+  BoxRayD bd{0, 0, 0, 0, 0, 0};
+  HitState h{INFTY, INT32_MAX, ID_NONE, false};
+  const int32_t* ex = nullptr;
+  int k = 0;
+  float tf = 0.0f;
+  int32_t node = DONE;
+  while (true) {
+    const bool need = node == DONE;
+    if (need && have) {
+      write_hit(a, ri, h);
+      have = false;
+    }
+    const unsigned long long mask = __ballot(need);
+    unsigned long long base = 0;
+    if (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long b = 0;
+      if (lane == leader) b = atomicAdd(next_ray, (unsigned long long)__popcll(mask));
+      base = __shfl(b, leader);
+    }
+    if (need) {
+      ri = (int64_t)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
+      if (ri >= a.n) break;
+      const int32_t tree = a.tree_per_ray ? a.tree_per_ray[ri] : a.tree;
+      if (tree < 1 || tree > a.ntrees) {
+        HitState miss{INFTY, INT32_MAX, ID_NONE, false};
+        write_hit(a, ri, miss);
+        continue;
+      }
+      const SurfTreeDev& TT = a.table[tree];
+      ray = make_ray(a.org + 3 * ri, a.dir + 3 * ri);
+      if (MIXED && !mixed_safe(TT, ray)) {
+        const unsigned int j = atomicAdd(a.fb_count, 1u);
+        a.fb_list[j] = (int32_t)ri;
+        continue;
+      }
+      T.nodes4_d = TT.nodes4_d;
+      T.nodes4_f = TT.nodes4_f;
+      T.tris = TT.tris;
+      T.normals = TT.normals;
+      ex = a.excl ? a.excl + (int64_t)a.k * ri : nullptr;
+      k = a.excl ? (a.k < MAXK ? a.k : MAXK) : 0;
+      h.t = a.tmax ? a.tmax[ri] : INFTY;
+      h.face = INT32_MAX;
+      h.surf = ID_NONE;
+      h.inside = false;
+      tf = f32_up(h.t);
+      if (MIXED) {
+        const float ox = (float)ray.o.x, oy = (float)ray.o.y, oz = (float)ray.o.z;
+        bf.ix = safe_inv_f((float)ray.d.x);
+        bf.iy = safe_inv_f((float)ray.d.y);
+        bf.iz = safe_inv_f((float)ray.d.z);
+        bf.nx = -ox * bf.ix;
+        bf.ny = -oy * bf.iy;
+        bf.nz = -oz * bf.iz;
+      } else {
+        bd.ix = safe_inv_d(ray.d.x);
+        bd.iy = safe_inv_d(ray.d.y);
+        bd.iz = safe_inv_d(ray.d.z);
+        bd.nx = -ray.o.x * bd.ix;
+        bd.ny = -ray.o.y * bd.iy;
+        bd.nz = -ray.o.z * bd.iz;
+      }
+      S.reset();
+      have = true;
+      node = TT.n_prims > 0 ? TT.root4 : DONE;
+      if (node == DONE) continue;
+    }
+    while (node >= 0) {
+      float te[4];
+      int32_t ch[4];
+      node4_test<MIXED>(T, node, bf, bd, tf, h.t, te, ch);
+      cswap(te[0], ch[0], te[1], ch[1]);
+      cswap(te[2], ch[2], te[3], ch[3]);
+      cswap(te[0], ch[0], te[2], ch[2]);
+      cswap(te[1], ch[1], te[3], ch[3]);
+      cswap(te[1], ch[1], te[2], ch[2]);
+      if (ch[0] != CHILD_EMPTY) {
+        if (ch[3] != CHILD_EMPTY) S.push(ch[3]);
+        if (ch[2] != CHILD_EMPTY) S.push(ch[2]);
+        if (ch[1] != CHILD_EMPTY) S.push(ch[1]);
+        node = ch[0];
+      } else {
+        node = S.empty() ? DONE : S.pop();
+      }
+    }
+    if (node != DONE) {
+      const int32_t first = leaf_first(node);
+      const int32_t cnt = leaf_count(node);
+      for (int32_t j = 0; j < cnt; ++j) tri_closest_hit<RAYFIRE>(T, first + j, ray, a.orientation, ex, k, h);
+      tf = f32_up(h.t);
+      node = S.empty() ? DONE : S.pop();
+    }
+  }
+}
+
 // Instrumented ray_fire: identical traversal, plus per-ray work counters
 // summed into counters[0..7] = rays, V, F, E, C, N, hits, mixed-fallbacks.
-template <bool MIXED>
+template <bool MIXED, bool WIDE>
 __global__ void k_ray_fire_stats(RayFireArgs a, int cap, unsigned long long* counters)
 {
   extern __shared__ int32_t lds[];
@@ -88,8 +248,8 @@ __global__ void k_ray_fire_stats(RayFireArgs a, int cap, unsigned long long* cou
   h.face = INT32_MAX;
   h.surf = ID_NONE;
   h.inside = false;
-  TraceStats st{0, 0, 0, 0, 0};
-  trace<RAYFIRE, MIXED, true>(T, ray, a.orientation, ex, k, lds + threadIdx.x, blockDim.x, cap, h, &st);
+  TraceStats st{0, 0, 0, 0, 0, 0, 0};
+  trace<RAYFIRE, MIXED, true, WIDE>(T, ray, a.orientation, ex, k, lds + threadIdx.x, blockDim.x, cap, h, &st);
   atomicAdd(counters + 0, 1ULL);
   atomicAdd(counters + 1, (unsigned long long)st.V);
   atomicAdd(counters + 2, (unsigned long long)st.F);
@@ -98,9 +258,11 @@ __global__ void k_ray_fire_stats(RayFireArgs a, int cap, unsigned long long* cou
   atomicAdd(counters + 5, (unsigned long long)st.N);
   if (h.surf != ID_NONE) atomicAdd(counters + 6, 1ULL);
   if (MIXED && !mixed_safe(T, ray)) atomicAdd(counters + 7, 1ULL);
+  if (st.WV) atomicAdd(counters + 8, (unsigned long long)st.WV);
+  if (st.WF) atomicAdd(counters + 9, (unsigned long long)st.WF);
 }
 
-template <bool MIXED>
+template <bool MIXED, bool WIDE>
 __global__ void k_piv(PivArgs a, int cap)
 {
   extern __shared__ int32_t lds[];
@@ -121,11 +283,11 @@ __global__ void k_piv(PivArgs a, int cap)
   h.face = INT32_MAX;
   h.surf = ID_NONE;
   h.inside = false;
-  trace<PIV, MIXED>(T, ray, ORIENT_ANY, ex, k, lds + threadIdx.x, blockDim.x, cap, h);
+  trace<PIV, MIXED, false, WIDE>(T, ray, ORIENT_ANY, ex, k, lds + threadIdx.x, blockDim.x, cap, h);
   a.inside[i] = (h.surf != ID_NONE && h.inside) ? 1 : 0;
 }
 
-template <bool MIXED>
+template <bool MIXED, bool WIDE>
 __global__ void k_find_volume(FindVolumeArgs a, int cap)
 {
   extern __shared__ int32_t lds[];
@@ -143,7 +305,8 @@ __global__ void k_find_volume(FindVolumeArgs a, int cap)
     h.face = INT32_MAX;
     h.surf = ID_NONE;
     h.inside = false;
-    trace<PIV, MIXED>(a.table[tree], ray, ORIENT_ANY, nullptr, 0, lds + threadIdx.x, blockDim.x, cap, h);
+    trace<PIV, MIXED, false, WIDE>(a.table[tree], ray, ORIENT_ANY, nullptr, 0, lds + threadIdx.x, blockDim.x, cap,
+                                   h);
     if (h.surf != ID_NONE && h.inside) {
       found = vid;
       break;
@@ -152,7 +315,7 @@ __global__ void k_find_volume(FindVolumeArgs a, int cap)
   a.vol_out[i] = found;
 }
 
-template <bool MIXED>
+template <bool MIXED, bool WIDE>
 __global__ void k_occluded(OccludedArgs a, int cap)
 {
   extern __shared__ int32_t lds[];
@@ -166,7 +329,8 @@ __global__ void k_occluded(OccludedArgs a, int cap)
     h.face = INT32_MAX;
     h.surf = ID_NONE;
     h.inside = false;
-    hit = trace<OCCL, MIXED>(a.table[a.tree], ray, ORIENT_ANY, nullptr, 0, lds + threadIdx.x, blockDim.x, cap, h);
+    hit = trace<OCCL, MIXED, false, WIDE>(a.table[a.tree], ray, ORIENT_ANY, nullptr, 0, lds + threadIdx.x,
+                                          blockDim.x, cap, h);
   }
   a.hit[i] = hit ? 1 : 0;
   if (a.dist) a.dist[i] = hit ? -INFTY : INFTY;
@@ -251,54 +415,111 @@ __global__ void k_closest(ClosestArgs a, int cap)
 inline dim3 grid1(int64_t n, int blk) { return dim3((unsigned)((n + blk - 1) / blk)); }
 inline size_t lds_bytes(const LaunchCfg& c) { return sizeof(int32_t) * (size_t)c.stack * c.block; }
 
+#define XDG_DISPATCH2(KERNEL, MIXED, WIDE, ...)                                              \
+  do {                                                                                       \
+    if (MIXED) {                                                                             \
+      if (WIDE)                                                                              \
+        hipLaunchKernelGGL((KERNEL<true, true>), __VA_ARGS__);                               \
+      else                                                                                   \
+        hipLaunchKernelGGL((KERNEL<true, false>), __VA_ARGS__);                              \
+    } else {                                                                                 \
+      if (WIDE)                                                                              \
+        hipLaunchKernelGGL((KERNEL<false, true>), __VA_ARGS__);                              \
+      else                                                                                   \
+        hipLaunchKernelGGL((KERNEL<false, false>), __VA_ARGS__);                             \
+    }                                                                                        \
+  } while (0)
+
+}  // namespace
+
+namespace {
+
+// resident blocks of a kernel on the whole device (cached per shape)
+template <class K>
+int64_t resident_grid(K kernel, int block, size_t lds)
+{
+  static int cu = 0;
+  if (cu == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
+      cu = 256;
+  }
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, lds) != hipSuccess || nb <= 0) nb = 1;
+  return (int64_t)nb * cu;
+}
+
 }  // namespace
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)
 {
   if (a.n <= 0) return;
-  if (mixed)
-    hipLaunchKernelGGL(k_ray_fire<true>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
-  else
-    hipLaunchKernelGGL(k_ray_fire<false>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+  const dim3 g = grid1(a.n, c.block), b(c.block);
+  const size_t sh = lds_bytes(c);
+  const bool persist = (c.persistent & (mixed ? 2 : 1)) != 0;
+  if (persist && c.wide && a.next_ray && (!mixed || (a.fb_list && a.fb_count))) {
+    const int64_t need = (a.n + c.block - 1) / c.block;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, a.next_ray);
+    if (mixed) {
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, a.fb_count);
+      const int64_t gp = std::min(need, resident_grid(k_ray_fire_pt<true>, c.block, sh));
+      hipLaunchKernelGGL((k_ray_fire_pt<true>), dim3((unsigned)gp), b, sh, s, a, c.stack, a.next_ray);
+      const unsigned gd = (unsigned)std::min<int64_t>(need, 1024);
+      hipLaunchKernelGGL((k_ray_fire_deferred<true>), dim3(gd), b, sh, s, a, c.stack);
+    } else {
+      const int64_t gp = std::min(need, resident_grid(k_ray_fire_pt<false>, c.block, sh));
+      hipLaunchKernelGGL((k_ray_fire_pt<false>), dim3((unsigned)gp), b, sh, s, a, c.stack, a.next_ray);
+    }
+    return;
+  }
+  if (mixed && a.fb_list && a.fb_count) {
+    const unsigned gd = (unsigned)std::min<int64_t>((a.n + c.block - 1) / c.block, 1024);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, a.fb_count);
+    if (c.wide) {
+      hipLaunchKernelGGL((k_ray_fire<true, true, true>), g, b, sh, s, a, c.stack);
+      hipLaunchKernelGGL((k_ray_fire_deferred<true>), dim3(gd), b, sh, s, a, c.stack);
+    } else {
+      hipLaunchKernelGGL((k_ray_fire<true, true, false>), g, b, sh, s, a, c.stack);
+      hipLaunchKernelGGL((k_ray_fire_deferred<false>), dim3(gd), b, sh, s, a, c.stack);
+    }
+  } else if (mixed) {
+    if (c.wide)
+      hipLaunchKernelGGL((k_ray_fire<true, false, true>), g, b, sh, s, a, c.stack);
+    else
+      hipLaunchKernelGGL((k_ray_fire<true, false, false>), g, b, sh, s, a, c.stack);
+  } else {
+    if (c.wide)
+      hipLaunchKernelGGL((k_ray_fire<false, false, true>), g, b, sh, s, a, c.stack);
+    else
+      hipLaunchKernelGGL((k_ray_fire<false, false, false>), g, b, sh, s, a, c.stack);
+  }
 }
 
 void launch_ray_fire_stats(const RayFireArgs& a, bool mixed, const LaunchCfg& c, unsigned long long* counters,
                            hipStream_t s)
 {
   if (a.n <= 0) return;
-  if (mixed)
-    hipLaunchKernelGGL(k_ray_fire_stats<true>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack,
-                       counters);
-  else
-    hipLaunchKernelGGL(k_ray_fire_stats<false>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack,
-                       counters);
+  XDG_DISPATCH2(k_ray_fire_stats, mixed, c.wide, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack,
+                counters);
 }
 
 void launch_point_in_volume(const PivArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)
 {
   if (a.n <= 0) return;
-  if (mixed)
-    hipLaunchKernelGGL(k_piv<true>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
-  else
-    hipLaunchKernelGGL(k_piv<false>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+  XDG_DISPATCH2(k_piv, mixed, c.wide, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
 }
 
 void launch_find_volume(const FindVolumeArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)
 {
   if (a.n <= 0) return;
-  if (mixed)
-    hipLaunchKernelGGL(k_find_volume<true>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
-  else
-    hipLaunchKernelGGL(k_find_volume<false>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+  XDG_DISPATCH2(k_find_volume, mixed, c.wide, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
 }
 
 void launch_occluded(const OccludedArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)
 {
   if (a.n <= 0) return;
-  if (mixed)
-    hipLaunchKernelGGL(k_occluded<true>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
-  else
-    hipLaunchKernelGGL(k_occluded<false>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+  XDG_DISPATCH2(k_occluded, mixed, c.wide, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
 }
 
 void launch_closest(const ClosestArgs& a, const LaunchCfg& c, hipStream_t s)
@@ -395,7 +616,25 @@ __global__ void k_write_tris(const double* __restrict__ v9, const int32_t* __res
   out[s] = r;
 }
 
+// face_normal of every triangle record (mesh_manager_interface.cpp:216-220)
+__global__ void k_tri_normals(const TriRec* __restrict__ tris, int64_t n, double* out)
+{
+  int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const TriRec& r = tris[s];
+  const D3 nrm = face_normal(d3(r.v[0], r.v[1], r.v[2]), d3(r.v[3], r.v[4], r.v[5]), d3(r.v[6], r.v[7], r.v[8]));
+  out[3 * s] = nrm.x;
+  out[3 * s + 1] = nrm.y;
+  out[3 * s + 2] = nrm.z;
+}
+
 }  // namespace
+
+void launch_tri_normals(const TriRec* tris, int64_t n, double* out, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_tri_normals, grid1(n, 256), dim3(256), 0, s, tris, n, out);
+}
 
 void launch_tri_boxes(const double* verts9, int64_t n, double bump, double* box6, hipStream_t s)
 {

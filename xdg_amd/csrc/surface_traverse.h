@@ -41,6 +41,12 @@ __device__ __forceinline__ float f32_up(double x)
   if ((double)f < x) f = nextafterf(f, INFINITY);
   return f;
 }
+__device__ __forceinline__ float f32_down_d(double x)
+{
+  float f = (float)x;
+  if ((double)f > x) f = nextafterf(f, -INFINITY);
+  return f;
+}
 
 // slab test of one box; returns entry distance in tn
 __device__ __forceinline__ bool slab_d(const double* b, const BoxRayD& r, double tmax, double& tn)
@@ -65,6 +71,33 @@ __device__ __forceinline__ bool slab_f(const float* b, const BoxRayF& r, float t
   return a <= c;
 }
 
+// Per-lane traversal stack in LDS, interleaved across the workgroup (entry i
+// of lane l at base[i * stride + l], so a wave's pushes hit 64 consecutive
+// words).  The top is kept as a pointer: a push or pop is one add instead of
+// a 32-bit multiply.  Pushes beyond the capacity are dropped; callers size
+// the capacity to the tree's proven stack bound, so none is dropped.
+struct LdsStack {
+  int32_t* top;
+  int32_t* base;
+  int32_t* end;
+  int stride;
+  __device__ __forceinline__ LdsStack(int32_t* b, int s, int cap) : top(b), base(b), end(b + cap * s), stride(s) {}
+  __device__ __forceinline__ void reset() { top = base; }
+  __device__ __forceinline__ bool empty() const { return top == base; }
+  __device__ __forceinline__ void push(int32_t v)
+  {
+    if (top < end) {
+      *top = v;
+      top += stride;
+    }
+  }
+  __device__ __forceinline__ int32_t pop()
+  {
+    top -= stride;
+    return *top;
+  }
+};
+
 struct HitState {
   double t;
   int32_t face;
@@ -74,10 +107,19 @@ struct HitState {
 
 // Work counters of the instrumented traversal (roofline accounting): V node
 // visits, F triangle tests, E Pluecker edge tests, C tests reaching the
-// distance computation, N normal evaluations.
+// distance computation, N normal evaluations; WV / WF count the node and
+// triangle iterations each wave executes (credited to its lowest active
+// lane), so V / (64 WV) is the SIMD efficiency of the node loop.
 struct TraceStats {
-  unsigned int V, F, E, C, N;
+  unsigned int V, F, E, C, N, WV, WF;
 };
+
+// 1 in the lowest active lane of the wave, else 0
+__device__ __forceinline__ unsigned int first_active_lane()
+{
+  const unsigned long long m = __ballot(1);
+  return (unsigned int)((int)(threadIdx.x & 63) == __ffsll((long long)m) - 1);
+}
 
 // Counts the edge tests plucker_any performs (same control flow).
 __device__ __forceinline__ void plucker_count(const D3& v0, const D3& v1, const D3& v2, const D3& d, const D3& rb,
@@ -96,26 +138,28 @@ __device__ __forceinline__ void plucker_count(const D3& v0, const D3& v1, const 
 }
 
 // Leaf triangle for RAYFIRE / PIV: TriangleIntersectionFunc with the tie rule.
-template <int MODE, bool STATS = false>
-__device__ __forceinline__ void tri_closest_hit(const TriRec* __restrict__ tris, int32_t p, const RayD& ray,
-                                                int orientation, const int32_t* ex, int k, HitState& h,
-                                                TraceStats* st = nullptr)
+// The normal is read from the build-time array (same bits as recomputing
+// face_normal here), so the leaf carries no sqrt/divide.
+template <int MODE, bool STATS = false, class TT = SurfTreeDev>
+__device__ __forceinline__ void tri_closest_hit(const TT& T, int32_t p, const RayD& ray, int orientation,
+                                                const int32_t* ex, int k, HitState& h, TraceStats* st = nullptr)
 {
-  const TriRec& r = tris[p];
-  const D3 v0 = d3(r.v[0], r.v[1], r.v[2]);
-  const D3 v1 = d3(r.v[3], r.v[4], r.v[5]);
-  const D3 v2 = d3(r.v[6], r.v[7], r.v[8]);
+  const auto r = as_global(T.tris + p);
+  const D3 v0 = d3(r->v[0], r->v[1], r->v[2]);
+  const D3 v1 = d3(r->v[3], r->v[4], r->v[5]);
+  const D3 v2 = d3(r->v[6], r->v[7], r->v[8]);
   double t;
   if (STATS) {
     st->F += 1;
     plucker_count(v0, v1, v2, ray.d, ray.rb, *st);
   }
   if (!plucker_any(v0, v1, v2, ray.d, ray.rb, ray.idx, ray.oi, ray.di, h.t, 0.0, t)) return;
-  const int32_t face = r.face;
+  const int32_t face = r->face;
   if (t == h.t && face >= h.face) return;  // tie rule
-  const int32_t s2 = r.surf2;
+  const int32_t s2 = r->surf2;
   if (STATS) st->N += 1;
-  D3 n = face_normal(v0, v1, v2);
+  const auto pn = as_global(T.normals + 3 * (int64_t)p);
+  D3 n = d3(pn[0], pn[1], pn[2]);
   if (s2 & 1) n = neg(n);
   if (MODE == RAYFIRE && orientation != ORIENT_ANY) {
     const double dp = dot(ray.d, n);
@@ -133,10 +177,10 @@ __device__ __forceinline__ void tri_closest_hit(const TriRec* __restrict__ tris,
 template <int MODE>
 __device__ __forceinline__ bool tri_any_hit(const TriRec* __restrict__ tris, int32_t p, const RayD& ray)
 {
-  const TriRec& r = tris[p];
-  const D3 v0 = d3(r.v[0], r.v[1], r.v[2]);
-  const D3 v1 = d3(r.v[3], r.v[4], r.v[5]);
-  const D3 v2 = d3(r.v[6], r.v[7], r.v[8]);
+  const auto r = as_global(tris + p);
+  const D3 v0 = d3(r->v[0], r->v[1], r->v[2]);
+  const D3 v1 = d3(r->v[3], r->v[4], r->v[5]);
+  const D3 v2 = d3(r->v[6], r->v[7], r->v[8]);
   double t;
   return plucker_any(v0, v1, v2, ray.d, ray.rb, ray.idx, ray.oi, ray.di, INFTY, 0.0, t);
 }
@@ -166,63 +210,198 @@ __device__ __forceinline__ bool traverse(const SurfTreeDev& T, const RayD& ray, 
     bd.ny = -ray.o.y * bd.iy;
     bd.nz = -ray.o.z * bd.iz;
   }
+  // The stack holds child codes, [slot][lane] in LDS (4 bytes per entry:
+  // storing entry distances as well halves the occupancy LDS allows).
+  LdsStack S(stk, stride, depth_cap);
   float tf = MODE == OCCL ? INFINITY : f32_up(h.t);
   int32_t node = T.root;
-  int sp = 0;
   while (true) {
     if (node >= 0) {
       int32_t c0, c1;
       bool h0, h1;
-      bool swap;
+      float e0, e1;
       if (STATS) st->V += 1;
       if (MIXED) {
-        const NodeF nd = T.nodes_f[node];
-        float t0, t1;
-        h0 = slab_f(nd.b0, bf, tf, t0);
-        h1 = slab_f(nd.b1, bf, tf, t1);
+        const NodeF nd = gload(T.nodes_f + node);
+        h0 = slab_f(nd.b0, bf, tf, e0);
+        h1 = slab_f(nd.b1, bf, tf, e1);
         c0 = nd.c0;
         c1 = nd.c1;
-        swap = t1 < t0;
       } else {
-        const NodeD nd = T.nodes_d[node];
+        const NodeD nd = gload(T.nodes_d + node);
         double t0, t1;
         double tm = MODE == OCCL ? INFTY : h.t;
         h0 = slab_d(nd.b0, bd, tm, t0);
         h1 = slab_d(nd.b1, bd, tm, t1);
+        e0 = f32_down_d(t0);
+        e1 = f32_down_d(t1);
         c0 = nd.c0;
         c1 = nd.c1;
-        swap = t1 < t0;
       }
       if (h0 && h1) {
-        if (swap) {
+        if (e1 < e0) {
           int32_t tmp = c0;
           c0 = c1;
           c1 = tmp;
+          e1 = e0;
         }
-        if (sp < depth_cap) stk[sp++ * stride] = c1;
+        S.push(c1);
         node = c0;
+        continue;
       } else if (h0) {
         node = c0;
+        continue;
       } else if (h1) {
         node = c1;
-      } else {
-        if (sp == 0) break;
-        node = stk[--sp * stride];
+        continue;
       }
-      continue;
+    } else {
+      const int32_t first = leaf_first(node);
+      const int32_t cnt = leaf_count(node);
+      for (int32_t j = 0; j < cnt; ++j) {
+        if (MODE == OCCL) {
+          if (tri_any_hit<MODE>(T.tris, first + j, ray)) return true;
+        } else {
+          tri_closest_hit<MODE, STATS>(T, first + j, ray, orientation, ex, k, h, st);
+        }
+      }
+      if (MODE != OCCL) tf = f32_up(h.t);
     }
+    if (S.empty()) break;
+    node = S.pop();
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// BVH4 traversal: one node fetch tests four child boxes; hits are ordered by
+// entry distance (5-comparator network), the nearest is visited next and the
+// others pushed farthest-first.
+// ---------------------------------------------------------------------------
+
+constexpr int32_t CHILD_EMPTY = INT32_MIN;  // never a valid leaf or node code
+
+__device__ __forceinline__ void cswap(float& ta, int32_t& ca, float& tb, int32_t& cb)
+{
+  if (tb < ta) {
+    const float t = ta;
+    ta = tb;
+    tb = t;
+    const int32_t c = ca;
+    ca = cb;
+    cb = c;
+  }
+}
+
+// The pointers a traversal needs, per lane (persistent kernels, where
+// neighbouring lanes may trace different trees).
+struct TreeView {
+  const Node4D* nodes4_d;
+  const Node4F* nodes4_f;
+  const TriRec* tris;
+  const double* normals;
+};
+
+template <bool MIXED, class TT = SurfTreeDev>
+__device__ __forceinline__ void node4_test(const TT& T, int32_t node, const BoxRayF& bf, const BoxRayD& bd,
+                                           float tf, double tm, float* te, int32_t* ch)
+{
+  if (MIXED) {
+    const Node4F nd = gload(T.nodes4_f + node);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const float x0 = fmaf(nd.lox[c], bf.ix, bf.nx), x1 = fmaf(nd.hix[c], bf.ix, bf.nx);
+      const float y0 = fmaf(nd.loy[c], bf.iy, bf.ny), y1 = fmaf(nd.hiy[c], bf.iy, bf.ny);
+      const float z0 = fmaf(nd.loz[c], bf.iz, bf.nz), z1 = fmaf(nd.hiz[c], bf.iz, bf.nz);
+      const float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+      const float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tf));
+      const bool hit = a <= b && nd.child[c] != CHILD_EMPTY;
+      te[c] = hit ? a : INFINITY;
+      ch[c] = hit ? nd.child[c] : CHILD_EMPTY;
+    }
+  } else {
+    const Node4D nd = gload(T.nodes4_d + node);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double x0 = fma(nd.lox[c], bd.ix, bd.nx), x1 = fma(nd.hix[c], bd.ix, bd.nx);
+      const double y0 = fma(nd.loy[c], bd.iy, bd.ny), y1 = fma(nd.hiy[c], bd.iy, bd.ny);
+      const double z0 = fma(nd.loz[c], bd.iz, bd.nz), z1 = fma(nd.hiz[c], bd.iz, bd.nz);
+      const double a = fmax(fmax(fmin(x0, x1), fmin(y0, y1)), fmax(fmin(z0, z1), 0.0));
+      const double b = fmin(fmin(fmax(x0, x1), fmax(y0, y1)), fmin(fmax(z0, z1), tm));
+      const bool hit = a <= b && nd.child[c] != CHILD_EMPTY;
+      te[c] = hit ? f32_down_d(a) : INFINITY;
+      ch[c] = hit ? nd.child[c] : CHILD_EMPTY;
+    }
+  }
+}
+
+template <int MODE, bool MIXED, bool STATS = false>
+__device__ __forceinline__ bool traverse4(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
+                                          int k, int32_t* stk, int stride, int depth_cap, HitState& h,
+                                          TraceStats* st = nullptr)
+{
+  if (T.n_prims <= 0) return false;
+  BoxRayD bd;
+  BoxRayF bf;
+  if (MIXED) {
+    float ox = (float)ray.o.x, oy = (float)ray.o.y, oz = (float)ray.o.z;
+    bf.ix = safe_inv_f((float)ray.d.x);
+    bf.iy = safe_inv_f((float)ray.d.y);
+    bf.iz = safe_inv_f((float)ray.d.z);
+    bf.nx = -ox * bf.ix;
+    bf.ny = -oy * bf.iy;
+    bf.nz = -oz * bf.iz;
+  } else {
+    bd.ix = safe_inv_d(ray.d.x);
+    bd.iy = safe_inv_d(ray.d.y);
+    bd.iz = safe_inv_d(ray.d.z);
+    bd.nx = -ray.o.x * bd.ix;
+    bd.ny = -ray.o.y * bd.iy;
+    bd.nz = -ray.o.z * bd.iz;
+  }
+  float tf = MODE == OCCL ? INFINITY : f32_up(h.t);
+  int32_t node = T.root4;
+  LdsStack S(stk, stride, depth_cap);
+  // while-while: lanes descend internal nodes together until every lane of
+  // the wave holds a leaf (or is done), then the leaves are tested together;
+  // node and leaf work never share a divergent iteration.
+  const int32_t DONE = CHILD_EMPTY;
+  while (node != DONE) {
+    while (node >= 0) {
+      if (STATS) {
+        st->V += 1;
+        st->WV += first_active_lane();
+      }
+      float te[4];
+      int32_t ch[4];
+      node4_test<MIXED>(T, node, bf, bd, tf, MODE == OCCL ? INFTY : h.t, te, ch);
+      cswap(te[0], ch[0], te[1], ch[1]);
+      cswap(te[2], ch[2], te[3], ch[3]);
+      cswap(te[0], ch[0], te[2], ch[2]);
+      cswap(te[1], ch[1], te[3], ch[3]);
+      cswap(te[1], ch[1], te[2], ch[2]);
+      if (ch[0] != CHILD_EMPTY) {
+        if (ch[3] != CHILD_EMPTY) S.push(ch[3]);
+        if (ch[2] != CHILD_EMPTY) S.push(ch[2]);
+        if (ch[1] != CHILD_EMPTY) S.push(ch[1]);
+        node = ch[0];
+      } else {
+        node = S.empty() ? DONE : S.pop();
+      }
+    }
+    if (node == DONE) break;
     const int32_t first = leaf_first(node);
     const int32_t cnt = leaf_count(node);
     for (int32_t j = 0; j < cnt; ++j) {
+      if (STATS) st->WF += first_active_lane();
       if (MODE == OCCL) {
         if (tri_any_hit<MODE>(T.tris, first + j, ray)) return true;
       } else {
-        tri_closest_hit<MODE, STATS>(T.tris, first + j, ray, orientation, ex, k, h, st);
+        tri_closest_hit<MODE, STATS>(T, first + j, ray, orientation, ex, k, h, st);
       }
     }
     if (MODE != OCCL) tf = f32_up(h.t);
-    if (sp == 0) break;
-    node = stk[--sp * stride];
+    node = S.empty() ? DONE : S.pop();
   }
   return false;
 }
@@ -245,15 +424,32 @@ __device__ __forceinline__ bool mixed_safe(const SurfTreeDev& T, const RayD& r)
   return m <= (double)T.o_limit;
 }
 
-template <int MODE, bool MIXED, bool STATS = false>
+// MIXED traces with FP32 boxes when the origin is within the tree's limit
+// and falls back to FP64 boxes otherwise; WIDE selects the BVH4.
+template <int MODE, bool MIXED, bool STATS = false, bool WIDE = false>
 __device__ __forceinline__ bool trace(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
                                       int k, int32_t* stk, int stride, int cap, HitState& h,
                                       TraceStats* st = nullptr)
 {
+  if (WIDE) {
+    if (MIXED && mixed_safe(T, ray))
+      return traverse4<MODE, true, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
+    return traverse4<MODE, false, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
+  }
   if (MIXED) {
     if (mixed_safe(T, ray)) return traverse<MODE, true, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
   }
   return traverse<MODE, false, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
+}
+
+// Mixed-only traversal for rays already known to be within the FP32 limit.
+template <int MODE, bool STATS = false, bool WIDE = false>
+__device__ __forceinline__ bool trace_f32(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
+                                          int k, int32_t* stk, int stride, int cap, HitState& h,
+                                          TraceStats* st = nullptr)
+{
+  if (WIDE) return traverse4<MODE, true, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
+  return traverse<MODE, true, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
 }
 
 }  // namespace xdgd

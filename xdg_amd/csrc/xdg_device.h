@@ -18,6 +18,25 @@
 
 namespace xdgd {
 
+// Tree data is read through the global address space: a pointer fetched
+// from a table in memory is otherwise generic, and generic (flat) loads also
+// count against the LDS counter, so each wait for a node would wait for the
+// LDS stack traffic as well.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(const T* p)
+{
+  return (gptr<T>)p;
+}
+template <class T>
+__device__ __forceinline__ T gload(const T* p)
+{
+  T v;
+  __builtin_memcpy(&v, (const void*)as_global(p), sizeof(T));
+  return v;
+}
+
 constexpr double INFTY = 1.7976931348623157e308;  // constants.h:15 (DBL_MAX)
 constexpr double PLUCKER_ZERO_TOL = 20.0 * 2.220446049250313080847e-16;
 constexpr double TINY_BIT = 1e-10;                // constants.h TINY_BIT
@@ -299,6 +318,22 @@ struct alignas(16) NodeF {  // mixed layout, 64 B: outward-rounded FP32 boxes
   int32_t c0, c1, pad0, pad1;
 };
 
+// 4-wide nodes (BVH4, collapsed from the binary LBVH): planes stored
+// structure-of-arrays so one node gives all four child boxes.  Empty child
+// slots hold an inverted box (lo = +inf, hi = -inf) that no ray can hit.
+// Padded to whole 128-byte cache lines: one line per mixed node, two per
+// plain node.
+struct alignas(128) Node4F {  // mixed layout, 112 B used, 128 B stride
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  int32_t child[4];
+  int32_t pad[4];
+};
+struct alignas(128) Node4D {  // plain FP64 layout, 208 B used, 256 B stride
+  double lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  int32_t child[4];
+  int32_t pad[12];
+};
+
 // Triangle record in leaf order, 80 B: FP64 vertices + ids.
 struct alignas(16) TriRec {
   double v[9];
@@ -330,6 +365,14 @@ struct SurfTreeDev {
   float o_limit;     // mixed traversal is conservative for |o|_inf <= o_limit
   int32_t depth;     // max root-to-leaf depth
   double bump;
+  // BVH4 over the same leaves (same leaf codes)
+  const Node4D* nodes4_d;
+  const Node4F* nodes4_f;
+  int32_t root4;     // child-encoded root of the BVH4
+  int32_t stack4;    // max traversal stack entries the BVH4 can need
+  // face_normal of each triangle (leaf order, 3 doubles), computed on the
+  // device at build time with the same expression as the reference
+  const double* normals;
 };
 
 struct ElemTreeDev {
