@@ -85,6 +85,24 @@ int xdg_hip_get_launch(xdg_hip_ctx* ctx, int* block, int* stack);
  * claim new rays from a counter as they finish.  Results are identical
  * either way. */
 int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent);
+/* Tuning options by name (no counterpart in the reference; results are
+ * identical for every setting):
+ *   "block"       workgroup size, multiple of 64 (default 256)
+ *   "stack"       minimum LDS stack entries per lane (default 1; raised to
+ *                 what each tree needs)
+ *   "persistent"  resident-grid ray_fire, bit 0 plain, bit 1 mixed (1)
+ *   "ray_order"   ray_fire batches of >= 65536 rays: 0 caller's order,
+ *                 1 sorted by (tree, direction octant, origin Morton code),
+ *                 2 as 1 plus one coarse direction bit per axis (0)
+ *   "speculative" ray_fire BVH4 node loop keeps descending with a parked
+ *                 leaf until every lane of the wave holds one (0)
+ *   "wide"        BVH4 traversal (1) or the binary LBVH (0)
+ *   "defer"       mixed ray_fire traces far-origin rays in an FP64 pass (1)
+ *   "leaf_size"   max triangles per leaf for trees built afterwards (4)
+ * Unknown names give XDG_HIP_ERR_NOT_FOUND, out-of-range values
+ * XDG_HIP_ERR_INVALID. */
+int xdg_hip_set_option(xdg_hip_ctx* ctx, const char* name, int value);
+int xdg_hip_get_option(xdg_hip_ctx* ctx, const char* name, int* value);
 int xdg_hip_synchronize(xdg_hip_ctx* ctx, void* stream);
 
 /* ---- tree construction (host arrays in, device LBVH out) ---------------- */
@@ -137,9 +155,10 @@ int xdg_hip_ray_fire(xdg_hip_ctx* ctx, int32_t tree, const int32_t* tree_per_ray
  * nodes visited, triangle tests, Pluecker edge tests, tests reaching the
  * distance computation, normal evaluations, hits, mixed-precision fallbacks
  * to FP64 boxes, node-loop iterations executed per wave (summed over waves),
- * triangle iterations executed per wave.  No counterpart in the reference
- * (Embree exposes no counters). */
-#define XDG_HIP_NUM_COUNTERS 10
+ * triangle iterations executed per wave, node visits of the longest ray of
+ * each wave (summed over waves), triangle tests of the longest ray of each
+ * wave.  No counterpart in the reference (Embree exposes no counters). */
+#define XDG_HIP_NUM_COUNTERS 12
 int xdg_hip_ray_fire_stats(xdg_hip_ctx* ctx, int32_t tree, const int32_t* tree_per_ray, int64_t n,
                            const double* org, const double* dir, const double* tmax, int orientation,
                            const int32_t* excl, int k, int precision, uint64_t* counters);
@@ -205,6 +224,71 @@ int xdg_hip_status(xdg_hip_ctx* ctx, uint32_t* flags);
  *     15 (p v0 v1 v2 v3) -> 1
  *   4 sqrt: 1 -> 1;  5 division a/b: 2 -> 1 */
 int xdg_hip_eval_primitive(xdg_hip_ctx* ctx, int kind, int64_t n, const double* in, double* out, void* stream);
+
+/* ---- batched particle random walk ----------------------------------------
+ * Replaces the serial history loop of tools/particle_sim.h:136-154
+ * (transport_particles) and its Particle methods (:34-130): each history
+ * samples a start uniform in [lo, hi] and an isotropic direction, finds its
+ * volume (XDG::find_volume, xdg.cpp:113-128), then repeats ray_fire
+ * (EXITING, excluding the last 8 faces hit since the last collision,
+ * ray_tracer.cpp:324), an exponential flight -log(1-u)*mfp, a track-length
+ * tally in the current volume, and either a collision (absorbed with
+ * probability p_absorb, else a new isotropic direction) or a surface
+ * crossing (MeshManager::next_volume, mesh_manager_interface.cpp:202-214).
+ * A particle entering volume ipc leaks (ipc_graveyard).  Random numbers:
+ * Philox4x32-10, key (seed, stream), counter (history, 0, draw, 0).
+ * Additions to the reference driver: Philox, sampled starts, absorption,
+ * the event cap.  One persistent launch runs all histories; lanes claim
+ * `chunk` histories at a time from a device counter. */
+typedef struct {
+  int32_t nvol;               /* volumes with trees, in find_volume order */
+  const int32_t* vol_ids;     /* host, nvol */
+  const int32_t* vol_trees;   /* host, nvol: surface tree of each volume */
+  int32_t ipc;                /* implicit complement volume id */
+  int32_t nsurf_ids;          /* sense tables cover surface ids [0, nsurf_ids) */
+  const int32_t* sense_fwd;   /* host: forward parent volume per surface id */
+  const int32_t* sense_rev;   /* host: reverse parent volume per surface id */
+  uint32_t seed, stream;
+  double lo[3], hi[3];        /* start box */
+  double mfp;                 /* mean free path */
+  double p_absorb;            /* absorption probability per collision */
+  int32_t max_events;         /* event cap per history */
+} xdg_hip_transport_problem;
+
+/* Per-history status codes */
+#define XDG_HIP_TR_ABSORBED 1
+#define XDG_HIP_TR_LEAKED 2
+#define XDG_HIP_TR_LOST 3     /* ray_fire missed (reference: fatal_error) */
+#define XDG_HIP_TR_STUCK 4    /* hit at distance 0 (reference: fatal_error) */
+#define XDG_HIP_TR_CAPPED 5   /* max_events reached */
+#define XDG_HIP_TR_ERROR 6    /* topology error (volume not a parent, ...) */
+
+int xdg_hip_transport_create(xdg_hip_ctx* ctx, const xdg_hip_transport_problem* p, int32_t* handle);
+int xdg_hip_transport_destroy(xdg_hip_ctx* ctx, int32_t handle);
+/* Launch configuration (0 keeps a value): workgroup size (64..256), traversal stack
+ * entries held in LDS (the rest spill to a global area; capped at what the
+ * trees need), histories claimed per lane at a time.  stack_need = entries
+ * the deepest tree of the problem can need. */
+int xdg_hip_transport_set_config(xdg_hip_ctx* ctx, int32_t handle, int block, int lds_stack, int chunk);
+int xdg_hip_transport_get_config(xdg_hip_ctx* ctx, int32_t handle, int* block, int* lds_stack, int* chunk,
+                                 int* stack_need);
+/* Histories [first, first + n): device outputs tally_fx (nvol, per volume
+ * slot, int64 two's complement of sum(llrint(len * 2^32))), and nullable
+ * per-history events (int32), status (uint8), sequence hash (uint64) over
+ * (volume, length) pairs.  Asynchronous on stream; allocates nothing after
+ * the first run of a configuration, so it may be captured. */
+int xdg_hip_transport_run(xdg_hip_ctx* ctx, int32_t handle, uint64_t first, int64_t n, int precision,
+                          uint64_t* tally_fx, int32_t* events, uint8_t* status, uint64_t* hash, void* stream);
+/* The same launch captured into an instantiated HIP graph (opaque handle):
+ * replay with xdg_hip_graph_launch, free with xdg_hip_graph_destroy. */
+int xdg_hip_transport_capture(xdg_hip_ctx* ctx, int32_t handle, uint64_t first, int64_t n, int precision,
+                              uint64_t* tally_fx, int32_t* events, uint8_t* status, uint64_t* hash, void** graph);
+int xdg_hip_graph_launch(xdg_hip_ctx* ctx, void* graph, void* stream);
+int xdg_hip_graph_destroy(xdg_hip_ctx* ctx, void* graph);
+/* Times block x lds_stack x chunk candidates on histories [first, first +
+ * sample) and keeps the fastest configuration (synchronous). */
+int xdg_hip_transport_autotune(xdg_hip_ctx* ctx, int32_t handle, uint64_t first, int64_t sample, int precision,
+                               double* best_ms);
 
 /* ---- device memory helpers for callers without their own allocator ----- */
 int xdg_hip_malloc(xdg_hip_ctx* ctx, size_t bytes, void** ptr);

@@ -79,8 +79,21 @@ def lib():
                                   C.c_uint32, _d, _d]
         L.xo_log.restype = C.c_double
         L.xo_log.argtypes = [C.c_double]
+        L.xo_transport.argtypes = [C.POINTER(TransportProblem), C.c_uint64,
+                                   C.c_int64, C.c_int, C.POINTER(C.c_int64),
+                                   _i32, _u8, C.POINTER(C.c_uint64)]
         _LIB = L
     return _LIB
+
+
+class TransportProblem(C.Structure):
+    _fields_ = [("nvol", C.c_int), ("trees", C.POINTER(C.c_void_p)),
+                ("vol_ids", C.POINTER(C.c_int32)), ("ipc", C.c_int32),
+                ("nsurf_ids", C.c_int), ("sense_fwd", C.POINTER(C.c_int32)),
+                ("sense_rev", C.POINTER(C.c_int32)), ("seed", C.c_uint32),
+                ("stream", C.c_uint32), ("lo", C.c_double * 3),
+                ("hi", C.c_double * 3), ("mfp", C.c_double),
+                ("p_absorb", C.c_double), ("max_events", C.c_int32)]
 
 
 def _p(a, t):
@@ -319,3 +332,51 @@ class TetMesh:
         if sequences is not None:
             out.append((off, se, sl))
         return tuple(out)
+
+
+# ---- batched particle random walk -----------------------------------------
+
+def sense_tables(model):
+    """Forward / reverse parent volume per surface id (-1 where absent)."""
+    ns = max(model.senses) + 1
+    fwd = np.full(ns, -1, np.int32)
+    rev = np.full(ns, -1, np.int32)
+    for sid, (f, r) in model.senses.items():
+        fwd[sid] = f
+        rev[sid] = r
+    return fwd, rev
+
+
+def transport(model, trees, vol_ids, seed, stream, lo, hi, mfp, p_absorb,
+              max_events, first, n, nthreads=0):
+    """xo_transport over histories first .. first+n-1.  trees: Tree per
+    entry of vol_ids (find_volume order).  Returns tally_fx (int64 per
+    slot, scale 2^32), events, status, hash."""
+    ids = _i32a(vol_ids)
+    fwd, rev = sense_tables(model)
+    arr = (_vp * len(trees))(*[t.h for t in trees])
+    pr = TransportProblem()
+    pr.nvol = len(trees)
+    pr.trees = C.cast(arr, C.POINTER(C.c_void_p))
+    pr.vol_ids = _p(ids, _i32)
+    pr.ipc = int(model.ipc)
+    pr.nsurf_ids = fwd.size
+    pr.sense_fwd = _p(fwd, _i32)
+    pr.sense_rev = _p(rev, _i32)
+    pr.seed = seed
+    pr.stream = stream
+    pr.lo = (C.c_double * 3)(*[float(x) for x in lo])
+    pr.hi = (C.c_double * 3)(*[float(x) for x in hi])
+    pr.mfp = float(mfp)
+    pr.p_absorb = float(p_absorb)
+    pr.max_events = int(max_events)
+    tally = np.zeros(len(trees), np.int64)
+    events = np.zeros(n, np.int32)
+    status = np.zeros(n, np.uint8)
+    hashes = np.zeros(n, np.uint64)
+    lib().xo_transport(C.byref(pr), int(first), int(n), nthreads,
+                       tally.ctypes.data_as(C.POINTER(C.c_int64)),
+                       _p(events, _i32), _p(status, _u8),
+                       hashes.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return {"tally_fx": tally, "events": events, "status": status,
+            "hash": hashes}

@@ -1293,3 +1293,160 @@ double xo_log(double x)
   double de = (double)e;
   return de * LN2_HI + (de * LN2_LO + lnm);
 }
+
+/* ------------------------------------------------------------------------ */
+/* batched particle random walk                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Marsaglia (1972) isotropic direction, as tools/ray_benchmark.h:24-40: one
+ * Philox block per attempt, the draw counter advances once per attempt. */
+static int iso_dir_draw(uint32_t seed, uint32_t stream, uint64_t item,
+                        uint32_t* dc, double* u)
+{
+  for (int a = 0; a < 1000; ++a) {
+    double ua, ub;
+    xo_uniform2(seed, stream, item, (*dc)++, &ua, &ub);
+    double x1 = ua * 2.0 - 1.0;
+    double x2 = ub * 2.0 - 1.0;
+    double s = x1 * x1 + x2 * x2;
+    if (s > 0.0 && s < 1.0) {
+      double t = 2.0 * sqrt(1.0 - s);
+      u[0] = x1 * t;
+      u[1] = x2 * t;
+      u[2] = 1.0 - 2.0 * s;
+      return 1;
+    }
+  }
+  return 0;
+}
+
+static int slot_of(const xo_transport_problem* p, int32_t vol)
+{
+  for (int v = 0; v < p->nvol; ++v)
+    if (p->vol_ids[v] == vol) return v;
+  return -1;
+}
+
+static inline void hist_push(int32_t* hist, int* n, int32_t face)
+{
+  if (*n == XO_TRANSPORT_HISTORY) {
+    for (int j = 1; j < XO_TRANSPORT_HISTORY; ++j) hist[j - 1] = hist[j];
+    *n -= 1;
+  }
+  hist[(*n)++] = face;
+}
+
+void xo_transport(const xo_transport_problem* p, uint64_t first, int64_t n,
+                  int nthreads, int64_t* tally_fx, int32_t* events,
+                  uint8_t* status, uint64_t* hash)
+{
+  int nt = resolve_threads(nthreads);
+  (void)nt;
+  OMP_FOR
+  for (int64_t i = 0; i < n; ++i) {
+    const uint64_t item = first + (uint64_t)i;
+    /* start: uniform in the box from blocks 0 and 1 (same protocol as the
+     * host box_points generator), then the direction from block 2 on */
+    uint32_t dc = 0;
+    double ua, ub, uc, unused;
+    double r[3], u[3];
+    xo_uniform2(p->seed, p->stream, item, dc++, &ua, &ub);
+    xo_uniform2(p->seed, p->stream, item, dc++, &uc, &unused);
+    r[0] = p->lo[0] + (p->hi[0] - p->lo[0]) * ua;
+    r[1] = p->lo[1] + (p->hi[1] - p->lo[1]) * ub;
+    r[2] = p->lo[2] + (p->hi[2] - p->lo[2]) * uc;
+    uint8_t st = XO_TR_ALIVE;
+    int32_t ev = 0;
+    uint64_t hh = XO_HASH_SEED;
+    if (!iso_dir_draw(p->seed, p->stream, item, &dc, u)) st = XO_TR_ERROR;
+    /* XDG::find_volume (xdg.cpp:113-128) with the particle's direction */
+    int32_t vol = p->ipc;
+    if (st == XO_TR_ALIVE) {
+      for (int v = 0; v < p->nvol; ++v) {
+        if (p->vol_ids[v] == p->ipc) continue;
+        hit_state h;
+        trace_one(p->trees[v], r, u, INFTY, XO_ANY, NULL, 0, 0, &h);
+        if (h.surf != XO_ID_NONE && v_dot(u, h.n) > 0.0) {
+          vol = p->vol_ids[v];
+          break;
+        }
+      }
+      if (vol == p->ipc) st = XO_TR_LEAKED;
+    }
+    int32_t hist[XO_TRANSPORT_HISTORY];
+    int hn = 0;
+    while (st == XO_TR_ALIVE) {
+      const int slot = slot_of(p, vol);
+      if (slot < 0) {
+        st = XO_TR_ERROR;
+        break;
+      }
+      /* Particle::surf_dist (particle_sim.h:41-54) */
+      hit_state h;
+      trace_one(p->trees[slot], r, u, INFTY, XO_EXITING, hist, hn, 0, &h);
+      if (h.surf == XO_ID_NONE) {
+        st = XO_TR_LOST;
+        break;
+      }
+      if (h.t == 0.0) {
+        st = XO_TR_STUCK;
+        break;
+      }
+      hist_push(hist, &hn, h.face); /* ray_tracer.cpp:324 appends the hit */
+      /* sample_collision_distance (particle_sim.h:56-58) */
+      xo_uniform2(p->seed, p->stream, item, dc++, &ua, &ub);
+      const double dcol = -xo_log(1.0 - ua) * p->mfp;
+      /* advance (particle_sim.h:70-83): r += s * u, tally the flight */
+      const double s = dcol < h.t ? dcol : h.t;
+      r[0] = r[0] + s * u[0];
+      r[1] = r[1] + s * u[1];
+      r[2] = r[2] + s * u[2];
+      if (tally_fx) {
+#pragma omp atomic
+        tally_fx[slot] += (int64_t)llrint(ldexp(s, 32));
+      }
+      hh = xo_mix64(hh ^ (uint64_t)(uint32_t)vol);
+      hh = xo_mix64(hh ^ dbits(s));
+      ev += 1;
+      if (dcol < h.t) {
+        /* collide (particle_sim.h:60-68), absorption first */
+        if (ub < p->p_absorb) {
+          st = XO_TR_ABSORBED;
+          break;
+        }
+        if (!iso_dir_draw(p->seed, p->stream, item, &dc, u)) {
+          st = XO_TR_ERROR;
+          break;
+        }
+        hn = 0;
+      } else {
+        /* cross_surface -> MeshManager::next_volume
+         * (mesh_manager_interface.cpp:202-214) */
+        const int32_t sid = h.surf;
+        int32_t nv = XO_ID_NONE;
+        if (sid >= 0 && sid < p->nsurf_ids) {
+          if (p->sense_fwd[sid] == vol)
+            nv = p->sense_rev[sid];
+          else if (p->sense_rev[sid] == vol)
+            nv = p->sense_fwd[sid];
+        }
+        if (nv == XO_ID_NONE) {
+          st = XO_TR_ERROR;
+          break;
+        }
+        vol = nv;
+        if (vol == p->ipc) {
+          st = XO_TR_LEAKED;
+          break;
+        }
+      }
+      if (ev >= p->max_events) {
+        st = XO_TR_CAPPED;
+        break;
+      }
+    }
+    if (events) events[i] = ev;
+    if (status) status[i] = st;
+    if (hash) hash[i] = hh;
+  }
+}

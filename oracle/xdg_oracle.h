@@ -34,10 +34,13 @@
  *   xo_segments                   sum(len) == chord invariant + MeshMock
  *                                 walk golden sequences
  *   xo_philox4x32_10              Random123 known-answer vectors
- *   xo_log                        glibc log() within 1 ulp
- *   xo_transport                  parity unpinned against the reference
+ *   xo_log                        glibc log() within 2 ulp
+ *   xo_transport                  no reference output exists
  *                                 (particle_sim.h has no absorption and uses
- *                                 drand48); pinned by conservation checks
+ *                                 drand48); pinned by the homogeneous-medium
+ *                                 closed forms (mean track mfp/p, mean
+ *                                 collisions 1/p), region-volume tally
+ *                                 fractions and thread-count independence
  *   xo_adjoint                    central finite differences
  */
 #ifndef XDG_ORACLE_H
@@ -195,6 +198,57 @@ void xo_uniform2(uint32_t seed, uint32_t stream, uint64_t item, uint32_t draw,
 /* Natural log from the exponent/mantissa split and an atanh series;
  * identical bits on host and device (plain IEEE ops, no FMA). */
 double xo_log(double x);
+
+/* ---- batched particle random walk ---------------------------------------- */
+
+/* Transport_particles (tools/particle_sim.h:136-154) as a batch of
+ * independent histories, with these additions: Philox streams instead of
+ * drand48, starts uniform in a box with isotropic directions, absorption
+ * with probability p_absorb at each collision, leakage when a particle
+ * enters the implicit complement (ipc_graveyard), an event cap.
+ *
+ * Per history h (Philox counter item = first + i, key (seed, stream); the
+ * draw counter dc starts at 0 and advances one block at a time):
+ *   start      blocks 0,1: r = lo + (hi-lo)*(ua0, ub0, ua1); dc = 2
+ *   direction  Marsaglia rejection, one block per attempt
+ *   volume     XDG::find_volume(r, u) over the given volume order
+ *   event      ray_fire(tree(vol), r, u, INFTY, EXITING, history) with the
+ *              last XO_TRANSPORT_HISTORY hit faces since the last collision
+ *              excluded (the hit face is appended, ray_tracer.cpp:324);
+ *              one block: dcol = -log(1 - ua) * mfp (xo_log); s = min(dcol,
+ *              t_hit); r += s * u; tally[vol] += s; if dcol < t_hit:
+ *              absorbed when ub < p_absorb, else new direction, history
+ *              cleared; otherwise vol = next_volume(vol, surf), leaked when
+ *              that is ipc.
+ * Outputs per history: event count, status (XO_TR_*), sequence hash over
+ * (vol, s) as in xo_segments.  tally_fx[slot] += llrint(s * 2^32) (int64,
+ * order independent). */
+#define XO_TRANSPORT_HISTORY 8
+#define XO_TR_ALIVE 0
+#define XO_TR_ABSORBED 1
+#define XO_TR_LEAKED 2
+#define XO_TR_LOST 3
+#define XO_TR_STUCK 4
+#define XO_TR_CAPPED 5
+#define XO_TR_ERROR 6
+
+typedef struct {
+  int nvol;                   /* volumes with surface trees, find order */
+  const xo_tree* const* trees;
+  const int32_t* vol_ids;
+  int32_t ipc;
+  int nsurf_ids;              /* sense tables are indexed by surface id */
+  const int32_t* sense_fwd;
+  const int32_t* sense_rev;
+  uint32_t seed, stream;
+  double lo[3], hi[3];
+  double mfp, p_absorb;
+  int32_t max_events;
+} xo_transport_problem;
+
+void xo_transport(const xo_transport_problem* p, uint64_t first, int64_t n,
+                  int nthreads, int64_t* tally_fx, int32_t* events,
+                  uint8_t* status, uint64_t* hash);
 
 #ifdef __cplusplus
 }

@@ -197,17 +197,22 @@ def sphere5(ctx):
     return m, build_tree(ctx, m, 1), O.Tree.from_model(m, 1)
 
 
-# traversal variants: (wide, persistent)
-VARIANTS = [(True, True), (True, False), (False, False)]
+# traversal variants: (wide, persistent, ray order mode, speculative);
+# batches of 65536 rays and more are reordered when the mode is non-zero
+VARIANTS = [(True, True, 0, 0), (True, False, 0, 0), (False, False, 0, 0),
+            (True, True, 1, 0), (True, False, 2, 0), (True, False, 0, 1),
+            (True, False, 1, 1)]
 
 
-@pytest.mark.parametrize("wide,persist", VARIANTS)
+@pytest.mark.parametrize("wide,persist,order,spec", VARIANTS)
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("orientation", [0, 1, -1])
-def test_sphere_ray_fire_bitexact(ctx, torch_mod, sphere5, prec, orientation, wide, persist):
+def test_sphere_ray_fire_bitexact(ctx, torch_mod, sphere5, prec, orientation, wide, persist, order, spec):
     torch = torch_mod
     m, tree, ot = sphere5
     ctx.set_traversal(wide=wide, defer=True, persistent=persist)
+    ctx.set_ray_order(order)
+    ctx.set_option("speculative", spec)
     n = 65536
     org = P.ball_points(1, P.STREAM_POS, n, 9.0)
     dirs = P.iso_dirs(1, P.STREAM_DIR, n)
@@ -218,6 +223,8 @@ def test_sphere_ray_fire_bitexact(ctx, torch_mod, sphere5, prec, orientation, wi
                                orientation=orientation, precision=prec)
     finally:
         ctx.set_traversal()
+        ctx.set_ray_order(0)
+        ctx.set_option("speculative", 0)
     rt, rs, rf = ot.ray_fire(org, dirs, orientation=orientation)
     assert np.array_equal(host(s), rs)
     assert np.array_equal(host(f), rf)
@@ -342,13 +349,20 @@ def test_nested_find_volume_and_fire(ctx, torch_mod):
     far = (np.abs(r - 5) > 0.1) & (np.abs(r - 10) > 0.1) & (np.abs(r - 15) > 0.2)
     analytic = np.select([r < 5, r < 10, r < 15], [1, 2, 3], 4)
     assert np.array_equal(ref_vol[far], analytic[far])
-    for prec in (0, 1):
+    for prec, order, spec in [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (1, 2, 0),
+                              (0, 0, 1), (1, 1, 1)]:
         vol = host(ctx.find_volume(trees, vols, m.ipc, dev(torch, pts),
                                    dev(torch, dirs), precision=prec))
         assert np.array_equal(vol, ref_vol)
         tpr = np.array([trees[vols.index(v)] for v in vol], np.int32)
-        t, s, f = ctx.ray_fire(0, dev(torch, pts), dev(torch, dirs),
-                               tree_per_ray=dev(torch, tpr), precision=prec)
+        ctx.set_ray_order(order)
+        ctx.set_option("speculative", spec)
+        try:
+            t, s, f = ctx.ray_fire(0, dev(torch, pts), dev(torch, dirs),
+                                   tree_per_ray=dev(torch, tpr), precision=prec)
+        finally:
+            ctx.set_ray_order(0)
+            ctx.set_option("speculative", 0)
         t, s, f = host(t), host(s), host(f)
         for v in vols:
             sel = vol == v

@@ -1,0 +1,135 @@
+"""Persistent transport kernel vs the oracle's replay of the same histories.
+
+Every history is compared bit for bit: event count, final status, the hash
+of its (volume, flight length) sequence, and the fixed-point per-volume
+track-length tallies (sums of llrint(len * 2^32), order independent).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpu_util import build_tree, host
+from xdg_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+SEED, STREAM = 3, 3
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(torch_mod):
+    from xdg_amd import capi
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def lattice(ctx):
+    """3x3x3 spheres (subdivision 2, r 0.8, pitch 2) in the box [0, 6]^3."""
+    m = G.sphere_lattice(3, 2, 0.8, 2.0)
+    vols = [v for v in m.volumes if v != m.ipc]
+    trees = [build_tree(ctx, m, v) for v in vols]
+    otrees = [O.Tree.from_model(m, v) for v in vols]
+    return m, vols, trees, otrees
+
+
+def _problem(ctx, lattice, mfp=2.0, p_absorb=0.3, max_events=10000):
+    m, vols, trees, otrees = lattice
+    h = ctx.transport_create(vols, trees, m.ipc, m.senses, SEED, STREAM,
+                             [0.0] * 3, [6.0] * 3, mfp, p_absorb, max_events)
+    ref = lambda first, n: O.transport(m, otrees, vols, SEED, STREAM,
+                                       [0.0] * 3, [6.0] * 3, mfp, p_absorb,
+                                       max_events, first, n)
+    return h, ref
+
+
+def _check(out, ref):
+    assert np.array_equal(host(out["tally_fx"]), ref["tally_fx"])
+    assert np.array_equal(host(out["events"]), ref["events"])
+    assert np.array_equal(host(out["status"]), ref["status"])
+    assert np.array_equal(host(out["hash"]).view(np.uint64), ref["hash"])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_transport_matches_oracle(ctx, lattice, prec):
+    h, ref = _problem(ctx, lattice)
+    try:
+        n, first = 20000, 1000
+        r = ref(first, n)
+        assert set(np.unique(r["status"])) <= {1, 2}
+        assert (r["status"] == 1).sum() > 1000 and (r["status"] == 2).sum() > 1000
+        out = ctx.transport_run(h, first, n, len(lattice[1]), precision=prec)
+        _check(out, r)
+    finally:
+        ctx.transport_destroy(h)
+
+
+def test_transport_configs_and_autotune(ctx, lattice):
+    """Block size, LDS stack depth (down to one entry, the rest spilling to
+    global memory) and claim chunk change nothing in the results."""
+    h, ref = _problem(ctx, lattice, mfp=0.7, p_absorb=0.2)
+    try:
+        n = 8000
+        r = ref(0, n)
+        need = ctx.transport_config(h)["stack_need"]
+        assert need >= 2
+        for block, ls, chunk in [(64, 1, 1), (128, need, 4), (256, 2, 16),
+                                 (192, need - 1, 3)]:
+            ctx.transport_set_config(h, block, ls, chunk)
+            cfg = ctx.transport_config(h)
+            assert (cfg["block"], cfg["lds_stack"], cfg["chunk"]) == (block, ls, chunk)
+            _check(ctx.transport_run(h, 0, n, len(lattice[1])), r)
+        ms = ctx.transport_autotune(h, 0, 4096)
+        assert ms > 0.0
+        _check(ctx.transport_run(h, 0, n, len(lattice[1])), r)
+    finally:
+        ctx.transport_destroy(h)
+
+
+def test_transport_graph_replays(ctx, torch_mod, lattice):
+    """A captured launch replayed several times, with eager work and
+    synchronisation between replays, gives the eager result every time."""
+    torch = torch_mod
+    h, ref = _problem(ctx, lattice)
+    try:
+        n, first = 6000, 77
+        nvol = len(lattice[1])
+        eager = ctx.transport_run(h, first, n, nvol)
+        out = ctx.transport_outputs(nvol, n)
+        g = ctx.transport_capture(h, first, n, out)
+        try:
+            for rep in range(3):
+                for v in out.values():
+                    v.fill_(7)
+                torch.cuda.synchronize()
+                ctx.graph_launch(g)
+                ctx.synchronize()
+                for k in eager:
+                    assert torch.equal(out[k], eager[k]), (rep, k)
+                x = torch.randn(1 << 16, device="cuda")
+                (x * 2).sum().item()
+        finally:
+            ctx.graph_destroy(g)
+        _check(eager, ref(first, n))
+    finally:
+        ctx.transport_destroy(h)
+
+
+def test_transport_event_cap(ctx, lattice):
+    h, ref = _problem(ctx, lattice, mfp=0.05, p_absorb=0.0, max_events=7)
+    try:
+        n = 3000
+        r = ref(0, n)
+        assert (r["status"] == 5).mean() > 0.9
+        _check(ctx.transport_run(h, 0, n, len(lattice[1])), r)
+    finally:
+        ctx.transport_destroy(h)

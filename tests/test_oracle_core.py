@@ -1,0 +1,135 @@
+"""Oracle functions against what the mathematics fixes, and the library's
+exported C-ABI.  CPU only.
+
+  * Philox4x32-10: Random123 known-answer vectors (C and numpy versions);
+  * xo_log: within 2 ulp of the C library's log over random doubles (it
+    only has to be the same function on host and device; the bound pins a
+    dropped series term or a wrong ln2 split);
+  * ray_fire / point_in_volume / closest / occluded: host BVH == brute force;
+  * find_volume: analytic classification of nested spheres;
+  * segments: per-segment lengths sum to the chord; tally sums conserve it;
+  * libxdg_hip.so exports every function include/xdg_hip.h declares.
+"""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from xdg_amd import geometry as G
+from xdg_amd import philox as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+KAT = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+       ((0xffffffff,) * 4, (0xffffffff, 0xffffffff),
+        (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+       ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344),
+        (0xa4093822, 0x299f31d0), (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+
+
+@pytest.mark.parametrize("ctr,key,want", KAT)
+def test_philox_known_answers(ctr, key, want):
+    assert tuple(O.philox(ctr, key)) == want
+    got = P.philox4x32_10(*ctr, *key)
+    assert tuple(int(x) for x in got) == want
+
+
+def test_uniform2_host_generators_agree():
+    item = np.arange(1000, dtype=np.uint64) * 7919
+    ua, ub = P.uniform2(3, 4, item, 5)
+    for i in range(0, 1000, 97):
+        a, b = O.uniform2(3, 4, int(item[i]), 5)
+        assert a == ua[i] and b == ub[i]
+    assert (ua >= 0).all() and (ua < 1).all()
+
+
+def test_log_within_two_ulp():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(0, 1, 20000), np.exp(rng.uniform(-700, 700, 20000)),
+                         [1.0, 0.5, 2.0, np.nextafter(1.0, 0), np.nextafter(1.0, 2),
+                          5e-324, 1.7976931348623157e308, math.sqrt(2.0)]])
+    xs = xs[xs > 0]
+    for x in xs:
+        got, want = O.log(float(x)), math.log(float(x))
+        assert abs(got - want) <= 2.0 * math.ulp(want) or got == want, x
+    assert O.log(1.0) == 0.0
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    m = G.single_sphere(3, 10.0)
+    return m, O.Tree.from_model(m, 1, bvh=True)
+
+
+def test_bvh_equals_brute_force(sphere):
+    m, t = sphere
+    n = 4000
+    org = P.ball_points(7, 0, n, 12.0)
+    dirs = P.iso_dirs(7, 1, n)
+    for ori in (-1, 0, 1):
+        a = t.ray_fire(org, dirs, orientation=ori)
+        b = t.ray_fire(org, dirs, orientation=ori, brute=True)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    assert np.array_equal(t.point_in_volume(org, dirs), t.point_in_volume(org, dirs, brute=True))
+    for x, y in zip(t.closest(org), t.closest(org, brute=True)):
+        assert np.array_equal(x, y)
+    for x, y in zip(t.occluded(org, dirs), t.occluded(org, dirs, brute=True)):
+        assert np.array_equal(x, y)
+    # inside points: PIV true; analytic distance bounds for the icosphere
+    r = np.sqrt((org ** 2).sum(1))
+    piv = t.point_in_volume(org, dirs)
+    assert piv[r < 9.0].all() and not piv[r > 10.0].any()
+
+
+def test_find_volume_nested_analytic():
+    m = G.nested_spheres(2)
+    vols = list(m.volumes)
+    trees = [O.Tree.from_model(m, v) for v in vols]
+    n = 3000
+    pts = P.box_points(9, 0, n, [-19] * 3, [19] * 3)
+    dirs = P.iso_dirs(9, 1, n)
+    got = O.find_volume(trees, vols, m.ipc, pts, dirs)
+    r = np.sqrt((pts ** 2).sum(1))
+    # subdivision-2 icospheres lie between 0.95 r and r
+    far = (np.abs(r - 5) > 0.4) & (np.abs(r - 10) > 0.7) & (np.abs(r - 15) > 1.0)
+    want = np.select([r < 5, r < 10, r < 15], [1, 2, 3], 4)
+    assert np.array_equal(got[far], want[far])
+
+
+def test_segments_conserve_chord():
+    mesh = G.kuhn_cube(6, 0.2)
+    tm = O.TetMesh(mesh)
+    n = 500
+    p0 = P.box_points(4, 0, n, [0.01] * 3, [0.99] * 3)
+    p1 = P.box_points(4, 1, n, [0.01] * 3, [0.99] * 3)
+    cnt, hsh, tal, (off, se, sl) = tm.segments(p0, p1, tally=True, sequences=np.full(n, 512))
+    chord = np.sqrt(((p1 - p0) ** 2).sum(1))
+    for i in range(n):
+        s = sl[off[i]:off[i] + cnt[i]]
+        assert abs(s.sum() - chord[i]) <= 1e-12 * max(1.0, chord[i])
+        assert (se[off[i]:off[i] + cnt[i]] >= 0).all()
+    assert abs(tal.sum() / 2.0 ** 40 - chord.sum()) < 1e-6
+
+
+def _declared_functions():
+    txt = open(os.path.join(ROOT, "include", "xdg_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(xdg_hip_\w+)\s*\(", txt)))
+
+
+def test_library_exports_header():
+    so = os.path.join(ROOT, "xdg_amd", "lib", "libxdg_hip.so")
+    if not os.path.exists(so):
+        pytest.skip("libxdg_hip.so not built")
+    out = subprocess.run(["nm", "-D", "--defined-only", so], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(l.split()[-1] for l in out.splitlines() if l.strip())
+    declared = _declared_functions()
+    assert len(declared) > 30
+    missing = [f for f in declared if f not in exported]
+    assert not missing, missing

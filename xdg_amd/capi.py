@@ -53,6 +53,19 @@ class TreeInfo(C.Structure):
                 ("stack4", C.c_int32), ("reserved", C.c_int32)]
 
 
+class TransportProblem(C.Structure):
+    """xdg_hip_transport_problem (include/xdg_hip.h)."""
+    _fields_ = [("nvol", _i32), ("vol_ids", _vp), ("vol_trees", _vp),
+                ("ipc", _i32), ("nsurf_ids", _i32), ("sense_fwd", _vp),
+                ("sense_rev", _vp), ("seed", C.c_uint32),
+                ("stream", C.c_uint32), ("lo", C.c_double * 3),
+                ("hi", C.c_double * 3), ("mfp", C.c_double),
+                ("p_absorb", C.c_double), ("max_events", _i32)]
+
+
+# per-history transport status codes (XDG_HIP_TR_*)
+TR_ABSORBED, TR_LEAKED, TR_LOST, TR_STUCK, TR_CAPPED, TR_ERROR = 1, 2, 3, 4, 5, 6
+
 _LIB = None
 
 
@@ -69,6 +82,8 @@ def lib():
             "xdg_hip_destroy": [_vp],
             "xdg_hip_set_launch": [_vp, C.c_int, C.c_int],
             "xdg_hip_set_traversal": [_vp, C.c_int, C.c_int, C.c_int],
+            "xdg_hip_set_option": [_vp, C.c_char_p, C.c_int],
+            "xdg_hip_get_option": [_vp, C.c_char_p, _vp],
             "xdg_hip_get_launch": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "xdg_hip_synchronize": [_vp, _vp],
             "xdg_hip_build_surface_tree": [_vp, _i64, _vp, _i64, _vp, _vp, _vp,
@@ -96,6 +111,24 @@ def lib():
                                        _vp, _vp, _vp, _vp],
             "xdg_hip_status": [_vp, C.POINTER(C.c_uint32)],
             "xdg_hip_eval_primitive": [_vp, C.c_int, _i64, _vp, _vp, _vp],
+            "xdg_hip_transport_create": [_vp, C.POINTER(TransportProblem),
+                                         C.POINTER(_i32)],
+            "xdg_hip_transport_destroy": [_vp, _i32],
+            "xdg_hip_transport_set_config": [_vp, _i32, C.c_int, C.c_int,
+                                             C.c_int],
+            "xdg_hip_transport_get_config": [_vp, _i32, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int)],
+            "xdg_hip_transport_run": [_vp, _i32, C.c_uint64, _i64, C.c_int,
+                                      _vp, _vp, _vp, _vp, _vp],
+            "xdg_hip_transport_capture": [_vp, _i32, C.c_uint64, _i64,
+                                          C.c_int, _vp, _vp, _vp, _vp,
+                                          C.POINTER(_vp)],
+            "xdg_hip_graph_launch": [_vp, _vp, _vp],
+            "xdg_hip_graph_destroy": [_vp, _vp],
+            "xdg_hip_transport_autotune": [_vp, _i32, C.c_uint64, _i64,
+                                           C.c_int, C.POINTER(C.c_double)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -188,6 +221,18 @@ class Context:
             mask = int(persistent)
         self._check(lib().xdg_hip_set_traversal(self.h, int(wide), int(defer), mask))
 
+    def set_option(self, name, value):
+        """Tuning option by name (see include/xdg_hip.h)."""
+        self._check(lib().xdg_hip_set_option(self.h, name.encode(), int(value)))
+
+    def get_option(self, name):
+        v = C.c_int()
+        self._check(lib().xdg_hip_get_option(self.h, name.encode(), C.byref(v)))
+        return v.value
+
+    def set_ray_order(self, mode):
+        self.set_option("ray_order", mode)
+
     def get_launch(self):
         b, s = C.c_int(), C.c_int()
         self._check(lib().xdg_hip_get_launch(self.h, C.byref(b), C.byref(s)))
@@ -259,14 +304,14 @@ class Context:
                        stream=None):
         """Work counters of the instrumented traversal (synchronous)."""
         n = org.shape[0]
-        cnt = (C.c_uint64 * 10)()
+        cnt = (C.c_uint64 * 12)()
         k = 0 if excl is None else excl.shape[1]
         self._check(lib().xdg_hip_ray_fire_stats(
             self.h, tree, _ptr(tree_per_ray), n, _ptr(org), _ptr(dirs),
             _ptr(tmax), orientation, _ptr(excl), k, precision, cnt))
         keys = ["rays", "nodes", "tri_tests", "edge_tests", "finishes",
                 "normals", "hits", "fallbacks", "wave_node_iters",
-                "wave_tri_iters"]
+                "wave_tri_iters", "wave_max_nodes", "wave_max_tris"]
         return dict(zip(keys, [int(x) for x in cnt]))
 
     @_ordered
@@ -342,6 +387,93 @@ class Context:
             self.h, etree, n, _ptr(p0), _ptr(p1), _ptr(tally), _ptr(cnt),
             _ptr(hsh), _ptr(seq_off), _ptr(seq_elem), _ptr(seq_len), stream), stream)
         return cnt, hsh
+
+    # ---- batched particle random walk ------------------------------------
+    def transport_create(self, vol_ids, vol_trees, ipc, senses, seed, stream,
+                         lo, hi, mfp, p_absorb, max_events=10000):
+        """Register a transport problem.  senses: {surface id: (forward
+        volume, reverse volume)}.  Returns the handle."""
+        ids = np.ascontiguousarray(vol_ids, dtype=np.int32)
+        trees = np.ascontiguousarray(vol_trees, dtype=np.int32)
+        ns = max(senses) + 1
+        fwd = np.full(ns, -1, np.int32)
+        rev = np.full(ns, -1, np.int32)
+        for sid, (f, r) in senses.items():
+            fwd[sid] = f
+            rev[sid] = r
+        p = TransportProblem()
+        p.nvol = ids.size
+        p.vol_ids = ids.ctypes.data
+        p.vol_trees = trees.ctypes.data
+        p.ipc = int(ipc)
+        p.nsurf_ids = ns
+        p.sense_fwd = fwd.ctypes.data
+        p.sense_rev = rev.ctypes.data
+        p.seed = int(seed)
+        p.stream = int(stream)
+        p.lo = (C.c_double * 3)(*[float(x) for x in lo])
+        p.hi = (C.c_double * 3)(*[float(x) for x in hi])
+        p.mfp = float(mfp)
+        p.p_absorb = float(p_absorb)
+        p.max_events = int(max_events)
+        h = _i32()
+        self._check(lib().xdg_hip_transport_create(self.h, C.byref(p), C.byref(h)))
+        return h.value
+
+    def transport_destroy(self, handle):
+        self._check(lib().xdg_hip_transport_destroy(self.h, handle))
+
+    def transport_set_config(self, handle, block=0, lds_stack=0, chunk=0):
+        self._check(lib().xdg_hip_transport_set_config(self.h, handle, block,
+                                                       lds_stack, chunk))
+
+    def transport_config(self, handle):
+        v = [C.c_int() for _ in range(4)]
+        self._check(lib().xdg_hip_transport_get_config(
+            self.h, handle, *[C.byref(x) for x in v]))
+        return dict(zip(["block", "lds_stack", "chunk", "stack_need"],
+                        [x.value for x in v]))
+
+    def transport_outputs(self, nvol, n, per_history=True):
+        z = lambda *a, **k: torch.zeros(*a, device=self.dev, **k)
+        out = {"tally_fx": z(nvol, dtype=torch.int64)}
+        if per_history:
+            out.update(events=z(n, dtype=torch.int32),
+                       status=z(n, dtype=torch.uint8),
+                       hash=z(n, dtype=torch.int64))
+        return out
+
+    @_ordered
+    def transport_run(self, handle, first, n, nvol, precision=MIXED,
+                      out=None, per_history=True, stream=None):
+        if out is None:
+            out = self.transport_outputs(nvol, n, per_history)
+        self._done(lib().xdg_hip_transport_run(
+            self.h, handle, int(first), int(n), precision,
+            _ptr(out["tally_fx"]), _ptr(out.get("events")),
+            _ptr(out.get("status")), _ptr(out.get("hash")), stream), stream)
+        return out
+
+    def transport_capture(self, handle, first, n, out, precision=MIXED):
+        """Instantiated graph of one transport launch writing into out."""
+        g = _vp()
+        self._check(lib().xdg_hip_transport_capture(
+            self.h, handle, int(first), int(n), precision,
+            _ptr(out["tally_fx"]), _ptr(out.get("events")),
+            _ptr(out.get("status")), _ptr(out.get("hash")), C.byref(g)))
+        return g.value
+
+    def graph_launch(self, graph, stream=None):
+        self._check(lib().xdg_hip_graph_launch(self.h, graph, stream))
+
+    def graph_destroy(self, graph):
+        self._check(lib().xdg_hip_graph_destroy(self.h, graph))
+
+    def transport_autotune(self, handle, first, sample, precision=MIXED):
+        ms = C.c_double()
+        self._check(lib().xdg_hip_transport_autotune(
+            self.h, handle, int(first), int(sample), precision, C.byref(ms)))
+        return ms.value
 
     def status(self):
         f = C.c_uint32()

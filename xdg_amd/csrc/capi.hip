@@ -28,6 +28,20 @@ struct SurfTreeHost {
   int64_t bytes = 0;
 };
 
+// One registered transport problem: device copies of the topology, the
+// tuned launch configuration and the buffers a launch needs (so a launch
+// allocates nothing and can be captured in a graph).
+struct TransportHost {
+  TransportDev dev{};
+  std::vector<DevBuf> bufs;
+  int block = 128, lds_stack = 0, chunk = 1;
+  int stack_need = 1;
+  int32_t* spill = nullptr;
+  size_t spill_entries = 0;
+  unsigned long long* d_next = nullptr;
+  std::vector<int32_t> host_trees;
+};
+
 struct ElemTreeHost {
   ElemTreeDev dev{};
   std::vector<DevBuf> bufs;
@@ -78,6 +92,10 @@ struct xdg_hip_ctx {
   size_t fb_cap = 0;
   unsigned int* d_fb_count = nullptr;
   unsigned long long* d_next = nullptr;  // persistent ray_fire: ray counter
+  void* d_order = nullptr;  // ray ordering: keys, indices and sort storage
+  int64_t order_cap = 0;
+  RayOrderScratch order_w{};
+  std::vector<TransportHost*> transports;  // handle = index + 1 (nullptr once destroyed)
   LaunchCfg cfg;
   int leaf_max = 4;
   int max_surf_depth = 1;
@@ -152,6 +170,29 @@ LaunchCfg cfg_for(xdg_hip_ctx* c, int depth)
   return k;
 }
 
+// ray ordering buffers for n rays (grown on demand, kept by the context)
+const RayOrderScratch& order_scratch(xdg_hip_ctx* c, int64_t n)
+{
+  if (n > c->order_cap) {
+    XDG_HIP_CHECK(hipDeviceSynchronize());
+    if (c->d_order) XDG_HIP_CHECK(hipFree(c->d_order));
+    c->d_order = nullptr;
+    c->order_cap = 0;
+    const size_t a = ((size_t)n * 4 + 255) & ~(size_t)255;
+    const size_t tb = ray_order_temp_bytes(n);
+    char* p = nullptr;
+    XDG_HIP_CHECK(hipMalloc((void**)&p, 4 * a + tb));
+    c->d_order = p;
+    c->order_cap = n;
+    c->order_w = RayOrderScratch{(uint32_t*)p, (uint32_t*)(p + a), (int32_t*)(p + 2 * a), (int32_t*)(p + 3 * a),
+                                 (void*)(p + 4 * a), tb};
+  }
+  return c->order_w;
+}
+
+// rays below this count are traced in the caller's order
+constexpr int64_t ORDER_MIN_RAYS = 1 << 16;
+
 template <class F>
 int guarded(xdg_hip_ctx* c, F&& f)
 {
@@ -203,7 +244,9 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   c->leaf_max = env_int("XDG_HIP_LEAF", 4);
   c->cfg.defer = env_int("XDG_HIP_DEFER", 1) != 0;
   c->cfg.wide = env_int("XDG_HIP_WIDE", 1) != 0;
+  c->cfg.spec = env_int("XDG_HIP_SPEC", 0) != 0;
   c->cfg.persistent = env_int("XDG_HIP_PERSIST", 1) & 3;
+  c->cfg.order = env_int("XDG_HIP_ORDER", 0);
   try {
     sync_tables(c);
   } catch (...) {
@@ -223,6 +266,12 @@ int xdg_hip_destroy(xdg_hip_ctx* c)
     (void)hipDeviceSynchronize();
     for (auto& t : c->surf) dev_free_all(t.bufs);
     for (auto& t : c->elem) dev_free_all(t.bufs);
+    for (auto* t : c->transports) {
+      if (!t) continue;
+      if (t->spill) (void)hipFree(t->spill);
+      dev_free_all(t->bufs);
+      delete t;
+    }
     if (c->d_surf) (void)hipFree(c->d_surf);
     if (c->d_elem) (void)hipFree(c->d_elem);
     if (c->d_small) (void)hipFree(c->d_small);
@@ -230,6 +279,7 @@ int xdg_hip_destroy(xdg_hip_ctx* c)
     if (c->d_fb_list) (void)hipFree(c->d_fb_list);
     if (c->d_fb_count) (void)hipFree(c->d_fb_count);
     if (c->d_next) (void)hipFree(c->d_next);
+    if (c->d_order) (void)hipFree(c->d_order);
     if (c->stream) (void)hipStreamDestroy(c->stream);
   }
   delete c;
@@ -255,6 +305,65 @@ int xdg_hip_set_traversal(xdg_hip_ctx* c, int wide, int defer, int persistent)
     c->cfg.wide = wide != 0;
     c->cfg.defer = defer != 0;
     c->cfg.persistent = persistent & 3;
+    return XDG_HIP_OK;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+struct OptionRef {
+  const char* name;
+  int lo, hi;
+  int* (*get)(xdg_hip_ctx*);
+};
+
+int* opt_block(xdg_hip_ctx* c) { return &c->cfg.block; }
+int* opt_stack(xdg_hip_ctx* c) { return &c->cfg.stack; }
+int* opt_persistent(xdg_hip_ctx* c) { return &c->cfg.persistent; }
+int* opt_order(xdg_hip_ctx* c) { return &c->cfg.order; }
+int* opt_spec(xdg_hip_ctx* c) { return &c->cfg.spec; }
+int* opt_wide(xdg_hip_ctx* c) { return &c->cfg.wide; }
+int* opt_defer(xdg_hip_ctx* c) { return &c->cfg.defer; }
+int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
+
+const OptionRef OPTIONS[] = {
+    {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 3, opt_persistent},
+    {"ray_order", 0, 2, opt_order},         {"speculative", 0, 1, opt_spec}, {"wide", 0, 1, opt_wide},
+    {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
+};
+
+const OptionRef& find_option(const char* name)
+{
+  require(name != nullptr, "null option name");
+  for (const auto& o : OPTIONS)
+    if (std::strcmp(o.name, name) == 0) return o;
+  throw NotFound{std::string("unknown option '") + name + "'"};
+}
+
+}  // namespace
+
+extern "C" {
+
+int xdg_hip_set_option(xdg_hip_ctx* c, const char* name, int value)
+{
+  return guarded(c, [&]() {
+    const OptionRef& o = find_option(name);
+    if (value < o.lo || value > o.hi)
+      throw InvalidArg{std::string("option '") + name + "' must be in [" + std::to_string(o.lo) + ", " +
+                       std::to_string(o.hi) + "]"};
+    if (std::strcmp(name, "block") == 0) require((value & 63) == 0, "block must be a multiple of 64");
+    *o.get(c) = value;
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_get_option(xdg_hip_ctx* c, const char* name, int* value)
+{
+  return guarded(c, [&]() {
+    require(value != nullptr, "null output");
+    *value = *find_option(name).get(c);
     return XDG_HIP_OK;
   });
 }
@@ -340,6 +449,16 @@ int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
       t.dev.n_prims = (int32_t)n_tris;
       t.dev.depth = lb.depth;
       t.dev.bump = bump;
+      for (int k = 0; k < 3; ++k) {
+        double lo = 0.0, hi = 0.0;
+        for (int64_t i = 0; i < n_tris * 3; ++i) {
+          const double x = v9[3 * i + k];
+          if (i == 0 || x < lo) lo = x;
+          if (i == 0 || x > hi) hi = x;
+        }
+        t.dev.box_lo[k] = (float)lo;
+        t.dev.box_hi[k] = (float)hi;
+      }
       double lim = bump * 1048576.0 - lb.max_abs;  // bump * 2^20 - max |coordinate|
       t.dev.o_limit = lim > 0.0 ? (float)lim : 0.0f;
       if ((double)t.dev.o_limit > lim) t.dev.o_limit = nextafterf(t.dev.o_limit, 0.0f);
@@ -527,6 +646,10 @@ int xdg_hip_ray_fire(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per_ray, 
       a.fb_count = c->d_fb_count;
     }
     a.next_ray = c->d_next;
+    if (c->cfg.order > 0 && n >= ORDER_MIN_RAYS) {
+      RayOrderArgs o{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir};
+      a.order = launch_ray_order(o, c->cfg.order, order_scratch(c, n), pick(c, stream));
+    }
     launch_ray_fire(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
@@ -553,6 +676,10 @@ int xdg_hip_ray_fire_stats(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per
       XDG_HIP_CHECK(hipMemcpy(d_cnt, counters, XDG_HIP_NUM_COUNTERS * sizeof(uint64_t), hipMemcpyHostToDevice));
       RayFireArgs a{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir, tmax, orientation,
                     excl, excl ? k : 0, nullptr, nullptr, nullptr};
+      if (c->cfg.order > 0 && n >= ORDER_MIN_RAYS) {
+        RayOrderArgs o{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir};
+        a.order = launch_ray_order(o, c->cfg.order, order_scratch(c, n), c->stream);
+      }
       launch_ray_fire_stats(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), d_cnt, c->stream);
       XDG_HIP_CHECK(hipGetLastError());
       XDG_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -795,6 +922,317 @@ int xdg_hip_memcpy_dtoh(xdg_hip_ctx* c, void* dst, const void* src, size_t bytes
 {
   return guarded(c, [&]() {
     if (bytes) XDG_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return XDG_HIP_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---- batched particle random walk ------------------------------------------
+
+namespace {
+
+TransportHost& transport_of(xdg_hip_ctx* c, int32_t handle)
+{
+  if (handle < 1 || (size_t)handle > c->transports.size() || !c->transports[handle - 1])
+    throw NotFound{"transport problem " + std::to_string(handle) + " is not registered"};
+  return *c->transports[handle - 1];
+}
+
+// refresh the tree table pointer (trees may have been added since)
+void transport_refresh(xdg_hip_ctx* c, TransportHost& t)
+{
+  t.dev.table = c->d_surf;
+  t.dev.ntrees = (int32_t)c->surf.size();
+  int need = 1;
+  for (int32_t id : t.host_trees) need = std::max(need, surf_tree(c, id).dev.stack4);
+  t.stack_need = need;
+  if (t.lds_stack <= 0 || t.lds_stack > need) t.lds_stack = need;
+}
+
+// global stack share for the shape (outside any capture: may allocate)
+TransportLaunch transport_prepare(xdg_hip_ctx* c, TransportHost& t, bool mixed, bool may_alloc)
+{
+  transport_refresh(c, t);
+  const TransportLaunch L = transport_shape(mixed, t.block, t.lds_stack, t.dev.nvol);
+  const size_t need = (size_t)(t.stack_need - t.lds_stack) * (size_t)(L.grid * L.block);
+  if (need > t.spill_entries) {
+    if (!may_alloc) throw InvalidArg{"transport launch needs a larger stack spill area; run it once before capture"};
+    XDG_HIP_CHECK(hipDeviceSynchronize());
+    if (t.spill) XDG_HIP_CHECK(hipFree(t.spill));
+    t.spill = nullptr;
+    t.spill_entries = 0;
+    XDG_HIP_CHECK(hipMalloc(&t.spill, need * sizeof(int32_t)));
+    t.spill_entries = need;
+  }
+  return L;
+}
+
+void transport_launch(xdg_hip_ctx* c, TransportHost& t, const TransportLaunch& L, bool mixed, uint64_t first,
+                      int64_t n, uint64_t* tally, int32_t* events, uint8_t* status, uint64_t* hash, hipStream_t s)
+{
+  TransportRun R{};
+  R.first = first;
+  R.n = n;
+  R.next = t.d_next;
+  R.tally_fx = (unsigned long long*)tally;
+  R.events = events;
+  R.status = status;
+  R.hash = hash;
+  R.chunk = t.chunk;
+  R.lds_stack = t.lds_stack;
+  R.stack_cap = t.stack_need;
+  R.spill = t.stack_need > t.lds_stack ? t.spill : nullptr;
+  R.spill_lanes = L.grid * L.block;
+  R.lds_tally = t.dev.nvol <= TRANSPORT_LDS_TALLY_MAX ? 1 : 0;
+  (void)c;
+  launch_transport(t.dev, R, mixed, L, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int xdg_hip_transport_create(xdg_hip_ctx* c, const xdg_hip_transport_problem* p, int32_t* handle)
+{
+  return guarded(c, [&]() {
+    require(p && handle, "null argument");
+    require(p->nvol >= 1 && p->vol_ids && p->vol_trees, "need at least one volume");
+    require(p->nsurf_ids >= 1 && p->sense_fwd && p->sense_rev, "need the surface sense tables");
+    require(std::isfinite(p->mfp) && p->mfp > 0.0, "mfp must be finite and > 0");
+    require(p->p_absorb >= 0.0 && p->p_absorb <= 1.0, "p_absorb must be in [0, 1]");
+    require(p->max_events >= 1, "max_events must be >= 1");
+    for (int k = 0; k < 3; ++k) require(std::isfinite(p->lo[k]) && std::isfinite(p->hi[k]), "start box not finite");
+    int32_t max_id = p->ipc;
+    for (int v = 0; v < p->nvol; ++v) {
+      require(p->vol_ids[v] >= 0, "volume ids must be >= 0");
+      surf_tree(c, p->vol_trees[v]);
+      max_id = std::max(max_id, p->vol_ids[v]);
+    }
+    require(max_id < (1 << 24), "volume ids must be < 2^24");
+    for (int i = 0; i < p->nsurf_ids; ++i) {
+      max_id = std::max(max_id, std::max(p->sense_fwd[i], p->sense_rev[i]));
+    }
+    std::vector<int32_t> id2slot((size_t)max_id + 1, -1);
+    for (int v = 0; v < p->nvol; ++v) {
+      require(id2slot[p->vol_ids[v]] < 0, "duplicate volume id");
+      id2slot[p->vol_ids[v]] = v;
+    }
+    auto* t = new TransportHost();
+    try {
+      auto up = [&](const int32_t* src, size_t n) {
+        int32_t* d = (int32_t*)dev_alloc(n * sizeof(int32_t), t->bufs);
+        XDG_HIP_CHECK(hipMemcpy(d, src, n * sizeof(int32_t), hipMemcpyHostToDevice));
+        return d;
+      };
+      t->dev.nvol = p->nvol;
+      t->dev.vol_tree = up(p->vol_trees, (size_t)p->nvol);
+      t->dev.vol_id = up(p->vol_ids, (size_t)p->nvol);
+      t->dev.id2slot = up(id2slot.data(), id2slot.size());
+      t->dev.n_ids = (int32_t)id2slot.size();
+      t->dev.ipc = p->ipc;
+      t->dev.nsurf = p->nsurf_ids;
+      t->dev.sense_fwd = up(p->sense_fwd, (size_t)p->nsurf_ids);
+      t->dev.sense_rev = up(p->sense_rev, (size_t)p->nsurf_ids);
+      t->dev.seed = p->seed;
+      t->dev.stream = p->stream;
+      for (int k = 0; k < 3; ++k) {
+        t->dev.lo[k] = p->lo[k];
+        t->dev.hi[k] = p->hi[k];
+      }
+      t->dev.mfp = p->mfp;
+      t->dev.p_absorb = p->p_absorb;
+      t->dev.max_events = p->max_events;
+      t->d_next = (unsigned long long*)dev_alloc(64, t->bufs);
+      t->host_trees.assign(p->vol_trees, p->vol_trees + p->nvol);
+      t->block = c->cfg.block > 256 ? 256 : c->cfg.block;
+      transport_refresh(c, *t);
+    } catch (...) {
+      dev_free_all(t->bufs);
+      delete t;
+      throw;
+    }
+    c->transports.push_back(t);
+    *handle = (int32_t)c->transports.size();
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_transport_destroy(xdg_hip_ctx* c, int32_t handle)
+{
+  return guarded(c, [&]() {
+    TransportHost& t = transport_of(c, handle);
+    XDG_HIP_CHECK(hipDeviceSynchronize());
+    if (t.spill) XDG_HIP_CHECK(hipFree(t.spill));
+    dev_free_all(t.bufs);
+    delete &t;
+    c->transports[handle - 1] = nullptr;
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_transport_set_config(xdg_hip_ctx* c, int32_t handle, int block, int lds_stack, int chunk)
+{
+  return guarded(c, [&]() {
+    TransportHost& t = transport_of(c, handle);
+    require(block == 0 || (block >= 64 && block <= 256 && block % 64 == 0), "block must be 64, 128, 192 or 256");
+    require(lds_stack >= 0 && lds_stack <= 64, "lds_stack must be in [0, 64]");
+    require(chunk >= 0 && chunk <= 4096, "chunk must be in [0, 4096]");
+    if (block) t.block = block;
+    if (lds_stack) t.lds_stack = lds_stack;
+    if (chunk) t.chunk = chunk;
+    transport_refresh(c, t);
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_transport_get_config(xdg_hip_ctx* c, int32_t handle, int* block, int* lds_stack, int* chunk,
+                                 int* stack_need)
+{
+  return guarded(c, [&]() {
+    TransportHost& t = transport_of(c, handle);
+    transport_refresh(c, t);
+    if (block) *block = t.block;
+    if (lds_stack) *lds_stack = t.lds_stack;
+    if (chunk) *chunk = t.chunk;
+    if (stack_need) *stack_need = t.stack_need;
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_transport_run(xdg_hip_ctx* c, int32_t handle, uint64_t first, int64_t n, int precision,
+                          uint64_t* tally_fx, int32_t* events, uint8_t* status, uint64_t* hash, void* stream)
+{
+  return guarded(c, [&]() {
+    TransportHost& t = transport_of(c, handle);
+    require(n >= 0, "negative n");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    require(tally_fx != nullptr, "tally_fx is null");
+    hipStream_t s = pick(c, stream);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    XDG_HIP_CHECK(hipStreamIsCapturing(s, &cs));
+    const bool mixed = precision == XDG_HIP_MIXED;
+    const TransportLaunch L = transport_prepare(c, t, mixed, cs == hipStreamCaptureStatusNone);
+    transport_launch(c, t, L, mixed, first, n, tally_fx, events, status, hash, s);
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_transport_capture(xdg_hip_ctx* c, int32_t handle, uint64_t first, int64_t n, int precision,
+                              uint64_t* tally_fx, int32_t* events, uint8_t* status, uint64_t* hash, void** graph)
+{
+  return guarded(c, [&]() {
+    TransportHost& t = transport_of(c, handle);
+    require(n >= 0, "negative n");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    require(tally_fx != nullptr && graph != nullptr, "null argument");
+    const bool mixed = precision == XDG_HIP_MIXED;
+    const TransportLaunch L = transport_prepare(c, t, mixed, true);
+    XDG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    hipGraph_t g = nullptr;
+    XDG_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      transport_launch(c, t, L, mixed, first, n, tally_fx, events, status, hash, c->stream);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    XDG_HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    XDG_HIP_CHECK(e);
+    *graph = (void*)ex;
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_graph_launch(xdg_hip_ctx* c, void* graph, void* stream)
+{
+  return guarded(c, [&]() {
+    require(graph != nullptr, "null graph");
+    XDG_HIP_CHECK(hipGraphLaunch((hipGraphExec_t)graph, pick(c, stream)));
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_graph_destroy(xdg_hip_ctx* c, void* graph)
+{
+  return guarded(c, [&]() {
+    if (graph) XDG_HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph));
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_transport_autotune(xdg_hip_ctx* c, int32_t handle, uint64_t first, int64_t sample, int precision,
+                               double* best_ms)
+{
+  return guarded(c, [&]() {
+    TransportHost& t = transport_of(c, handle);
+    require(sample >= 1, "sample must be >= 1");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    const bool mixed = precision == XDG_HIP_MIXED;
+    transport_refresh(c, t);
+    std::vector<DevBuf> tmp;
+    uint64_t* d_tally = (uint64_t*)dev_alloc((size_t)t.dev.nvol * 8, tmp);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    XDG_HIP_CHECK(hipEventCreate(&e0));
+    XDG_HIP_CHECK(hipEventCreate(&e1));
+    const int keep[3] = {t.block, t.lds_stack, t.chunk};
+    double best = -1.0;
+    int bb = keep[0], bs = keep[1], bc = keep[2];
+    try {
+      std::vector<int> stacks = {std::min(t.stack_need, 8), std::min(t.stack_need, 16), t.stack_need};
+      std::sort(stacks.begin(), stacks.end());
+      stacks.erase(std::unique(stacks.begin(), stacks.end()), stacks.end());
+      for (int block : {64, 128, 256})
+        for (int ls : stacks)
+          for (int chunk : {1, 4, 16}) {
+            t.block = block;
+            t.lds_stack = ls;
+            t.chunk = chunk;
+            TransportLaunch L;
+            try {
+              L = transport_prepare(c, t, mixed, true);
+            } catch (const HipError&) {
+              continue;  // shape does not fit a CU
+            }
+            transport_launch(c, t, L, mixed, first, sample, d_tally, nullptr, nullptr, nullptr, c->stream);
+            XDG_HIP_CHECK(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < 2; ++r)
+              transport_launch(c, t, L, mixed, first, sample, d_tally, nullptr, nullptr, nullptr, c->stream);
+            XDG_HIP_CHECK(hipEventRecord(e1, c->stream));
+            XDG_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            XDG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (best < 0.0 || ms * 0.5 < best) {
+              best = ms * 0.5;
+              bb = block;
+              bs = ls;
+              bc = chunk;
+            }
+          }
+    } catch (...) {
+      t.block = keep[0];
+      t.lds_stack = keep[1];
+      t.chunk = keep[2];
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      dev_free_all(tmp);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    XDG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    dev_free_all(tmp);
+    require(best >= 0.0, "no launch shape fits");
+    t.block = bb;
+    t.lds_stack = bs;
+    t.chunk = bc;
+    transport_prepare(c, t, mixed, true);
+    if (best_ms) *best_ms = best;
     return XDG_HIP_OK;
   });
 }

@@ -26,17 +26,6 @@ namespace {
 
 constexpr int MAX_WALK_STEPS = 1 << 16;
 constexpr int MAX_SEG_LOOPS = 64;
-constexpr unsigned long long HASH_SEED = 0x6a09e667f3bcc909ULL;
-
-__device__ __forceinline__ unsigned long long mix64(unsigned long long x)
-{
-  x ^= x >> 30;
-  x *= 0xbf58476d1ce4e5b9ULL;
-  x ^= x >> 27;
-  x *= 0x94d049bb133111ebULL;
-  x ^= x >> 31;
-  return x;
-}
 
 __device__ __forceinline__ float f32_dn(double x)
 {

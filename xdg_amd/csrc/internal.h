@@ -79,21 +79,49 @@ struct RayFireArgs {
   unsigned int* fb_count;
   // persistent kernels (nullable): next unclaimed ray of the batch
   unsigned long long* next_ray;
+  // nullable: slot j of the launch traces ray order[j] (results still land
+  // at the ray's own index)
+  const int32_t* order;
 };
+
+// ---- ray ordering (ray_order.hip) -----------------------------------------
+struct RayOrderArgs {
+  const xdgd::SurfTreeDev* table;
+  int32_t ntrees;
+  int32_t tree;
+  const int32_t* tree_per_ray;
+  int64_t n;
+  const double* org;
+  const double* dir;
+};
+struct RayOrderScratch {
+  uint32_t* keys_in;
+  uint32_t* keys_out;
+  int32_t* idx_in;
+  int32_t* idx_out;
+  void* temp;
+  size_t temp_bytes;
+};
+size_t ray_order_temp_bytes(int64_t n);
+// mode 1: (tree, octant, origin Morton); mode 2: adds one coarse direction
+// bit per axis.  Returns the permutation (w.idx_out), valid on stream s.
+const int32_t* launch_ray_order(const RayOrderArgs& a, int mode, const RayOrderScratch& w, hipStream_t s);
 
 struct LaunchCfg {
   int block = 256;
   int stack = 64;     // LDS stack entries per lane
-  bool defer = true;  // mixed ray_fire: FP64 pass for far-origin rays
-  bool wide = true;   // surface queries traverse the BVH4
+  int defer = 1;  // mixed ray_fire: FP64 pass for far-origin rays
+  int wide = 1;   // surface queries traverse the BVH4
   // ray_fire on a resident grid with per-lane ray refill, per precision:
   // bit 0 plain, bit 1 mixed
   int persistent = 1;
+  int order = 0;  // ray_fire coherence ordering mode (0 = off)
+  int spec = 0;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
-// counters[10] (device, zeroed): rays, V, F, E, C, N, hits, mixed fallbacks,
-// wave node iterations, wave triangle iterations
+// counters[12] (device, zeroed): rays, V, F, E, C, N, hits, mixed fallbacks,
+// wave node iterations, wave triangle iterations, per-wave max V, max F
 void launch_ray_fire_stats(const RayFireArgs& a, bool mixed, const LaunchCfg& c, unsigned long long* counters,
                            hipStream_t s);
 
@@ -148,6 +176,53 @@ struct ClosestArgs {
   int32_t* face;
 };
 void launch_closest(const ClosestArgs& a, const LaunchCfg& c, hipStream_t s);
+
+// ---- batched particle random walk (transport.hip) -------------------------
+// Device-resident problem (arrays owned by the context's transport handle).
+struct TransportDev {
+  const xdgd::SurfTreeDev* table;
+  int32_t ntrees;
+  int32_t nvol;
+  const int32_t* vol_tree;  // slot -> surface tree id
+  const int32_t* vol_id;    // slot -> volume id (find_volume order)
+  const int32_t* id2slot;   // volume id -> slot (-1), ids [0, n_ids)
+  int32_t n_ids;
+  int32_t ipc;
+  int32_t nsurf;            // sense tables indexed by surface id [0, nsurf)
+  const int32_t* sense_fwd;
+  const int32_t* sense_rev;
+  uint32_t seed, stream;
+  double lo[3], hi[3];
+  double mfp, p_absorb;
+  int32_t max_events;
+};
+// One launch over histories [first, first + n).
+struct TransportRun {
+  uint64_t first;
+  int64_t n;
+  unsigned long long* next;      // claim counter (zeroed by the launcher)
+  unsigned long long* tally_fx;  // per slot, int64 two's complement, scale 2^32
+  int32_t* events;               // nullable, per history
+  uint8_t* status;               // nullable
+  uint64_t* hash;                // nullable
+  int chunk;                     // histories a lane claims at a time
+  int lds_stack;                 // stack entries kept in LDS
+  int stack_cap;                 // total stack entries (LDS + global)
+  int32_t* spill;                // global stack share: (cap - lds) * spill_lanes
+  int64_t spill_lanes;
+  int lds_tally;                 // per-workgroup LDS tally (nvol small enough)
+};
+struct TransportLaunch {
+  int block;
+  int64_t grid;
+  size_t lds_bytes;
+};
+// Shape of the persistent grid for (block, lds_stack, nvol): resident
+// workgroups on the whole device.
+TransportLaunch transport_shape(bool mixed, int block, int lds_stack, int nvol);
+void launch_transport(const TransportDev& P, const TransportRun& R, bool mixed, const TransportLaunch& L,
+                      hipStream_t s);
+constexpr int TRANSPORT_LDS_TALLY_MAX = 4096;
 
 // ---- element queries (element_query.hip) ---------------------------------
 struct FindElementArgs {

@@ -37,7 +37,7 @@ constexpr int MAXK = 8;
 
 // One ray of ray_fire.  DEFER (mixed only): a ray beyond the FP32-rounding
 // limit is queued for the FP64 pass instead of traced here.
-template <bool MIXED, bool DEFER, bool WIDE>
+template <bool MIXED, bool DEFER, bool WIDE, bool SPEC = false>
 __device__ __forceinline__ void ray_fire_one(const RayFireArgs& a, int64_t i, int32_t* stk, int cap)
 {
   int32_t tree = a.tree_per_ray ? a.tree_per_ray[i] : a.tree;
@@ -64,9 +64,9 @@ __device__ __forceinline__ void ray_fire_one(const RayFireArgs& a, int64_t i, in
   h.surf = ID_NONE;
   h.inside = false;
   if (DEFER)
-    trace_f32<RAYFIRE, false, WIDE>(T, ray, a.orientation, ex, k, stk, blockDim.x, cap, h);
+    trace_f32<RAYFIRE, false, WIDE, SPEC>(T, ray, a.orientation, ex, k, stk, blockDim.x, cap, h);
   else
-    trace<RAYFIRE, MIXED, false, WIDE>(T, ray, a.orientation, ex, k, stk, blockDim.x, cap, h);
+    trace<RAYFIRE, MIXED, false, WIDE, SPEC>(T, ray, a.orientation, ex, k, stk, blockDim.x, cap, h);
   if (h.surf == ID_NONE) {
     tout[i] = INFTY;
     a.surf_out[i] = ID_NONE;
@@ -78,13 +78,13 @@ __device__ __forceinline__ void ray_fire_one(const RayFireArgs& a, int64_t i, in
   }
 }
 
-template <bool MIXED, bool DEFER, bool WIDE>
+template <bool MIXED, bool DEFER, bool WIDE, bool SPEC = false>
 __global__ void k_ray_fire(RayFireArgs a, int cap)
 {
   extern __shared__ int32_t lds[];
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= a.n) return;
-  ray_fire_one<MIXED, DEFER, WIDE>(a, i, lds + threadIdx.x, cap);
+  ray_fire_one<MIXED, DEFER, WIDE, SPEC>(a, a.order ? (int64_t)a.order[i] : i, lds + threadIdx.x, cap);
 }
 
 // FP64 pass over the rays the mixed kernel queued (grid-stride; the count is
@@ -153,8 +153,9 @@ __global__ void k_ray_fire_pt(RayFireArgs a, int cap, unsigned long long* next_r
       base = __shfl(b, leader);
     }
     if (need) {
-      ri = (int64_t)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
-      if (ri >= a.n) break;
+      const int64_t slot = (int64_t)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
+      if (slot >= a.n) break;
+      ri = a.order ? (int64_t)a.order[slot] : slot;
       const int32_t tree = a.tree_per_ray ? a.tree_per_ray[ri] : a.tree;
       if (tree < 1 || tree > a.ntrees) {
         HitState miss{INFTY, INT32_MAX, ID_NONE, false};
@@ -231,11 +232,12 @@ __global__ void k_ray_fire_pt(RayFireArgs a, int cap, unsigned long long* next_r
 // Instrumented ray_fire: identical traversal, plus per-ray work counters
 // summed into counters[0..7] = rays, V, F, E, C, N, hits, mixed-fallbacks.
 template <bool MIXED, bool WIDE>
-__global__ void k_ray_fire_stats(RayFireArgs a, int cap, unsigned long long* counters)
+__global__ void k_ray_fire_stats(RayFireArgs a, int cap, unsigned long long* counters, int spec)
 {
   extern __shared__ int32_t lds[];
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= a.n) return;
+  if (a.order) i = a.order[i];
   int32_t tree = a.tree_per_ray ? a.tree_per_ray[i] : a.tree;
   if (tree < 1 || tree > a.ntrees) return;
   const SurfTreeDev& T = a.table[tree];
@@ -249,7 +251,10 @@ __global__ void k_ray_fire_stats(RayFireArgs a, int cap, unsigned long long* cou
   h.surf = ID_NONE;
   h.inside = false;
   TraceStats st{0, 0, 0, 0, 0, 0, 0};
-  trace<RAYFIRE, MIXED, true, WIDE>(T, ray, a.orientation, ex, k, lds + threadIdx.x, blockDim.x, cap, h, &st);
+  if (WIDE && spec)
+    trace<RAYFIRE, MIXED, true, WIDE, true>(T, ray, a.orientation, ex, k, lds + threadIdx.x, blockDim.x, cap, h, &st);
+  else
+    trace<RAYFIRE, MIXED, true, WIDE>(T, ray, a.orientation, ex, k, lds + threadIdx.x, blockDim.x, cap, h, &st);
   atomicAdd(counters + 0, 1ULL);
   atomicAdd(counters + 1, (unsigned long long)st.V);
   atomicAdd(counters + 2, (unsigned long long)st.F);
@@ -260,6 +265,13 @@ __global__ void k_ray_fire_stats(RayFireArgs a, int cap, unsigned long long* cou
   if (MIXED && !mixed_safe(T, ray)) atomicAdd(counters + 7, 1ULL);
   if (st.WV) atomicAdd(counters + 8, (unsigned long long)st.WV);
   if (st.WF) atomicAdd(counters + 9, (unsigned long long)st.WF);
+  // longest ray of the wave (node visits, triangle tests): the iterations a
+  // wave needs if lanes never wait for each other between leaves
+  const unsigned int mv = wave_max_u24(st.V), mf = wave_max_u24(st.F);
+  if (first_active_lane()) {
+    atomicAdd(counters + 10, (unsigned long long)mv);
+    atomicAdd(counters + 11, (unsigned long long)mf);
+  }
 }
 
 template <bool MIXED, bool WIDE>
@@ -477,7 +489,10 @@ void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipSt
     const unsigned gd = (unsigned)std::min<int64_t>((a.n + c.block - 1) / c.block, 1024);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, a.fb_count);
     if (c.wide) {
-      hipLaunchKernelGGL((k_ray_fire<true, true, true>), g, b, sh, s, a, c.stack);
+      if (c.spec)
+        hipLaunchKernelGGL((k_ray_fire<true, true, true, true>), g, b, sh, s, a, c.stack);
+      else
+        hipLaunchKernelGGL((k_ray_fire<true, true, true>), g, b, sh, s, a, c.stack);
       hipLaunchKernelGGL((k_ray_fire_deferred<true>), dim3(gd), b, sh, s, a, c.stack);
     } else {
       hipLaunchKernelGGL((k_ray_fire<true, true, false>), g, b, sh, s, a, c.stack);
@@ -489,7 +504,9 @@ void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipSt
     else
       hipLaunchKernelGGL((k_ray_fire<true, false, false>), g, b, sh, s, a, c.stack);
   } else {
-    if (c.wide)
+    if (c.wide && c.spec)
+      hipLaunchKernelGGL((k_ray_fire<false, false, true, true>), g, b, sh, s, a, c.stack);
+    else if (c.wide)
       hipLaunchKernelGGL((k_ray_fire<false, false, true>), g, b, sh, s, a, c.stack);
     else
       hipLaunchKernelGGL((k_ray_fire<false, false, false>), g, b, sh, s, a, c.stack);
@@ -501,7 +518,7 @@ void launch_ray_fire_stats(const RayFireArgs& a, bool mixed, const LaunchCfg& c,
 {
   if (a.n <= 0) return;
   XDG_DISPATCH2(k_ray_fire_stats, mixed, c.wide, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack,
-                counters);
+                counters, c.spec);
 }
 
 void launch_point_in_volume(const PivArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)

@@ -114,6 +114,23 @@ struct TraceStats {
   unsigned int V, F, E, C, N, WV, WF;
 };
 
+// max of v (< 2^24) over the active lanes of the wave, in every lane
+// (bitwise from the top: only ballots, so inactive lanes never contribute)
+__device__ __forceinline__ unsigned int wave_max_u24(unsigned int v)
+{
+  const int lane = threadIdx.x & 63;
+  unsigned long long cand = __ballot(1);
+  unsigned int m = 0;
+  for (int b = 23; b >= 0; --b) {
+    const unsigned long long has = __ballot(((v >> b) & 1u) != 0u && ((cand >> lane) & 1ull) != 0ull);
+    if (has) {
+      m |= 1u << b;
+      cand &= has;
+    }
+  }
+  return m;
+}
+
 // 1 in the lowest active lane of the wave, else 0
 __device__ __forceinline__ unsigned int first_active_lane()
 {
@@ -335,10 +352,9 @@ __device__ __forceinline__ void node4_test(const TT& T, int32_t node, const BoxR
   }
 }
 
-template <int MODE, bool MIXED, bool STATS = false>
-__device__ __forceinline__ bool traverse4(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
-                                          int k, int32_t* stk, int stride, int depth_cap, HitState& h,
-                                          TraceStats* st = nullptr)
+template <int MODE, bool MIXED, bool STATS, bool SPEC, class STK>
+__device__ __forceinline__ bool traverse4_s(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
+                                            int k, STK& S, HitState& h, TraceStats* st = nullptr)
 {
   if (T.n_prims <= 0) return false;
   BoxRayD bd;
@@ -361,11 +377,14 @@ __device__ __forceinline__ bool traverse4(const SurfTreeDev& T, const RayD& ray,
   }
   float tf = MODE == OCCL ? INFINITY : f32_up(h.t);
   int32_t node = T.root4;
-  LdsStack S(stk, stride, depth_cap);
+  S.reset();
   // while-while: lanes descend internal nodes together until every lane of
   // the wave holds a leaf (or is done), then the leaves are tested together;
-  // node and leaf work never share a divergent iteration.
+  // node and leaf work never share a divergent iteration.  SPEC: a lane that
+  // reaches a leaf while others still descend parks it and keeps descending
+  // (with the distance bound it has), so fewer lanes idle in the node loop.
   const int32_t DONE = CHILD_EMPTY;
+  int32_t leaf = DONE;
   while (node != DONE) {
     while (node >= 0) {
       if (STATS) {
@@ -388,10 +407,22 @@ __device__ __forceinline__ bool traverse4(const SurfTreeDev& T, const RayD& ray,
       } else {
         node = S.empty() ? DONE : S.pop();
       }
+      if (SPEC) {
+        if (node < 0 && node != DONE && leaf == DONE) {
+          leaf = node;
+          node = S.empty() ? DONE : S.pop();
+        }
+        if (__all(leaf != DONE)) break;
+      }
     }
-    if (node == DONE) break;
-    const int32_t first = leaf_first(node);
-    const int32_t cnt = leaf_count(node);
+    if (leaf == DONE) {
+      if (node == DONE) break;
+      leaf = node;  // a leaf: node < 0 here
+      node = S.empty() ? DONE : S.pop();
+    }
+    const int32_t first = leaf_first(leaf);
+    const int32_t cnt = leaf_count(leaf);
+    leaf = DONE;
     for (int32_t j = 0; j < cnt; ++j) {
       if (STATS) st->WF += first_active_lane();
       if (MODE == OCCL) {
@@ -401,10 +432,60 @@ __device__ __forceinline__ bool traverse4(const SurfTreeDev& T, const RayD& ray,
       }
     }
     if (MODE != OCCL) tf = f32_up(h.t);
-    node = S.empty() ? DONE : S.pop();
   }
   return false;
 }
+
+template <int MODE, bool MIXED, bool STATS = false, bool SPEC = false>
+__device__ __forceinline__ bool traverse4(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
+                                          int k, int32_t* stk, int stride, int depth_cap, HitState& h,
+                                          TraceStats* st = nullptr)
+{
+  LdsStack S(stk, stride, depth_cap);
+  return traverse4_s<MODE, MIXED, STATS, SPEC>(T, ray, orientation, ex, k, S, h, st);
+}
+
+// Stack with its first entries in LDS and the rest in a per-lane global
+// region (entry i >= lds_cap of lane l at glb[(i - lds_cap) * gstride + l]).
+// The LDS share bounds occupancy; the global share keeps deep trees exact.
+struct HybridStack {
+  int32_t* lt;      // next free LDS slot
+  int32_t* lbase;
+  int32_t* gt;      // next free global slot
+  int32_t* gbase;
+  int lstride;
+  int64_t gstride;
+  int n, lcap, cap;
+  __device__ __forceinline__ void reset()
+  {
+    lt = lbase;
+    gt = gbase;
+    n = 0;
+  }
+  __device__ __forceinline__ bool empty() const { return n == 0; }
+  __device__ __forceinline__ void push(int32_t v)
+  {
+    if (n >= cap) return;
+    if (n < lcap) {
+      *lt = v;
+      lt += lstride;
+    } else {
+      *gt = v;
+      gt += gstride;
+    }
+    ++n;
+  }
+  __device__ __forceinline__ int32_t pop()
+  {
+    --n;
+    if (n < lcap) {
+      lt -= lstride;
+      return *lt;
+    }
+    gt -= gstride;
+    return *gt;
+  }
+};
 
 __device__ __forceinline__ RayD make_ray(const double* o, const double* d)
 {
@@ -426,15 +507,15 @@ __device__ __forceinline__ bool mixed_safe(const SurfTreeDev& T, const RayD& r)
 
 // MIXED traces with FP32 boxes when the origin is within the tree's limit
 // and falls back to FP64 boxes otherwise; WIDE selects the BVH4.
-template <int MODE, bool MIXED, bool STATS = false, bool WIDE = false>
+template <int MODE, bool MIXED, bool STATS = false, bool WIDE = false, bool SPEC = false>
 __device__ __forceinline__ bool trace(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
                                       int k, int32_t* stk, int stride, int cap, HitState& h,
                                       TraceStats* st = nullptr)
 {
   if (WIDE) {
     if (MIXED && mixed_safe(T, ray))
-      return traverse4<MODE, true, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
-    return traverse4<MODE, false, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
+      return traverse4<MODE, true, STATS, SPEC>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
+    return traverse4<MODE, false, STATS, SPEC>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
   }
   if (MIXED) {
     if (mixed_safe(T, ray)) return traverse<MODE, true, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
@@ -443,12 +524,12 @@ __device__ __forceinline__ bool trace(const SurfTreeDev& T, const RayD& ray, int
 }
 
 // Mixed-only traversal for rays already known to be within the FP32 limit.
-template <int MODE, bool STATS = false, bool WIDE = false>
+template <int MODE, bool STATS = false, bool WIDE = false, bool SPEC = false>
 __device__ __forceinline__ bool trace_f32(const SurfTreeDev& T, const RayD& ray, int orientation, const int32_t* ex,
                                           int k, int32_t* stk, int stride, int cap, HitState& h,
                                           TraceStats* st = nullptr)
 {
-  if (WIDE) return traverse4<MODE, true, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
+  if (WIDE) return traverse4<MODE, true, STATS, SPEC>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
   return traverse<MODE, true, STATS>(T, ray, orientation, ex, k, stk, stride, cap, h, st);
 }
 

@@ -306,6 +306,97 @@ __device__ __forceinline__ bool tet_contains(const D3& p, const D3& v0, const D3
 // BVH2 node with both children's boxes inline; one node fetch yields both
 // child boxes.  Child encoding: c >= 0 internal node index; c < 0 leaf,
 // ~c = (first << 3) | (count - 1) over the leaf-ordered primitive array.
+// Per-item sequence hash (element walks, particle histories): h starts at
+// HASH_SEED; each entry (id, length) does h = mix64(h ^ (uint32)id), then
+// h = mix64(h ^ bits(length)); mix64 is the splitmix64 finaliser.
+constexpr unsigned long long HASH_SEED = 0x6a09e667f3bcc909ULL;
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x)
+{
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ULL;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebULL;
+  x ^= x >> 31;
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// Random numbers and the natural log used by the transport sampler.
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11): counter (item_lo,
+// item_hi, draw, 0), key (seed, stream); one block gives two 53-bit
+// uniforms (w >> 11) * 2^-53 from the words (x0<<32|x1), (x2<<32|x3).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void uniform2(uint32_t seed, uint32_t stream, uint64_t item, uint32_t draw, double& ua,
+                                         double& ub)
+{
+  uint32_t c0 = (uint32_t)item, c1 = (uint32_t)(item >> 32), c2 = draw, c3 = 0u;
+  uint32_t k0 = seed, k1 = stream;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    const uint32_t n3 = (uint32_t)p0;
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+    c3 = n3;
+  }
+  const uint64_t a = ((uint64_t)c0 << 32) | c1;
+  const uint64_t b = ((uint64_t)c2 << 32) | c3;
+  ua = (double)(a >> 11) * 0x1.0p-53;
+  ub = (double)(b >> 11) * 0x1.0p-53;
+}
+
+// ln(x) for positive x: x = m 2^e with m in (sqrt(1/2), sqrt(2)];
+// ln(m) = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716, atanh series to
+// s^23 (Horner, plain IEEE ops); ln 2 split into a head with 32 significant
+// bits (e * head is exact) and a tail.  Written out so the host sampler
+// produces the same bits.
+__device__ __forceinline__ double log_series(double x)
+{
+  if (!(x > 0.0)) return x == 0.0 ? -__builtin_huge_val() : __builtin_nan("");
+  if (x == __builtin_huge_val()) return x;
+  uint64_t b = (uint64_t)__double_as_longlong(x);
+  int e = (int)((b >> 52) & 0x7ff);
+  if (e == 0) {
+    x = x * 0x1.0p54;
+    b = (uint64_t)__double_as_longlong(x);
+    e = (int)((b >> 52) & 0x7ff) - 54;
+  }
+  e -= 1023;
+  double m = __longlong_as_double((long long)((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
+  if (m > 0x1.6a09e667f3bcdp+0) {
+    m = m * 0.5;
+    e += 1;
+  }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double p = 1.0 / 23.0;
+  p = 1.0 / 21.0 + z * p;
+  p = 1.0 / 19.0 + z * p;
+  p = 1.0 / 17.0 + z * p;
+  p = 1.0 / 15.0 + z * p;
+  p = 1.0 / 13.0 + z * p;
+  p = 1.0 / 11.0 + z * p;
+  p = 1.0 / 9.0 + z * p;
+  p = 1.0 / 7.0 + z * p;
+  p = 1.0 / 5.0 + z * p;
+  p = 1.0 / 3.0 + z * p;
+  const double two_s = s + s;
+  const double lnm = two_s + two_s * (z * p);
+  const double de = (double)e;
+  return de * 0x1.62e42fefa0000p-1 + (de * 0x1.cf79abc9e3b3ap-40 + lnm);
+}
+
 struct alignas(16) NodeD {  // plain FP64 layout, 112 B
   double b0[6];             // child 0: lo.x lo.y lo.z hi.x hi.y hi.z
   double b1[6];             // child 1
@@ -373,6 +464,9 @@ struct SurfTreeDev {
   // face_normal of each triangle (leaf order, 3 doubles), computed on the
   // device at build time with the same expression as the reference
   const double* normals;
+  // bounds of the tree's vertices (ray ordering only; not used by queries)
+  float box_lo[3];
+  float box_hi[3];
 };
 
 struct ElemTreeDev {
