@@ -137,6 +137,11 @@ int xdg_hip_attach_boundary(xdg_hip_ctx* ctx, int32_t element_tree, int32_t surf
 
 int xdg_hip_get_tree_info(xdg_hip_ctx* ctx, int32_t tree, int element, xdg_hip_tree_info* info);
 int xdg_hip_num_trees(xdg_hip_ctx* ctx, int32_t* surface_trees, int32_t* element_trees);
+/* Frees a surface tree's device memory; its id is not reused and later
+ * queries on it return XDG_HIP_ERR_NOT_FOUND.  (The reference keeps trees
+ * for the RayTracer's lifetime; this serves callers that rebuild geometry,
+ * such as the differentiable-surface harness.) */
+int xdg_hip_release_surface_tree(xdg_hip_ctx* ctx, int32_t tree);
 
 /* ---- queries (device arrays) -------------------------------------------- */
 
@@ -289,6 +294,18 @@ int xdg_hip_graph_destroy(xdg_hip_ctx* ctx, void* graph);
  * sample) and keeps the fastest configuration (synchronous). */
 int xdg_hip_transport_autotune(xdg_hip_ctx* ctx, int32_t handle, uint64_t first, int64_t sample, int precision,
                                double* best_ms);
+
+/* ---- adjoint of the ray distance ------------------------------------------
+ * No counterpart in the reference.  For the rays of a ray_fire batch (t,
+ * face as returned; face ids must index the rows of tris), accumulates
+ * dldv[3 nv] += sum_i dldt[i] dt_i/dverts (FP64 atomics) and writes
+ * dldo[3 n] = dldt dt/do, dldd[3 n] = dldt dt/dd (zero for misses), with
+ * t = n.(v0 - o)/(n.d), n = (v1 - v0) x (v2 - v0).  All arrays device,
+ * dldv/dldo/dldd nullable.  Asynchronous on stream. */
+int xdg_hip_ray_distance_adjoint(xdg_hip_ctx* ctx, int64_t n, const double* org, const double* dir,
+                                 const double* t, const int32_t* face, const double* dldt, int64_t nv,
+                                 const double* verts, int64_t ntri, const int32_t* tris, double* dldv,
+                                 double* dldo, double* dldd, void* stream);
 
 /* ---- device memory helpers for callers without their own allocator ----- */
 int xdg_hip_malloc(xdg_hip_ctx* ctx, size_t bytes, void** ptr);

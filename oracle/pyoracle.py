@@ -79,6 +79,8 @@ def lib():
                                   C.c_uint32, _d, _d]
         L.xo_log.restype = C.c_double
         L.xo_log.argtypes = [C.c_double]
+        L.xo_adjoint.argtypes = [C.c_int64, _d, C.c_int64, _i32, C.c_int64,
+                                 _d, _d, _d, _i32, _d, _d, _d, _d]
         L.xo_transport.argtypes = [C.POINTER(TransportProblem), C.c_uint64,
                                    C.c_int64, C.c_int, C.POINTER(C.c_int64),
                                    _i32, _u8, C.POINTER(C.c_uint64)]
@@ -380,3 +382,20 @@ def transport(model, trees, vol_ids, seed, stream, lo, hi, mfp, p_absorb,
                        hashes.ctypes.data_as(C.POINTER(C.c_uint64)))
     return {"tally_fx": tally, "events": events, "status": status,
             "hash": hashes}
+
+
+# ---- adjoint of the ray distance -------------------------------------------
+
+def adjoint(verts, tris, org, dirs, t, face, dldt):
+    """xo_adjoint: returns (dL/dverts (nv, 3), dL/dorg (n, 3), dL/ddir)."""
+    verts, org, dirs, t, dldt = (_f64(verts), _f64(org), _f64(dirs), _f64(t),
+                                 _f64(dldt))
+    tris, face = _i32a(tris), _i32a(face)
+    nv, n = verts.shape[0], org.shape[0]
+    gv = np.zeros((nv, 3))
+    go = np.zeros((n, 3))
+    gd = np.zeros((n, 3))
+    lib().xo_adjoint(nv, _p(verts, _d), tris.shape[0], _p(tris, _i32), n,
+                     _p(org, _d), _p(dirs, _d), _p(t, _d), _p(face, _i32),
+                     _p(dldt, _d), _p(gv, _d), _p(go, _d), _p(gd, _d))
+    return gv, go, gd

@@ -1450,3 +1450,59 @@ void xo_transport(const xo_transport_problem* p, uint64_t first, int64_t n,
     if (hash) hash[i] = hh;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* adjoint of the ray distance                                               */
+/* ------------------------------------------------------------------------ */
+
+void xo_adjoint(int64_t nv, const double* verts, int64_t ntri,
+                const int32_t* tris, int64_t n, const double* org,
+                const double* dir, const double* t, const int32_t* face,
+                const double* dldt, double* dldv, double* dldo, double* dldd)
+{
+  (void)nv;
+  for (int64_t i = 0; i < n; ++i) {
+    double* go = dldo ? dldo + 3 * i : NULL;
+    double* gd = dldd ? dldd + 3 * i : NULL;
+    const int32_t f = face[i];
+    const double ti = t[i];
+    int ok = f >= 0 && f < ntri && ti < INFTY;
+    double e1[3], e2[3], nn[3], q[3], a1[3], a2[3], den = 0.0;
+    const double *v0 = NULL, *v1 = NULL, *v2 = NULL;
+    if (ok) {
+      v0 = verts + 3 * (int64_t)tris[3 * (int64_t)f];
+      v1 = verts + 3 * (int64_t)tris[3 * (int64_t)f + 1];
+      v2 = verts + 3 * (int64_t)tris[3 * (int64_t)f + 2];
+      v_sub(v1, v0, e1);
+      v_sub(v2, v0, e2);
+      v_cross(e1, e2, nn);
+      den = v_dot(nn, dir + 3 * i);
+      ok = den != 0.0;
+    }
+    if (!ok) {
+      if (go) go[0] = go[1] = go[2] = 0.0;
+      if (gd) gd[0] = gd[1] = gd[2] = 0.0;
+      continue;
+    }
+    const double w = dldt[i] / den;
+    const double* o = org + 3 * i;
+    const double* d = dir + 3 * i;
+    /* q = v0 - (o + t d), in the triangle's plane */
+    for (int k = 0; k < 3; ++k) q[k] = v0[k] - (o[k] + ti * d[k]);
+    v_cross(e2, q, a1); /* dt/dv1 * den */
+    v_cross(q, e1, a2); /* dt/dv2 * den */
+    if (dldv) {
+      const int64_t i0 = tris[3 * (int64_t)f], i1 = tris[3 * (int64_t)f + 1],
+                    i2 = tris[3 * (int64_t)f + 2];
+      for (int k = 0; k < 3; ++k) {
+        dldv[3 * i1 + k] += w * a1[k];
+        dldv[3 * i2 + k] += w * a2[k];
+        dldv[3 * i0 + k] += w * (nn[k] - a1[k] - a2[k]);
+      }
+    }
+    for (int k = 0; k < 3; ++k) {
+      if (go) go[k] = -w * nn[k];
+      if (gd) gd[k] = -w * ti * nn[k];
+    }
+  }
+}

@@ -41,7 +41,8 @@
  *                                 closed forms (mean track mfp/p, mean
  *                                 collisions 1/p), region-volume tally
  *                                 fractions and thread-count independence
- *   xo_adjoint                    central finite differences
+ *   xo_adjoint                    central finite differences of the
+ *                                 oracle's own ray_fire distances
  */
 #ifndef XDG_ORACLE_H
 #define XDG_ORACLE_H
@@ -249,6 +250,21 @@ typedef struct {
 void xo_transport(const xo_transport_problem* p, uint64_t first, int64_t n,
                   int nthreads, int64_t* tally_fx, int32_t* events,
                   uint8_t* status, uint64_t* hash);
+
+/* ---- adjoint of the ray distance ----------------------------------------- */
+
+/* For rays that hit triangle f = face[i] of the mesh (verts, tris) at
+ * distance t[i], t = n.(v0 - o) / (n.d) with n = (v1 - v0) x (v2 - v0), so
+ * with D = n.d and q = v0 - (o + t d):
+ *   dt/do = -n / D,  dt/dd = -t n / D,
+ *   dt/dv1 = (e2 x q) / D,  dt/dv2 = (q x e1) / D,
+ *   dt/dv0 = (n - e2 x q - q x e1) / D      (e1 = v1 - v0, e2 = v2 - v0).
+ * dldv (3 nv) accumulates sum_i dldt[i] dt_i/dv serially; dldo, dldd (3 n)
+ * are written (zero for misses).  No reference counterpart. */
+void xo_adjoint(int64_t nv, const double* verts, int64_t ntri,
+                const int32_t* tris, int64_t n, const double* org,
+                const double* dir, const double* t, const int32_t* face,
+                const double* dldt, double* dldv, double* dldo, double* dldd);
 
 #ifdef __cplusplus
 }

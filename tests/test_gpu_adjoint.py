@@ -1,0 +1,126 @@
+"""Ray-distance adjoint on the device: against the oracle's closed form, and
+through the torch.autograd.Function against central finite differences
+(h = 1e-6, relative 1e-6) of the device's own distances."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpu_util import bits, dev, host
+from xdg_amd import geometry as G
+from xdg_amd import philox as P
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-6
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(torch_mod):
+    from xdg_amd import capi
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def problem():
+    m = G.perturbed_sphere(3, 10.0, 0.05)
+    n = 100000
+    org = P.ball_points(5, 0, n, 9.5)
+    dirs = P.iso_dirs(5, 1, n)
+    w = P.normals(5, 2, n)
+    return m, org, dirs, w
+
+
+def test_adjoint_matches_oracle(ctx, torch_mod, problem):
+    torch = torch_mod
+    from xdg_amd import adjoint as A
+    m, org, dirs, w = problem
+    surf = A.DifferentiableSurface(ctx, m.tris)
+    verts = dev(torch, m.verts)
+    tree = surf.tree_for(verts)
+    o, d = dev(torch, org), dev(torch, dirs)
+    t, s, f = ctx.ray_fire(tree, o, d, precision=1)
+    rt, rs, rf = O.Tree.from_model(m, 1).ray_fire(org, dirs)
+    assert np.array_equal(host(f), rf) and np.array_equal(bits(host(t)), bits(rt))
+    gv = torch.zeros_like(verts)
+    go = torch.empty_like(o)
+    gd = torch.empty_like(d)
+    ctx.ray_distance_adjoint(o, d, t, f, dev(torch, w), verts, surf.tris, gv, go, gd)
+    ov, oo, od = O.adjoint(m.verts, m.tris, org, dirs, rt, rf, w)
+    assert np.array_equal(bits(host(go)), bits(oo))
+    assert np.array_equal(bits(host(gd)), bits(od))
+    g = host(gv)
+    assert np.allclose(g, ov, rtol=1e-12, atol=1e-12 * np.abs(ov).max())
+
+
+def test_autograd_matches_central_differences(ctx, torch_mod, problem):
+    torch = torch_mod
+    from xdg_amd import adjoint as A
+    m, org, dirs, w = problem
+    surf = A.DifferentiableSurface(ctx, m.tris)
+    verts = dev(torch, m.verts).requires_grad_(True)
+    o = dev(torch, org).requires_grad_(True)
+    d = dev(torch, dirs).requires_grad_(True)
+    wt = dev(torch, w)
+    t = A.ray_distance(o, d, verts, surf)
+    (wt * t).sum().backward()
+    gv = host(verts.grad)
+    rng = np.random.default_rng(3)
+
+    def fire(v):
+        vt = dev(torch, v)
+        tr = surf.tree_for(vt)
+        tt, _, _ = ctx.ray_fire(tr, o.detach(), d.detach(), precision=1)
+        return host(tt)
+
+    checked = 0
+    for vi in rng.choice(m.verts.shape[0], 16, replace=False):
+        for k in range(3):
+            vp = m.verts.copy()
+            vm = m.verts.copy()
+            vp[vi, k] += H
+            vm[vi, k] -= H
+            fd = (w * (fire(vp) - fire(vm))).sum() / (2 * H)
+            assert abs(fd - gv[vi, k]) <= 1e-6 * max(1.0, abs(gv[vi, k])), (vi, k, fd, gv[vi, k])
+            checked += 1
+    assert checked == 48
+    # ray gradients: per-ray central differences on a subset
+    sel = slice(0, 2000)
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = H
+        tree = surf.tree_for(dev(torch, m.verts))
+        for base, grad in ((org, o.grad), (dirs, d.grad)):
+            plus = base.copy()
+            minus = base.copy()
+            plus[sel] += e
+            minus[sel] -= e
+            if base is org:
+                tp, _, _ = ctx.ray_fire(tree, dev(torch, plus), d.detach(), precision=1)
+                tm, _, _ = ctx.ray_fire(tree, dev(torch, minus), d.detach(), precision=1)
+            else:
+                tp, _, _ = ctx.ray_fire(tree, o.detach(), dev(torch, plus), precision=1)
+                tm, _, _ = ctx.ray_fire(tree, o.detach(), dev(torch, minus), precision=1)
+            fd = w[sel] * (host(tp)[sel] - host(tm)[sel]) / (2 * H)
+            g = host(grad)[sel, k]
+            assert np.allclose(fd, g, rtol=1e-6, atol=1e-6)
+
+
+def test_released_tree_is_gone(ctx, problem):
+    from xdg_amd import capi
+    m, _, _, _ = problem
+    v9, f, s, fl = m.tree_arrays(1)
+    tree = ctx.build_surface_tree(v9.reshape(-1, 3), np.arange(v9.shape[0] * 3, dtype=np.int32).reshape(-1, 3),
+                                  f, s, fl, m.bump(1))
+    ctx.release_surface_tree(tree)
+    with pytest.raises(capi.XdgHipError):
+        ctx.tree_info(tree)

@@ -93,6 +93,7 @@ def lib():
             "xdg_hip_attach_boundary": [_vp, _i32, _i32, _i64, _vp],
             "xdg_hip_get_tree_info": [_vp, _i32, C.c_int, C.POINTER(TreeInfo)],
             "xdg_hip_num_trees": [_vp, C.POINTER(_i32), C.POINTER(_i32)],
+            "xdg_hip_release_surface_tree": [_vp, _i32],
             "xdg_hip_ray_fire": [_vp, _i32, _vp, _i64, _vp, _vp, _vp, C.c_int,
                                  _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp],
             "xdg_hip_ray_fire_stats": [_vp, _i32, _vp, _i64, _vp, _vp, _vp,
@@ -129,6 +130,9 @@ def lib():
             "xdg_hip_graph_destroy": [_vp, _vp],
             "xdg_hip_transport_autotune": [_vp, _i32, C.c_uint64, _i64,
                                            C.c_int, C.POINTER(C.c_double)],
+            "xdg_hip_ray_distance_adjoint": [_vp, _i64, _vp, _vp, _vp, _vp,
+                                             _vp, _i64, _vp, _i64, _vp, _vp,
+                                             _vp, _vp, _vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -253,6 +257,9 @@ class Context:
             self.h, v.size // 3, vp, t.size // 3, tp, fp, sp, flp, float(bump),
             C.byref(out)))
         return out.value
+
+    def release_surface_tree(self, tree):
+        self._check(lib().xdg_hip_release_surface_tree(self.h, tree))
 
     def build_element_tree(self, verts, tets, adj, face_order):
         v, vp = _host(verts, np.float64)
@@ -474,6 +481,18 @@ class Context:
         self._check(lib().xdg_hip_transport_autotune(
             self.h, handle, int(first), int(sample), precision, C.byref(ms)))
         return ms.value
+
+    # ---- adjoint of the ray distance --------------------------------------
+    @_ordered
+    def ray_distance_adjoint(self, org, dirs, t, face, dldt, verts, tris,
+                             dldv=None, dldo=None, dldd=None, stream=None):
+        """dL/d(verts) accumulated into dldv; dL/d(org), dL/d(dir) written
+        into dldo / dldd (all device tensors; None skips)."""
+        n = org.shape[0]
+        self._done(lib().xdg_hip_ray_distance_adjoint(
+            self.h, n, _ptr(org), _ptr(dirs), _ptr(t), _ptr(face), _ptr(dldt),
+            verts.shape[0], _ptr(verts), tris.shape[0], _ptr(tris),
+            _ptr(dldv), _ptr(dldo), _ptr(dldd), stream), stream)
 
     def status(self):
         f = C.c_uint32()

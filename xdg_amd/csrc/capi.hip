@@ -22,6 +22,7 @@ namespace {
 struct SurfTreeHost {
   SurfTreeDev dev{};
   std::vector<DevBuf> bufs;
+  bool released = false;
   int32_t min_face = 0, max_face = -1;
   int64_t n_nodes = 0;
   int64_t n_nodes4 = 0;
@@ -138,7 +139,7 @@ void require(bool ok, const char* msg)
 
 const SurfTreeHost& surf_tree(xdg_hip_ctx* c, int32_t tree)
 {
-  if (tree < 1 || (size_t)tree > c->surf.size())
+  if (tree < 1 || (size_t)tree > c->surf.size() || c->surf[tree - 1].released)
     throw NotFound{"surface tree " + std::to_string(tree) + " is not registered"};
   return c->surf[tree - 1];
 }
@@ -610,6 +611,20 @@ int xdg_hip_get_tree_info(xdg_hip_ctx* c, int32_t tree, int element, xdg_hip_tre
   });
 }
 
+int xdg_hip_release_surface_tree(xdg_hip_ctx* c, int32_t tree)
+{
+  return guarded(c, [&]() {
+    surf_tree(c, tree);
+    SurfTreeHost& t = c->surf[tree - 1];
+    XDG_HIP_CHECK(hipDeviceSynchronize());
+    dev_free_all(t.bufs);
+    t.dev = SurfTreeDev{};
+    t.released = true;
+    sync_tables(c);
+    return XDG_HIP_OK;
+  });
+}
+
 int xdg_hip_num_trees(xdg_hip_ctx* c, int32_t* surface_trees, int32_t* element_trees)
 {
   return guarded(c, [&]() {
@@ -888,6 +903,26 @@ int xdg_hip_eval_primitive(xdg_hip_ctx* c, int kind, int64_t n, const double* in
     if (n == 0) return XDG_HIP_OK;
     require(in && out, "null array");
     launch_eval(kind, n, in, out, pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_ray_distance_adjoint(xdg_hip_ctx* c, int64_t n, const double* org, const double* dir, const double* t,
+                                 const int32_t* face, const double* dldt, int64_t nv, const double* verts,
+                                 int64_t ntri, const int32_t* tris, double* dldv, double* dldo, double* dldd,
+                                 void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0 && nv >= 0 && ntri >= 0, "negative size");
+    if (n == 0) return XDG_HIP_OK;
+    require(org && dir && t && face && dldt, "null ray array");
+    require(!dldv || (verts && tris), "dldv needs verts and tris");
+    require(verts || !dldv, "null verts");
+    require(ntri == 0 || tris, "null tris");
+    AdjointArgs a{n, org, dir, t, face, dldt, nv, verts, ntri, tris, dldv, dldo, dldd};
+    if (!verts) a.ntri = 0;  // every ray reads as a miss without the mesh
+    launch_adjoint(a, 256, pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
   });

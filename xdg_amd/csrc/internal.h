@@ -224,6 +224,24 @@ void launch_transport(const TransportDev& P, const TransportRun& R, bool mixed, 
                       hipStream_t s);
 constexpr int TRANSPORT_LDS_TALLY_MAX = 4096;
 
+// ---- adjoint of the ray distance (adjoint.hip) ----------------------------
+struct AdjointArgs {
+  int64_t n;
+  const double* org;
+  const double* dir;
+  const double* t;
+  const int32_t* face;
+  const double* dldt;
+  int64_t nv;
+  const double* verts;
+  int64_t ntri;
+  const int32_t* tris;
+  double* dldv;  // nullable, accumulated
+  double* dldo;  // nullable, written
+  double* dldd;  // nullable, written
+};
+void launch_adjoint(const AdjointArgs& a, int block, hipStream_t s);
+
 // ---- element queries (element_query.hip) ---------------------------------
 struct FindElementArgs {
   const xdgd::ElemTreeDev* etable;

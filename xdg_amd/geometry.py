@@ -35,6 +35,23 @@ NUMERICAL_PRECISION = 1e-3  # ray_tracing_interface.h:146
 # surface models
 # ---------------------------------------------------------------------------
 
+def box_bump(lo, hi) -> float:
+    """max(BoundingBox::dilation(), numerical_precision): the chord of the
+    box times FP_BOX_TOL, at least 1e-3 (ray_tracing_interface.cpp:20-24)."""
+    w = np.asarray(hi, dtype=np.float64) - np.asarray(lo, dtype=np.float64)
+    chord = math.sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2])
+    return max(chord * FP_BOX_TOL, NUMERICAL_PRECISION)
+
+
+def vertex_bump(verts) -> float:
+    """Bump of a volume whose surfaces use exactly these vertices (the box
+    starts all-zero, as MeshManager::volume_bounding_box does)."""
+    v = np.asarray(verts, dtype=np.float64).reshape(-1, 3)
+    if v.size == 0:
+        return box_bump(np.zeros(3), np.zeros(3))
+    return box_bump(np.minimum(0.0, v.min(axis=0)), np.maximum(0.0, v.max(axis=0)))
+
+
 @dataclass
 class SurfaceModel:
     verts: np.ndarray                 # (nv, 3) float64
@@ -89,9 +106,7 @@ class SurfaceModel:
     def bump(self, vol: int) -> float:
         """RayTracer::bounding_box_bump (ray_tracing_interface.cpp:20-24)."""
         lo, hi = self.volume_bbox(vol)
-        w = hi - lo
-        chord = math.sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2])
-        return max(chord * FP_BOX_TOL, NUMERICAL_PRECISION)
+        return box_bump(lo, hi)
 
     def next_volume(self, vol: int, surf: int) -> int:
         """MeshManager::next_volume (mesh_manager_interface.cpp:202-214)."""
