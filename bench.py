@@ -1,17 +1,20 @@
 """Benchmark of the geometry-query hot path on one MI355X.
 
-Workload (the metric's configuration): three nested icospheres r = 5/10/15
-(subdivision 6, 81 920 triangles each) inside the cube [-20, 20]^3; 10 M
-query points uniform in [-16, 16]^3 with isotropic directions (Philox seed 2).
-Every point is located with find_volume, then one step = ray_fire of all
-points against the surface tree of the volume that contains them
-(EXITING, distance and surface/face ids), on device-resident inputs.
+Headline workload (the metric's configuration): three nested icospheres
+r = 5/10/15 (subdivision 6, 81 920 triangles each) inside the cube
+[-20, 20]^3; 10 M query points uniform in [-16, 16]^3 with isotropic
+directions (Philox seed 2).  Every point is located with find_volume, then
+one step = ray_fire of all points against the surface tree of the volume
+that contains them (EXITING, distance and surface/face ids), with inputs
+resident in device memory.
 
-Reported: the mixed FP32-box / FP64-Pluecker kernel's Mrays/s (value), the
-plain FP64 kernel's Mrays/s, whether the two produce identical hits,
-find_volume Mqueries/s, the tet-walk Msegs/s (64^3 x 6 jittered Kuhn tets,
-4 M segments), a roofline object for the dominant kernel and the CPU oracle
-baseline timed on this host.
+Reported on the same JSON line: the mixed FP32-box / FP64-Pluecker kernel's
+Mrays/s (value) and the plain FP64 kernel's, whether their hits are
+identical, find_volume Mqueries/s, the tet walk (64^3 x 6 jittered Kuhn
+tets, 4 M segments, Msegs/s), the particle transport (8^3 spheres of 5 120
+triangles in [0, 16]^3, 2^20 histories, one captured graph launch), the
+ray-distance adjoint (perturbed icosphere, 81 920 triangles, 4 M rays),
+a roofline object for the dominant kernel and the CPU baseline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -45,9 +48,13 @@ def parse():
     p.add_argument("--points", type=int, default=10_000_000)
     p.add_argument("--segments", type=int, default=4_000_000)
     p.add_argument("--tet-n", type=int, default=64)
-    p.add_argument("--cpu-sample", type=int, default=2_000_000)
+    p.add_argument("--histories", type=int, default=1 << 20)
+    p.add_argument("--adjoint-rays", type=int, default=4_000_000)
+    p.add_argument("--cpu-sample", type=int, default=10_000_000)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-tet", action="store_true")
+    p.add_argument("--no-transport", action="store_true")
+    p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--copy-probe", action="store_true",
                    help="also run a 256 MiB device copy (scale for PMC byte counters)")
     return p.parse_args()
@@ -70,6 +77,313 @@ def time_launches(torch, stream, fn, steps, warmup):
     return per, wall
 
 
+def build_trees(ctx, model, vols):
+    trees = []
+    for v in vols:
+        v9, f, s, fl = model.tree_arrays(v)
+        nt = v9.shape[0]
+        trees.append(ctx.build_surface_tree(
+            v9.reshape(-1, 3), np.arange(3 * nt, dtype=np.int32).reshape(-1, 3),
+            f, s, fl, model.bump(v)))
+    return trees
+
+
+# ---------------------------------------------------------------------------
+# headline: find_volume + ray_fire on the nested spheres
+# ---------------------------------------------------------------------------
+
+def bench_ray_fire(args, ctx, torch, stream, sh):
+    from xdg_amd import capi
+    from xdg_amd import geometry as G
+    from xdg_amd import philox as P
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    model = G.nested_spheres(6)
+    vols = list(model.volumes)
+    trees = build_trees(ctx, model, vols)
+    log(f"built {len(trees)} surface trees in {time.perf_counter() - t0:.2f}s: "
+        + ", ".join(f"v{v}:{ctx.tree_info(t)['n_prims']}" for v, t in zip(vols, trees)))
+
+    n = args.points
+    t0 = time.perf_counter()
+    pts = P.box_points(2, P.STREAM_POS, n, [-16.0] * 3, [16.0] * 3)
+    dirs = P.iso_dirs(2, P.STREAM_DIR, n)
+    d_pts = torch.from_numpy(pts).to(dev)
+    d_dirs = torch.from_numpy(dirs).to(dev)
+    log(f"generated {n} queries in {time.perf_counter() - t0:.2f}s")
+
+    # find_volume (mixed and plain must agree)
+    d_vol = torch.empty(n, dtype=torch.int32, device=dev)
+    fv = lambda prec: ctx.find_volume(trees, vols, model.ipc, d_pts, d_dirs,  # noqa: E731
+                                      precision=prec, out=d_vol, stream=sh)
+    fv_per, _ = time_launches(torch, stream, lambda: fv(capi.MIXED), 3, 1)
+    vol_mixed = d_vol.clone()
+    fv(capi.PLAIN)
+    torch.cuda.synchronize()
+    vol_identical = bool(torch.equal(vol_mixed, d_vol))
+    fv_mqs = n / (sum(fv_per) / len(fv_per)) / 1e6
+    lut = torch.zeros(max(vols) + 1, dtype=torch.int32, device=dev)
+    for v, t in zip(vols, trees):
+        lut[v] = t
+    d_tpr = lut[d_vol.long()].contiguous()
+    counts = torch.bincount(d_vol.long(), minlength=max(vols) + 1).cpu().numpy()
+    log(f"find_volume {fv_mqs:.1f} Mqueries/s; volumes {counts.tolist()}; "
+        f"mixed==plain {vol_identical}")
+
+    outs = {}
+    for prec in (capi.MIXED, capi.PLAIN):
+        outs[prec] = (torch.empty(n, dtype=torch.float64, device=dev),
+                      torch.empty(n, dtype=torch.int32, device=dev),
+                      torch.empty(n, dtype=torch.int32, device=dev))
+
+    def fire(prec):
+        ctx.ray_fire(0, d_pts, d_dirs, tree_per_ray=d_tpr, precision=prec,
+                     out=outs[prec], stream=sh)
+
+    per_m, _ = time_launches(torch, stream, lambda: fire(capi.MIXED), args.steps, args.warmup)
+    per_p, _ = time_launches(torch, stream, lambda: fire(capi.PLAIN), args.steps, args.warmup)
+    avg_m = sum(per_m) / len(per_m)
+    avg_p = sum(per_p) / len(per_p)
+    mixed_mrays = n / avg_m / 1e6
+    plain_mrays = n / avg_p / 1e6
+    tm, sm, fm = outs[capi.MIXED]
+    tp, sp_, fp = outs[capi.PLAIN]
+    identical = bool(torch.equal(sm, sp_) and torch.equal(fm, fp)
+                     and torch.equal(tm.view(torch.int64), tp.view(torch.int64)))
+    hit = sp_ >= 0
+    rel = ((tm[hit] - tp[hit]).abs() / tp[hit].abs()).max().item() if hit.any() else 0.0
+    log(f"ray_fire mixed {mixed_mrays:.1f} Mrays/s ({avg_m * 1e3:.3f} ms), plain "
+        f"{plain_mrays:.1f} Mrays/s ({avg_p * 1e3:.3f} ms), ratio "
+        f"{mixed_mrays / plain_mrays:.3f}, identical {identical}, max rel dt {rel:.3e}")
+
+    # Algorithmic work per ray, counted on a 2^16 subsample with the plain
+    # FP64 binary layout (2-child node 128 B, triangle record 80 B):
+    #   bytes = 48 ray in + 16 hit out + 128 V + 80 F
+    #   fp64 flops = 40 V + 9 F + 23 E + 26 C + 29 N
+    ns = min(n, 1 << 16)
+    sub = (d_pts[:ns].contiguous(), d_dirs[:ns].contiguous(), d_tpr[:ns].contiguous())
+    ctx.set_traversal(wide=False)
+    try:
+        sb = ctx.ray_fire_stats(0, sub[0], sub[1], tree_per_ray=sub[2], precision=capi.PLAIN)
+    finally:
+        ctx.set_traversal()
+    r_ = sb["rays"]
+    V, F, E = sb["nodes"] / r_, sb["tri_tests"] / r_, sb["edge_tests"] / r_
+    Cf, N = sb["finishes"] / r_, sb["normals"] / r_
+    bytes_ray = 48 + 16 + 128 * V + 80 * F
+    flops_ray = 40 * V + 9 * F + 23 * E + 26 * Cf + 29 * N
+    achieved = bytes_ray * n / avg_m / 1e9
+    fp64_tflops = flops_ray * n / avg_m / 1e12
+    # the BVH4 the kernel actually walks, and its divergence
+    s4 = ctx.ray_fire_stats(0, sub[0], sub[1], tree_per_ray=sub[2], precision=capi.MIXED)
+    V4 = s4["nodes"] / s4["rays"]
+    simd_node = s4["nodes"] / (64 * max(s4["wave_node_iters"], 1))
+    simd_tri = s4["tri_tests"] / (64 * max(s4["wave_tri_iters"], 1))
+    log(f"per ray (binary FP64 layout): V {V:.2f}, F {F:.2f}, E {E:.2f}, C {Cf:.2f}, "
+        f"N {N:.2f} -> {bytes_ray:.0f} B/ray, {achieved:.0f} GB/s, FP64 "
+        f"{fp64_tflops:.2f} TFLOP/s; BVH4 nodes/ray {V4:.2f}, SIMD efficiency "
+        f"node loop {simd_node:.3f}, triangle loop {simd_tri:.3f}")
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "ray_fire_mixed_nested_spheres_pmc.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            rec = json.load(f)
+        if rec.get("queries") == n:
+            traffic = rec.get("hbm_bytes_per_launch")
+    res = {
+        "n": n, "avg_m": avg_m, "avg_p": avg_p, "mixed_mrays": mixed_mrays,
+        "plain_mrays": plain_mrays, "identical": identical, "rel": rel,
+        "fv_mqs": fv_mqs, "vol_identical": vol_identical,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "k_ray_fire<mixed> (+ deferred FP64 pass)",
+                     "bytes_per_ray": round(bytes_ray, 2),
+                     "work_per_ray": {"V": round(V, 3), "F": round(F, 3), "E": round(E, 3),
+                                      "C": round(Cf, 3), "N": round(N, 3)},
+                     "fp64_tflops": round(fp64_tflops, 4),
+                     "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                     "fp64_frac": round(fp64_tflops / FP64_PEAK_TFLOPS, 4),
+                     "bvh4_nodes_per_ray": round(V4, 3),
+                     "simd_efficiency_node_loop": round(simd_node, 4),
+                     "simd_efficiency_triangle_loop": round(simd_tri, 4)},
+        "model": model, "vols": vols, "pts": pts, "dirs": dirs,
+        "d_vol": d_vol, "t_mixed": tm,
+    }
+    return res
+
+
+# ---------------------------------------------------------------------------
+# tet walk (tally_segments)
+# ---------------------------------------------------------------------------
+
+def bench_tet(args, ctx, torch, stream, sh):
+    from xdg_amd import geometry as G
+    from xdg_amd import philox as P
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    mesh = G.kuhn_cube(args.tet_n, 0.2, seed=4)
+    et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
+    bm = mesh.boundary_model()
+    v9, f, s, fl = bm.tree_arrays(bm.ipc)
+    stree = ctx.build_surface_tree(
+        v9.reshape(-1, 3), np.arange(3 * v9.shape[0], dtype=np.int32).reshape(-1, 3),
+        f, s, fl, bm.bump(bm.ipc))
+    ctx.attach_boundary(et, stree, mesh.bface_elem)
+    ns_ = args.segments
+    p0 = torch.from_numpy(P.box_points(4, P.STREAM_POS, ns_, [0.0] * 3, [1.0] * 3)).to(dev)
+    p1 = torch.from_numpy(P.box_points(4, P.STREAM_DIR, ns_, [0.0] * 3, [1.0] * 3)).to(dev)
+    tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device=dev)
+    cnt = torch.empty(ns_, dtype=torch.int32, device=dev)
+    hsh = torch.empty(ns_, dtype=torch.int64, device=dev)
+    log(f"tet mesh {mesh.tets.shape[0]} tets built in {time.perf_counter() - t0:.2f}s")
+
+    def walk():
+        ctx.tally_segments(et, p0, p1, tally=tally, out=(cnt, hsh), stream=sh)
+
+    per_w, _ = time_launches(torch, stream, walk, max(2, args.steps // 5), 1)
+    avg_w = sum(per_w) / len(per_w)
+    steps_per_seg = cnt.double().mean().item()
+    status = ctx.status()
+    # bytes per segment = 48 + 16 + S (16 conn + 16 adj + 96 verts + 16 tally RMW)
+    seg_bytes = 64 + steps_per_seg * 144
+    tet = {"msegs_s": round(ns_ / avg_w / 1e6, 3), "ms_per_launch": round(avg_w * 1e3, 4),
+           "segments": ns_, "tets": int(mesh.tets.shape[0]),
+           "elements_per_segment": round(steps_per_seg, 3),
+           "element_steps_per_s": ns_ * steps_per_seg / avg_w,
+           "algorithmic_gbs": round(seg_bytes * ns_ / avg_w / 1e9, 2),
+           "capped": bool(status & 1)}
+    log(f"tet walk {tet['msegs_s']:.2f} Msegs/s, {steps_per_seg:.1f} elements/segment")
+    return tet
+
+
+# ---------------------------------------------------------------------------
+# particle transport: one captured graph launch over all histories
+# ---------------------------------------------------------------------------
+
+def bench_transport(args, ctx, torch, stream, sh):
+    from xdg_amd import capi
+    from xdg_amd import geometry as G
+    t0 = time.perf_counter()
+    m = G.sphere_lattice(8, 4, 0.8, 2.0)
+    vols = [v for v in m.volumes if v != m.ipc]
+    trees = build_trees(ctx, m, vols)
+    log(f"lattice: {len(trees)} volume trees built in {time.perf_counter() - t0:.2f}s")
+    h = ctx.transport_create(vols, trees, m.ipc, m.senses, 3, 3, [0.0] * 3, [16.0] * 3,
+                             2.0, 0.3, 100000)
+    n = args.histories
+    nvol = len(vols)
+    t0 = time.perf_counter()
+    tune_ms = ctx.transport_autotune(h, 0, min(n, 1 << 16), capi.MIXED)
+    cfg = ctx.transport_config(h)
+    log(f"autotune {time.perf_counter() - t0:.1f}s -> {cfg} ({tune_ms:.3f} ms / 65536)")
+    out = ctx.transport_outputs(nvol, n)
+    g = ctx.transport_capture(h, 0, n, out, capi.MIXED)
+    try:
+        per, _ = time_launches(torch, stream, lambda: ctx.graph_launch(g, sh),
+                               max(2, args.steps // 5), 1)
+    finally:
+        torch.cuda.synchronize()
+        ctx.graph_destroy(g)
+    avg = sum(per) / len(per)
+    ev = out["events"].long().sum().item()
+    st = torch.bincount(out["status"].long(), minlength=7).cpu().numpy()
+    tally = out["tally_fx"].cpu().numpy().astype(np.float64) / 2.0 ** 32
+    ctx.transport_destroy(h)
+    res = {"mhistories_s": round(n / avg / 1e6, 4), "ms_per_launch": round(avg * 1e3, 3),
+           "histories": n, "events_per_s": ev / avg, "events_per_history": ev / n,
+           "absorbed": int(st[1]), "leaked": int(st[2]),
+           "lost_stuck_capped_error": int(st[3:].sum()),
+           "total_track_length": float(tally.sum()), "config": cfg,
+           "launch": "captured hipGraph, one persistent kernel"}
+    log(f"transport {res['mhistories_s']:.3f} Mhistories/s, {ev / avg / 1e6:.1f} Mevents/s, "
+        f"status {st.tolist()}")
+    return res
+
+
+# ---------------------------------------------------------------------------
+# ray-distance adjoint
+# ---------------------------------------------------------------------------
+
+def bench_adjoint(args, ctx, torch, stream, sh):
+    from xdg_amd import capi
+    from xdg_amd import geometry as G
+    from xdg_amd import philox as P
+    from xdg_amd import adjoint as A
+    dev = torch.device("cuda", 0)
+    m = G.perturbed_sphere(6, 10.0, 0.05)
+    n = args.adjoint_rays
+    org = torch.from_numpy(P.ball_points(5, P.STREAM_POS, n, 9.5)).to(dev)
+    dirs = torch.from_numpy(P.iso_dirs(5, P.STREAM_DIR, n)).to(dev)
+    w = torch.from_numpy(P.normals(5, P.STREAM_AUX, n)).to(dev)
+    surf = A.DifferentiableSurface(ctx, m.tris)
+    verts = torch.from_numpy(m.verts).to(dev)
+    tree = surf.tree_for(verts)
+    t, s, f = ctx.ray_fire(tree, org, dirs, precision=capi.MIXED)
+    gv = torch.zeros_like(verts)
+    go = torch.empty_like(org)
+    gd = torch.empty_like(dirs)
+
+    def adj():
+        gv.zero_()
+        ctx.ray_distance_adjoint(org, dirs, t, f, w, verts, surf.tris, gv, go, gd, stream=sh)
+
+    per, _ = time_launches(torch, stream, adj, max(2, args.steps // 2), 1)
+    avg = sum(per) / len(per)
+    # bytes per ray: 48 ray + 8 t + 4 face + 8 dL/dt + 12 ids + 72 verts
+    # + 9 x 16 atomic RMW + 48 dL/d(o, d)
+    b = 48 + 8 + 4 + 8 + 12 + 72 + 144 + 48
+    res = {"mrays_s": round(n / avg / 1e6, 3), "ms_per_launch": round(avg * 1e3, 4),
+           "rays": n, "algorithmic_gbs": round(b * n / avg / 1e9, 2),
+           "grad_norm": float(gv.norm().item())}
+    log(f"adjoint {res['mrays_s']:.1f} Mrays/s ({res['ms_per_launch']:.3f} ms)")
+    return res
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the oracle's BVH path on this host's cores
+# ---------------------------------------------------------------------------
+
+def cpu_baseline(args, hl):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    model, vols, pts, dirs = hl["model"], hl["vols"], hl["pts"], hl["dirs"]
+    n = hl["n"]
+    ncpu = len(os.sched_getaffinity(0))
+    m = min(args.cpu_sample, n)
+    vol_h = hl["d_vol"][:m].cpu().numpy()
+    otrees = {v: O.Tree.from_model(model, v) for v in vols}
+    groups = [(v, np.nonzero(vol_h == v)[0]) for v in vols]
+    sub = [(v, idx, np.ascontiguousarray(pts[idx]), np.ascontiguousarray(dirs[idx]))
+           for v, idx in groups if idx.size]
+    res_t = np.empty(m)
+
+    def trace():
+        for v, idx, p_, d_ in sub:
+            t_, s_, f_ = otrees[v].ray_fire(p_, d_, nthreads=ncpu)
+            res_t[idx] = t_
+
+    trace()  # warm-up
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        trace()
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    agree = bool(np.array_equal(res_t.view(np.uint64),
+                                hl["t_mixed"][:m].cpu().numpy().view(np.uint64)))
+    cpu = {"value": round(m / dt / 1e6, 3), "unit": "Mrays/s", "cores": ncpu,
+           "kind": "port",
+           "sample": f"{m} of the {n} nested-sphere queries, ray_fire in their "
+                     f"containing volume with the C oracle (FP64 host BVH, "
+                     f"OpenMP dynamic schedule); warm-up then median of 5 "
+                     f"passes, {dt:.2f} s each, {sum(times) + dt:.1f} s in all; "
+                     f"distances bit-identical to the GPU: {agree}"}
+    log(f"cpu baseline {cpu['value']:.2f} Mrays/s on {ncpu} threads ({dt:.2f}s per pass)")
+    return cpu
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -82,8 +396,6 @@ def main():
 
     import torch
     from xdg_amd import capi
-    from xdg_amd import geometry as G
-    from xdg_amd import philox as P
 
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
@@ -102,170 +414,21 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    # ---- geometry and trees ------------------------------------------------
-    t0 = time.perf_counter()
-    model = G.nested_spheres(6)
-    vols = list(model.volumes)
-    trees = []
-    for v in vols:
-        v9, f, s, fl = model.tree_arrays(v)
-        nt = v9.shape[0]
-        trees.append(ctx.build_surface_tree(
-            v9.reshape(-1, 3), np.arange(3 * nt, dtype=np.int32).reshape(-1, 3),
-            f, s, fl, model.bump(v)))
-    log(f"built {len(trees)} surface trees in {time.perf_counter() - t0:.2f}s: "
-        + ", ".join(f"v{v}:{ctx.tree_info(t)['n_prims']}/d{ctx.tree_info(t)['depth']}"
-                    for v, t in zip(vols, trees)))
+    hl = bench_ray_fire(args, ctx, torch, stream, sh)
+    tet = None if args.no_tet else bench_tet(args, ctx, torch, stream, sh)
+    tr = None if args.no_transport else bench_transport(args, ctx, torch, stream, sh)
+    adj = None if args.no_adjoint else bench_adjoint(args, ctx, torch, stream, sh)
+    cpu = None if args.no_cpu else cpu_baseline(args, hl)
 
-    n = args.points
-    t0 = time.perf_counter()
-    pts = P.box_points(2, P.STREAM_POS, n, [-16.0] * 3, [16.0] * 3)
-    dirs = P.iso_dirs(2, P.STREAM_DIR, n)
-    d_pts = torch.from_numpy(pts).to(dev)
-    d_dirs = torch.from_numpy(dirs).to(dev)
-    log(f"generated {n} queries in {time.perf_counter() - t0:.2f}s")
-
-    # ---- find_volume (mixed and plain must agree) ---------------------------
-    d_vol = torch.empty(n, dtype=torch.int32, device=dev)
-    fv = lambda prec: ctx.find_volume(trees, vols, model.ipc, d_pts, d_dirs,  # noqa: E731
-                                      precision=prec, out=d_vol, stream=sh)
-    fv_per, _ = time_launches(torch, stream, lambda: fv(capi.MIXED), 3, 1)
-    vol_mixed = d_vol.clone()
-    fv(capi.PLAIN)
-    torch.cuda.synchronize()
-    vol_identical = bool(torch.equal(vol_mixed, d_vol))
-    fv_mqs = n / (sum(fv_per) / len(fv_per)) / 1e6
-    lut = torch.zeros(max(vols) + 1, dtype=torch.int32, device=dev)
-    for v, t in zip(vols, trees):
-        lut[v] = t
-    d_tpr = lut[d_vol.long()].contiguous()
-    counts = torch.bincount(d_vol.long(), minlength=max(vols) + 1).cpu().numpy()
-    log(f"find_volume {fv_mqs:.1f} Mqueries/s; volumes {counts.tolist()}; "
-        f"mixed==plain {vol_identical}")
-
-    # ---- ray_fire: mixed (value) and plain ---------------------------------
-    outs = {}
-    for prec in (capi.MIXED, capi.PLAIN):
-        outs[prec] = (torch.empty(n, dtype=torch.float64, device=dev),
-                      torch.empty(n, dtype=torch.int32, device=dev),
-                      torch.empty(n, dtype=torch.int32, device=dev))
-
-    def fire(prec):
-        ctx.ray_fire(0, d_pts, d_dirs, tree_per_ray=d_tpr, precision=prec,
-                     out=outs[prec], stream=sh)
-
-    per_m, wall_m = time_launches(torch, stream, lambda: fire(capi.MIXED),
-                                  args.steps, args.warmup)
-    per_p, wall_p = time_launches(torch, stream, lambda: fire(capi.PLAIN),
-                                  args.steps, args.warmup)
-    avg_m = sum(per_m) / len(per_m)
-    avg_p = sum(per_p) / len(per_p)
-    mixed_mrays = n / avg_m / 1e6
-    plain_mrays = n / avg_p / 1e6
-    tm, sm, fm = outs[capi.MIXED]
-    tp, sp_, fp = outs[capi.PLAIN]
-    identical = bool(torch.equal(sm, sp_) and torch.equal(fm, fp)
-                     and torch.equal(tm.view(torch.int64), tp.view(torch.int64)))
-    hit = sp_ >= 0
-    rel = ((tm[hit] - tp[hit]).abs() / tp[hit].abs()).max().item() if hit.any() else 0.0
-    log(f"ray_fire mixed {mixed_mrays:.1f} Mrays/s ({avg_m * 1e3:.3f} ms), plain "
-        f"{plain_mrays:.1f} Mrays/s ({avg_p * 1e3:.3f} ms), ratio "
-        f"{mixed_mrays / plain_mrays:.3f}, identical {identical}, max rel dt {rel:.3e}")
-
-    # ---- roofline accounting from the instrumented traversal ---------------
-    ns = min(n, 1 << 16)
-    st = ctx.ray_fire_stats(0, d_pts[:ns].contiguous(), d_dirs[:ns].contiguous(),
-                            tree_per_ray=d_tpr[:ns].contiguous(),
-                            precision=capi.MIXED)
-    V = st["nodes"] / st["rays"]
-    F = st["tri_tests"] / st["rays"]
-    E = st["edge_tests"] / st["rays"]
-    Cf = st["finishes"] / st["rays"]
-    N = st["normals"] / st["rays"]
-    # bytes per ray: origin+direction 48, tree id 4, t/surf/face 16;
-    # mixed node 64 B (two FP32 child boxes + ids), triangle record 80 B
-    bytes_ray = 48 + 4 + 16 + 64 * V + 80 * F
-    flops_ray = 9 * F + 23 * E + 26 * Cf + 29 * N  # FP64 (boxes are FP32)
-    achieved = bytes_ray * n / avg_m / 1e9
-    log(f"per ray: nodes {V:.2f}, tri tests {F:.2f}, edge tests {E:.2f}, "
-        f"finishes {Cf:.2f}, normals {N:.2f}, fallbacks {st['fallbacks']}; "
-        f"{bytes_ray:.0f} B/ray -> {achieved:.0f} GB/s; "
-        f"FP64 {flops_ray * n / avg_m / 1e12:.2f} TFLOP/s")
-    traffic = None
-    pmc = os.path.join(ROOT, "profiles", "ray_fire_mixed_nested_spheres_pmc.json")
-    if os.path.exists(pmc):
-        with open(pmc) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-
-    # ---- tet walk -------------------------------------------------------------
-    tet = None
-    if not args.no_tet:
-        t0 = time.perf_counter()
-        mesh = G.kuhn_cube(args.tet_n, 0.2, seed=4)
-        et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
-        bm = mesh.boundary_model()
-        v9, f, s, fl = bm.tree_arrays(bm.ipc)
-        stree = ctx.build_surface_tree(
-            v9.reshape(-1, 3), np.arange(3 * v9.shape[0], dtype=np.int32).reshape(-1, 3),
-            f, s, fl, bm.bump(bm.ipc))
-        ctx.attach_boundary(et, stree, mesh.bface_elem)
-        ns_ = args.segments
-        p0 = torch.from_numpy(P.box_points(4, P.STREAM_POS, ns_, [0.0] * 3, [1.0] * 3)).to(dev)
-        p1 = torch.from_numpy(P.box_points(4, P.STREAM_DIR, ns_, [0.0] * 3, [1.0] * 3)).to(dev)
-        tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device=dev)
-        cnt = torch.empty(ns_, dtype=torch.int32, device=dev)
-        hsh = torch.empty(ns_, dtype=torch.int64, device=dev)
-        log(f"tet mesh {mesh.tets.shape[0]} tets built in {time.perf_counter() - t0:.2f}s")
-
-        def walk():
-            ctx.tally_segments(et, p0, p1, tally=tally, out=(cnt, hsh), stream=sh)
-
-        per_w, _ = time_launches(torch, stream, walk, max(2, args.steps // 5), 1)
-        avg_w = sum(per_w) / len(per_w)
-        steps_per_seg = cnt.double().mean().item()
-        status = ctx.status()
-        tet = {"msegs_s": ns_ / avg_w / 1e6, "ms_per_launch": avg_w * 1e3,
-               "segments": ns_, "tets": int(mesh.tets.shape[0]),
-               "elements_per_segment": steps_per_seg,
-               "element_steps_per_s": ns_ * steps_per_seg / avg_w,
-               "capped": bool(status & 1)}
-        log(f"tet walk {tet['msegs_s']:.2f} Msegs/s, {steps_per_seg:.1f} elements/segment")
-
-    # ---- CPU baseline: the oracle's BVH path on this host's cores ----------
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as O
-        ncpu = len(os.sched_getaffinity(0))
-        m = min(args.cpu_sample, n)
-        vol_h = d_vol[:m].cpu().numpy()
-        otrees = {v: O.Tree.from_model(model, v) for v in vols if v != model.ipc}
-        otrees[model.ipc] = O.Tree.from_model(model, model.ipc)
-        groups = [(v, np.nonzero(vol_h == v)[0]) for v in vols]
-        t0 = time.perf_counter()
-        res_t = np.empty(m)
-        for v, idx in groups:
-            if idx.size:
-                t_, s_, f_ = otrees[v].ray_fire(pts[idx], dirs[idx], nthreads=ncpu)
-                res_t[idx] = t_
-        dt = time.perf_counter() - t0
-        agree = bool(np.array_equal(res_t.view(np.uint64),
-                                    tm[:m].cpu().numpy().view(np.uint64)))
-        cpu = {"value": m / dt / 1e6, "unit": "Mrays/s", "cores": ncpu,
-               "kind": "port",
-               "sample": f"first {m} of the {n} nested-sphere queries, ray_fire "
-                         f"in their containing volume (FP64 host BVH, OpenMP), "
-                         f"{dt:.1f} s; distances bit-identical to the GPU: {agree}"}
-        log(f"cpu baseline {cpu['value']:.2f} Mrays/s on {ncpu} threads ({dt:.1f}s)")
-
+    n = hl["n"]
     line = {
         "metric": METRIC,
-        "value": round(mixed_mrays, 3),
+        "value": round(hl["mixed_mrays"], 3),
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(avg_m * 1e3, 4),
+        "ms_per_step": round(hl["avg_m"] * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -276,21 +439,16 @@ def main():
                    "queries": n, "query_box": [-16, 16], "seed": 2,
                    "orientation": "EXITING", "box_precision": "f32 outward-rounded",
                    "triangle_test": "f64 Pluecker", "parallelism": "single GPU"},
-        "plain_fp64_mrays_s": round(plain_mrays, 3),
-        "mixed_over_plain": round(mixed_mrays / plain_mrays, 4),
-        "identical_hits": identical,
-        "max_rel_distance_error": rel,
-        "find_volume_mqueries_s": round(fv_mqs, 3),
-        "find_volume_mixed_equals_plain": vol_identical,
+        "plain_fp64_mrays_s": round(hl["plain_mrays"], 3),
+        "mixed_over_plain": round(hl["mixed_mrays"] / hl["plain_mrays"], 4),
+        "identical_hits": hl["identical"],
+        "max_rel_distance_error": hl["rel"],
+        "find_volume_mqueries_s": round(hl["fv_mqs"], 3),
+        "find_volume_mixed_equals_plain": hl["vol_identical"],
         "tet_walk": tet,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     "bytes_per_ray": round(bytes_ray, 2),
-                     "nodes_per_ray": round(V, 3), "tri_tests_per_ray": round(F, 3),
-                     "fp64_tflops": round(flops_ray * n / avg_m / 1e12, 4),
-                     "fp64_peak_tflops": FP64_PEAK_TFLOPS},
+        "transport": tr,
+        "adjoint": adj,
+        "roofline": hl["roofline"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

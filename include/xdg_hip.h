@@ -87,7 +87,7 @@ int xdg_hip_get_launch(xdg_hip_ctx* ctx, int* block, int* stack);
 int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent);
 /* Tuning options by name (no counterpart in the reference; results are
  * identical for every setting):
- *   "block"       workgroup size, multiple of 64 (default 256)
+ *   "block"       workgroup size, multiple of 64 (default 64)
  *   "stack"       minimum LDS stack entries per lane (default 1; raised to
  *                 what each tree needs)
  *   "persistent"  resident-grid ray_fire, bit 0 plain, bit 1 mixed (1)
@@ -98,7 +98,7 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *                 leaf until every lane of the wave holds one (0)
  *   "wide"        BVH4 traversal (1) or the binary LBVH (0)
  *   "defer"       mixed ray_fire traces far-origin rays in an FP64 pass (1)
- *   "leaf_size"   max triangles per leaf for trees built afterwards (4)
+ *   "leaf_size"   max triangles per leaf for trees built afterwards (2)
  * Unknown names give XDG_HIP_ERR_NOT_FOUND, out-of-range values
  * XDG_HIP_ERR_INVALID. */
 int xdg_hip_set_option(xdg_hip_ctx* ctx, const char* name, int value);

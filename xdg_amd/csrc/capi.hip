@@ -98,7 +98,7 @@ struct xdg_hip_ctx {
   RayOrderScratch order_w{};
   std::vector<TransportHost*> transports;  // handle = index + 1 (nullptr once destroyed)
   LaunchCfg cfg;
-  int leaf_max = 4;
+  int leaf_max = 2;
   int max_surf_depth = 1;
   int max_surf_stack4 = 1;
   int max_elem_depth = 1;
@@ -240,9 +240,9 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
     delete c;
     return XDG_HIP_ERR_HIP;
   }
-  c->cfg.block = env_int("XDG_HIP_BLOCK", 256);
+  c->cfg.block = env_int("XDG_HIP_BLOCK", 64);
   c->cfg.stack = env_int("XDG_HIP_STACK", 1);
-  c->leaf_max = env_int("XDG_HIP_LEAF", 4);
+  c->leaf_max = env_int("XDG_HIP_LEAF", 2);
   c->cfg.defer = env_int("XDG_HIP_DEFER", 1) != 0;
   c->cfg.wide = env_int("XDG_HIP_WIDE", 1) != 0;
   c->cfg.spec = env_int("XDG_HIP_SPEC", 0) != 0;

@@ -108,7 +108,7 @@ size_t ray_order_temp_bytes(int64_t n);
 const int32_t* launch_ray_order(const RayOrderArgs& a, int mode, const RayOrderScratch& w, hipStream_t s);
 
 struct LaunchCfg {
-  int block = 256;
+  int block = 64;
   int stack = 64;     // LDS stack entries per lane
   int defer = 1;  // mixed ray_fire: FP64 pass for far-origin rays
   int wide = 1;   // surface queries traverse the BVH4
