@@ -50,7 +50,7 @@ extern "C" {
 #define XDG_HIP_EXITING 0
 #define XDG_HIP_ENTERING 1
 
-#define XDG_HIP_MAX_EXCLUDE 8 /* exclusion slots per ray */
+#define XDG_HIP_MAX_EXCLUDE 1024 /* exclusion slots per ray */
 
 typedef struct xdg_hip_ctx xdg_hip_ctx;
 

@@ -426,3 +426,15 @@ def test_mock_segments_sequences(ctx, torch_mod, golden_dir):
         a, b = off[i], off[i] + rc[i]
         assert np.array_equal(ge[a:b], se[a:b])
         assert np.array_equal(bits(gl[a:b]), bits(sl[a:b]))
+
+
+def test_fresh_context_status_is_clear(torch_mod):
+    """The sticky status word starts at zero on every new context, also when
+    its memory is recycled from contexts that were destroyed."""
+    from xdg_amd import capi
+    for _ in range(3):
+        c = capi.Context(0)
+        junk = torch_mod.full((1 << 20,), -1, dtype=torch_mod.int32, device="cuda")
+        del junk
+        assert c.status() == 0
+        c.close()

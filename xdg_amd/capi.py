@@ -30,7 +30,7 @@ PERSIST_DEFAULT = 1  # persistent ray_fire for the plain kernel only
 ANY = -1
 EXITING = 0
 ENTERING = 1
-MAX_EXCLUDE = 8
+MAX_EXCLUDE = 1024
 
 INFTY = np.finfo(np.float64).max
 

@@ -236,7 +236,12 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->d_status, sizeof(uint32_t) * 4) != hipSuccess ||
       hipMalloc(&c->d_fb_count, sizeof(unsigned int) * 4) != hipSuccess ||
-      hipMalloc(&c->d_next, sizeof(unsigned long long) * 2) != hipSuccess) {
+      hipMalloc(&c->d_next, sizeof(unsigned long long) * 2) != hipSuccess ||
+      hipMemset(c->d_status, 0, sizeof(uint32_t) * 4) != hipSuccess) {  // status flags start clear
+    if (c->d_next) (void)hipFree(c->d_next);
+    if (c->d_fb_count) (void)hipFree(c->d_fb_count);
+    if (c->d_status) (void)hipFree(c->d_status);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return XDG_HIP_ERR_HIP;
   }
@@ -251,7 +256,7 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   try {
     sync_tables(c);
   } catch (...) {
-    delete c;
+    xdg_hip_destroy(c);
     return XDG_HIP_ERR_HIP;
   }
   *out = c;
@@ -642,7 +647,7 @@ int xdg_hip_ray_fire(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per_ray, 
     require(n >= 0, "negative n");
     require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
     require(orientation >= -1 && orientation <= 1, "unknown orientation");
-    require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 8]");
+    require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 1024]");
     int depth = tree_per_ray ? surf_need(c, nullptr) : surf_need(c, &surf_tree(c, tree));
     if (n == 0) return XDG_HIP_OK;
     require(org && dir && t_out && surf_out, "null array");
@@ -680,7 +685,7 @@ int xdg_hip_ray_fire_stats(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per
     require(counters != nullptr, "counters is null");
     require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
     require(orientation >= -1 && orientation <= 1, "unknown orientation");
-    require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 8]");
+    require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 1024]");
     int depth = tree_per_ray ? surf_need(c, nullptr) : surf_need(c, &surf_tree(c, tree));
     for (int j = 0; j < XDG_HIP_NUM_COUNTERS; ++j) counters[j] = 0;
     if (n == 0) return XDG_HIP_OK;
@@ -714,7 +719,7 @@ int xdg_hip_point_in_volume(xdg_hip_ctx* c, int32_t tree, int64_t n, const doubl
   return guarded(c, [&]() {
     require(n >= 0, "negative n");
     require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
-    require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 8]");
+    require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 1024]");
     const SurfTreeHost& t = surf_tree(c, tree);
     if (n == 0) return XDG_HIP_OK;
     require(pts && inside_out, "null array");

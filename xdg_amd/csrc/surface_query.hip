@@ -26,6 +26,7 @@
 
 #include "internal.h"
 #include "surface_traverse.h"
+#include "xdg_hip.h"
 
 namespace xdgh {
 
@@ -33,7 +34,7 @@ using namespace xdgd;
 
 namespace {
 
-constexpr int MAXK = 8;
+constexpr int MAXK = XDG_HIP_MAX_EXCLUDE;  // exclusion entries per ray (read in place)
 
 // One ray of ray_fire.  DEFER (mixed only): a ray beyond the FP32-rounding
 // limit is queued for the FP64 pass instead of traced here.
@@ -55,9 +56,8 @@ __device__ __forceinline__ void ray_fire_one(const RayFireArgs& a, int64_t i, in
     a.fb_list[j] = (int32_t)i;
     return;
   }
-  int32_t ex[MAXK];
-  int k = a.excl ? (a.k < MAXK ? a.k : MAXK) : 0;
-  for (int j = 0; j < k; ++j) ex[j] = a.excl[(int64_t)a.k * i + j];
+  const int32_t* ex = a.excl ? a.excl + (int64_t)a.k * i : nullptr;
+  const int k = a.excl ? (a.k < MAXK ? a.k : MAXK) : 0;
   HitState h;
   h.t = a.tmax ? a.tmax[i] : INFTY;
   h.face = INT32_MAX;
@@ -242,9 +242,8 @@ __global__ void k_ray_fire_stats(RayFireArgs a, int cap, unsigned long long* cou
   if (tree < 1 || tree > a.ntrees) return;
   const SurfTreeDev& T = a.table[tree];
   RayD ray = make_ray(a.org + 3 * i, a.dir + 3 * i);
-  int32_t ex[MAXK];
-  int k = a.excl ? (a.k < MAXK ? a.k : MAXK) : 0;
-  for (int j = 0; j < k; ++j) ex[j] = a.excl[(int64_t)a.k * i + j];
+  const int32_t* ex = a.excl ? a.excl + (int64_t)a.k * i : nullptr;
+  const int k = a.excl ? (a.k < MAXK ? a.k : MAXK) : 0;
   HitState h;
   h.t = a.tmax ? a.tmax[i] : INFTY;
   h.face = INT32_MAX;
@@ -287,9 +286,8 @@ __global__ void k_piv(PivArgs a, int cap)
   const SurfTreeDev& T = a.table[a.tree];
   const double* d = a.dir ? a.dir + 3 * i : a.dflt;
   RayD ray = make_ray(a.pts + 3 * i, d);
-  int32_t ex[MAXK];
-  int k = a.excl ? (a.k < MAXK ? a.k : MAXK) : 0;
-  for (int j = 0; j < k; ++j) ex[j] = a.excl[(int64_t)a.k * i + j];
+  const int32_t* ex = a.excl ? a.excl + (int64_t)a.k * i : nullptr;
+  const int k = a.excl ? (a.k < MAXK ? a.k : MAXK) : 0;
   HitState h;
   h.t = INFTY;
   h.face = INT32_MAX;
