@@ -564,10 +564,11 @@ void bvh4_collapse(const LbvhOut& bin, int64_t n, std::vector<DevBuf>& owned, Bv
         nf.hiy[j] = cs[j].fbox[4];
         nf.hiz[j] = cs[j].fbox[5];
       } else {
-        nd.lox[j] = nd.loy[j] = nd.loz[j] = 0.0;
-        nd.hix[j] = nd.hiy[j] = nd.hiz[j] = 0.0;
-        nf.lox[j] = nf.loy[j] = nf.loz[j] = 0.0f;
-        nf.hix[j] = nf.hiy[j] = nf.hiz[j] = 0.0f;
+        // inverted box: the near-plane entry distance is +inf for every ray
+        nd.lox[j] = nd.loy[j] = nd.loz[j] = HUGE_VAL;
+        nd.hix[j] = nd.hiy[j] = nd.hiz[j] = -HUGE_VAL;
+        nf.lox[j] = nf.loy[j] = nf.loz[j] = HUGE_VALF;
+        nf.hix[j] = nf.hiy[j] = nf.hiz[j] = -HUGE_VALF;
         nd.child[j] = nf.child[j] = EMPTY;
       }
     }
@@ -593,7 +594,10 @@ void bvh4_collapse(const LbvhOut& bin, int64_t n, std::vector<DevBuf>& owned, Bv
     for (const Node4D& nd : od) {
       for (int j = 0; j < 4; ++j) {
         const int32_t c = nd.child[j];
-        if (c == EMPTY) continue;
+        if (c == EMPTY) {
+          if (!(nd.lox[j] == HUGE_VAL && nd.hix[j] == -HUGE_VAL)) throw HipError{"bvh4: empty slot not inverted"};
+          continue;
+        }
         if (c >= 0) {
           if ((size_t)c >= od.size()) throw HipError{"bvh4: child index out of range"};
           continue;

@@ -137,6 +137,7 @@ __global__ void k_ray_fire_pt(RayFireArgs a, int cap, unsigned long long* next_r
   const int32_t* ex = nullptr;
   int k = 0;
   float tf = 0.0f;
+  PlaneOfs po{0, 0, 0, 0, 0, 0};
   int32_t node = DONE;
   while (true) {
     const bool need = node == DONE;
@@ -196,6 +197,7 @@ __global__ void k_ray_fire_pt(RayFireArgs a, int cap, unsigned long long* next_r
         bd.ny = -ray.o.y * bd.iy;
         bd.nz = -ray.o.z * bd.iz;
       }
+      po = MIXED ? plane_ofs_f(bf) : plane_ofs_d(bd);
       S.reset();
       have = true;
       node = TT.n_prims > 0 ? TT.root4 : DONE;
@@ -204,7 +206,7 @@ __global__ void k_ray_fire_pt(RayFireArgs a, int cap, unsigned long long* next_r
     while (node >= 0) {
       float te[4];
       int32_t ch[4];
-      node4_test<MIXED>(T, node, bf, bd, tf, h.t, te, ch);
+      node4_test<MIXED>(T, node, bf, bd, po, tf, h.t, te, ch);
       cswap(te[0], ch[0], te[1], ch[1]);
       cswap(te[2], ch[2], te[3], ch[3]);
       cswap(te[0], ch[0], te[2], ch[2]);

@@ -319,35 +319,85 @@ struct TreeView {
   const double* normals;
 };
 
+// Byte offsets of a ray's near and far planes inside a BVH4 node: with the
+// direction's sign per axis known, the entry distance of a slab is the near
+// plane's and the exit distance the far plane's, so the slab test needs no
+// min/max pair per axis.  An empty slot's inverted box (lo = +inf,
+// hi = -inf) then gives entry +inf and can never be hit.
+struct PlaneOfs {
+  uint32_t nx, fx, ny, fy, nz, fz;
+};
+template <bool MIXED>
+__device__ __forceinline__ PlaneOfs plane_ofs(bool negx, bool negy, bool negz)
+{
+  constexpr uint32_t P = MIXED ? 16u : 32u;  // bytes per plane (4 children)
+  PlaneOfs o;
+  o.nx = negx ? P : 0u;
+  o.fx = P - o.nx;
+  o.ny = 2 * P + (negy ? P : 0u);
+  o.fy = 5 * P - o.ny;
+  o.nz = 4 * P + (negz ? P : 0u);
+  o.fz = 9 * P - o.nz;
+  return o;
+}
+__device__ __forceinline__ PlaneOfs plane_ofs_f(const BoxRayF& b)
+{
+  return plane_ofs<true>(signbit(b.ix), signbit(b.iy), signbit(b.iz));
+}
+__device__ __forceinline__ PlaneOfs plane_ofs_d(const BoxRayD& b)
+{
+  return plane_ofs<false>(signbit(b.ix), signbit(b.iy), signbit(b.iz));
+}
+
+template <class V, class P>
+__device__ __forceinline__ V gload_at(const P* base, uint32_t byte_ofs)
+{
+  return gload(reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_ofs));
+}
+
 template <bool MIXED, class TT = SurfTreeDev>
 __device__ __forceinline__ void node4_test(const TT& T, int32_t node, const BoxRayF& bf, const BoxRayD& bd,
-                                           float tf, double tm, float* te, int32_t* ch)
+                                           const PlaneOfs& po, float tf, double tm, float* te, int32_t* ch)
 {
   if (MIXED) {
-    const Node4F nd = gload(T.nodes4_f + node);
+    const Node4F* nd = T.nodes4_f + node;
+    const float4 nx = gload_at<float4>(nd, po.nx), fx = gload_at<float4>(nd, po.fx);
+    const float4 ny = gload_at<float4>(nd, po.ny), fy = gload_at<float4>(nd, po.fy);
+    const float4 nz = gload_at<float4>(nd, po.nz), fz = gload_at<float4>(nd, po.fz);
+    const int4 cc = gload_at<int4>(nd, 96u);
+    const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+    const int32_t cca[4] = {cc.x, cc.y, cc.z, cc.w};
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const float x0 = fmaf(nd.lox[c], bf.ix, bf.nx), x1 = fmaf(nd.hix[c], bf.ix, bf.nx);
-      const float y0 = fmaf(nd.loy[c], bf.iy, bf.ny), y1 = fmaf(nd.hiy[c], bf.iy, bf.ny);
-      const float z0 = fmaf(nd.loz[c], bf.iz, bf.nz), z1 = fmaf(nd.hiz[c], bf.iz, bf.nz);
-      const float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-      const float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tf));
-      const bool hit = a <= b && nd.child[c] != CHILD_EMPTY;
+      const float a = fmaxf(fmaxf(fmaf(nxa[c], bf.ix, bf.nx), fmaf(nya[c], bf.iy, bf.ny)),
+                            fmaxf(fmaf(nza[c], bf.iz, bf.nz), 0.0f));
+      const float b = fminf(fminf(fmaf(fxa[c], bf.ix, bf.nx), fmaf(fya[c], bf.iy, bf.ny)),
+                            fminf(fmaf(fza[c], bf.iz, bf.nz), tf));
+      const bool hit = a <= b;
       te[c] = hit ? a : INFINITY;
-      ch[c] = hit ? nd.child[c] : CHILD_EMPTY;
+      ch[c] = hit ? cca[c] : CHILD_EMPTY;
     }
   } else {
-    const Node4D nd = gload(T.nodes4_d + node);
+    const Node4D* nd = T.nodes4_d + node;
+    const double4 nx = gload_at<double4>(nd, po.nx), fx = gload_at<double4>(nd, po.fx);
+    const double4 ny = gload_at<double4>(nd, po.ny), fy = gload_at<double4>(nd, po.fy);
+    const double4 nz = gload_at<double4>(nd, po.nz), fz = gload_at<double4>(nd, po.fz);
+    const int4 cc = gload_at<int4>(nd, 192u);
+    const double nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+    const double nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+    const double nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+    const int32_t cca[4] = {cc.x, cc.y, cc.z, cc.w};
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const double x0 = fma(nd.lox[c], bd.ix, bd.nx), x1 = fma(nd.hix[c], bd.ix, bd.nx);
-      const double y0 = fma(nd.loy[c], bd.iy, bd.ny), y1 = fma(nd.hiy[c], bd.iy, bd.ny);
-      const double z0 = fma(nd.loz[c], bd.iz, bd.nz), z1 = fma(nd.hiz[c], bd.iz, bd.nz);
-      const double a = fmax(fmax(fmin(x0, x1), fmin(y0, y1)), fmax(fmin(z0, z1), 0.0));
-      const double b = fmin(fmin(fmax(x0, x1), fmax(y0, y1)), fmin(fmax(z0, z1), tm));
-      const bool hit = a <= b && nd.child[c] != CHILD_EMPTY;
+      const double a = fmax(fmax(fma(nxa[c], bd.ix, bd.nx), fma(nya[c], bd.iy, bd.ny)),
+                            fmax(fma(nza[c], bd.iz, bd.nz), 0.0));
+      const double b = fmin(fmin(fma(fxa[c], bd.ix, bd.nx), fma(fya[c], bd.iy, bd.ny)),
+                            fmin(fma(fza[c], bd.iz, bd.nz), tm));
+      const bool hit = a <= b;
       te[c] = hit ? f32_down_d(a) : INFINITY;
-      ch[c] = hit ? nd.child[c] : CHILD_EMPTY;
+      ch[c] = hit ? cca[c] : CHILD_EMPTY;
     }
   }
 }
@@ -375,6 +425,7 @@ __device__ __forceinline__ bool traverse4_s(const SurfTreeDev& T, const RayD& ra
     bd.ny = -ray.o.y * bd.iy;
     bd.nz = -ray.o.z * bd.iz;
   }
+  const PlaneOfs po = MIXED ? plane_ofs_f(bf) : plane_ofs_d(bd);
   float tf = MODE == OCCL ? INFINITY : f32_up(h.t);
   int32_t node = T.root4;
   S.reset();
@@ -393,7 +444,7 @@ __device__ __forceinline__ bool traverse4_s(const SurfTreeDev& T, const RayD& ra
       }
       float te[4];
       int32_t ch[4];
-      node4_test<MIXED>(T, node, bf, bd, tf, MODE == OCCL ? INFTY : h.t, te, ch);
+      node4_test<MIXED>(T, node, bf, bd, po, tf, MODE == OCCL ? INFTY : h.t, te, ch);
       cswap(te[0], ch[0], te[1], ch[1]);
       cswap(te[2], ch[2], te[3], ch[3]);
       cswap(te[0], ch[0], te[2], ch[2]);
