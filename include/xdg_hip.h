@@ -95,7 +95,7 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *                 1 sorted by (tree, direction octant, origin Morton code),
  *                 2 as 1 plus one coarse direction bit per axis (0)
  *   "speculative" ray_fire BVH4 node loop keeps descending with a parked
- *                 leaf until every lane of the wave holds one (0)
+ *                 leaf until every lane of the wave holds one (1)
  *   "wide"        BVH4 traversal (1) or the binary LBVH (0)
  *   "defer"       mixed ray_fire traces far-origin rays in an FP64 pass (1)
  *   "leaf_size"   max triangles per leaf for trees built afterwards (2)

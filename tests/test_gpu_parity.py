@@ -224,7 +224,7 @@ def test_sphere_ray_fire_bitexact(ctx, torch_mod, sphere5, prec, orientation, wi
     finally:
         ctx.set_traversal()
         ctx.set_ray_order(0)
-        ctx.set_option("speculative", 0)
+        ctx.set_option("speculative", 1)  # the default
     rt, rs, rf = ot.ray_fire(org, dirs, orientation=orientation)
     assert np.array_equal(host(s), rs)
     assert np.array_equal(host(f), rf)
@@ -362,7 +362,7 @@ def test_nested_find_volume_and_fire(ctx, torch_mod):
                                    tree_per_ray=dev(torch, tpr), precision=prec)
         finally:
             ctx.set_ray_order(0)
-            ctx.set_option("speculative", 0)
+            ctx.set_option("speculative", 1)  # the default
         t, s, f = host(t), host(s), host(f)
         for v in vols:
             sel = vol == v

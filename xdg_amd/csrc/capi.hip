@@ -250,7 +250,7 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   c->leaf_max = env_int("XDG_HIP_LEAF", 2);
   c->cfg.defer = env_int("XDG_HIP_DEFER", 1) != 0;
   c->cfg.wide = env_int("XDG_HIP_WIDE", 1) != 0;
-  c->cfg.spec = env_int("XDG_HIP_SPEC", 0) != 0;
+  c->cfg.spec = env_int("XDG_HIP_SPEC", 1) != 0;
   c->cfg.persistent = env_int("XDG_HIP_PERSIST", 1) & 3;
   c->cfg.order = env_int("XDG_HIP_ORDER", 0);
   try {

@@ -116,7 +116,7 @@ struct LaunchCfg {
   // bit 0 plain, bit 1 mixed
   int persistent = 1;
   int order = 0;  // ray_fire coherence ordering mode (0 = off)
-  int spec = 0;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
+  int spec = 1;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);

@@ -121,7 +121,7 @@ __device__ __forceinline__ void write_hit(const RayFireArgs& a, int64_t i, const
 // for the FP64 pass (k_ray_fire_deferred).  Every lane leaves the loop once
 // the counter passes n, so the grid always drains.
 template <bool MIXED>
-__global__ void k_ray_fire_pt(RayFireArgs a, int cap, unsigned long long* next_ray)
+__device__ __forceinline__ void ray_fire_pt_body(const RayFireArgs& a, int cap, unsigned long long* next_ray)
 {
   extern __shared__ int32_t lds[];
   LdsStack S(lds + threadIdx.x, blockDim.x, cap);
@@ -229,6 +229,18 @@ __global__ void k_ray_fire_pt(RayFireArgs a, int cap, unsigned long long* next_r
       node = S.empty() ? DONE : S.pop();
     }
   }
+}
+
+// The mixed kernel holds five waves per SIMD (at most 102 VGPRs); left to
+// itself the compiler settles on four, which costs more than it saves.
+__global__ __attribute__((amdgpu_waves_per_eu(5))) void k_ray_fire_pt_mixed(RayFireArgs a, int cap,
+                                                                           unsigned long long* next_ray)
+{
+  ray_fire_pt_body<true>(a, cap, next_ray);
+}
+__global__ void k_ray_fire_pt_plain(RayFireArgs a, int cap, unsigned long long* next_ray)
+{
+  ray_fire_pt_body<false>(a, cap, next_ray);
 }
 
 // Instrumented ray_fire: identical traversal, plus per-ray work counters
@@ -475,13 +487,13 @@ void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipSt
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, a.next_ray);
     if (mixed) {
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, a.fb_count);
-      const int64_t gp = std::min(need, resident_grid(k_ray_fire_pt<true>, c.block, sh));
-      hipLaunchKernelGGL((k_ray_fire_pt<true>), dim3((unsigned)gp), b, sh, s, a, c.stack, a.next_ray);
+      const int64_t gp = std::min(need, resident_grid(k_ray_fire_pt_mixed, c.block, sh));
+      hipLaunchKernelGGL(k_ray_fire_pt_mixed, dim3((unsigned)gp), b, sh, s, a, c.stack, a.next_ray);
       const unsigned gd = (unsigned)std::min<int64_t>(need, 1024);
       hipLaunchKernelGGL((k_ray_fire_deferred<true>), dim3(gd), b, sh, s, a, c.stack);
     } else {
-      const int64_t gp = std::min(need, resident_grid(k_ray_fire_pt<false>, c.block, sh));
-      hipLaunchKernelGGL((k_ray_fire_pt<false>), dim3((unsigned)gp), b, sh, s, a, c.stack, a.next_ray);
+      const int64_t gp = std::min(need, resident_grid(k_ray_fire_pt_plain, c.block, sh));
+      hipLaunchKernelGGL(k_ray_fire_pt_plain, dim3((unsigned)gp), b, sh, s, a, c.stack, a.next_ray);
     }
     return;
   }

@@ -78,29 +78,27 @@ __device__ __forceinline__ D3 normalize(D3 a)
 }
 
 // first() plucker.h:35-43
+// lexicographic (x, y, z) order, plucker.h:35-43 (written without branches:
+// neighbouring lanes test different edges)
 __device__ __forceinline__ bool first(const D3& a, const D3& b)
 {
-  if (a.x < b.x) return true;
-  if (a.x > b.x) return false;
-  if (a.y < b.y) return true;
-  if (a.y > b.y) return false;
-  return a.z < b.z;
+  const bool lx = a.x < b.x, gx = a.x > b.x, ly = a.y < b.y, gy = a.y > b.y, lz = a.z < b.z;
+  return lx || (!gx && (ly || (!gy && lz)));
 }
 
-// plucker_edge_test plucker.h:45-62
+// plucker_edge_test plucker.h:45-62.  The canonical endpoint is chosen by
+// selects and the one expression is evaluated for both orientations, so a
+// wave whose lanes disagree on the order does not run both branches; the
+// operations and their order are those of the reference in either case.
 __device__ __forceinline__ double edge_test(const D3& va, const D3& vb, const D3& ray, const D3& rn)
 {
-  double pip;
-  if (first(va, vb)) {
-    const D3 e = sub(vb, va);
-    const D3 en = cross(e, va);
-    pip = dot(ray, en) + dot(rn, e);
-  } else {
-    const D3 e = sub(va, vb);
-    const D3 en = cross(e, vb);
-    pip = dot(ray, en) + dot(rn, e);
-    pip = -pip;
-  }
+  const bool f = first(va, vb);
+  const D3 p = f ? va : vb;
+  const D3 q = f ? vb : va;
+  const D3 e = sub(q, p);
+  const D3 en = cross(e, p);
+  double pip = dot(ray, en) + dot(rn, e);
+  pip = f ? pip : -pip;
   if (PLUCKER_ZERO_TOL > fabs(pip)) pip = 0.0;
   return pip;
 }
