@@ -275,9 +275,10 @@ def bench_transport(args, ctx, torch, stream, sh):
     n = args.histories
     nvol = len(vols)
     t0 = time.perf_counter()
-    tune_ms = ctx.transport_autotune(h, 0, min(n, 1 << 16), capi.MIXED)
+    ns = min(n, 1 << 18)
+    tune_ms = ctx.transport_autotune(h, 0, ns, capi.MIXED)
     cfg = ctx.transport_config(h)
-    log(f"autotune {time.perf_counter() - t0:.1f}s -> {cfg} ({tune_ms:.3f} ms / 65536)")
+    log(f"autotune {time.perf_counter() - t0:.1f}s -> {cfg} ({tune_ms:.3f} ms / {ns} histories)")
     out = ctx.transport_outputs(nvol, n)
     g = ctx.transport_capture(h, 0, n, out, capi.MIXED)
     try:

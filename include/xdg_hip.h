@@ -81,7 +81,7 @@ int xdg_hip_get_launch(xdg_hip_ctx* ctx, int* block, int* stack);
 /* Traversal variant: wide != 0 uses the 4-wide tree (default), else the
  * binary LBVH; defer != 0 (default) traces far-origin mixed ray_fire rays in
  * a separate FP64 pass; persistent selects, per precision (bit 0 plain,
- * bit 1 mixed; default 1), ray_fire on a device-resident grid whose lanes
+ * bit 1 mixed; default 3), ray_fire on a device-resident grid whose lanes
  * claim new rays from a counter as they finish.  Results are identical
  * either way. */
 int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent);
@@ -90,7 +90,7 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *   "block"       workgroup size, multiple of 64 (default 64)
  *   "stack"       minimum LDS stack entries per lane (default 1; raised to
  *                 what each tree needs)
- *   "persistent"  resident-grid ray_fire, bit 0 plain, bit 1 mixed (1)
+ *   "persistent"  resident-grid ray_fire, bit 0 plain, bit 1 mixed (3)
  *   "ray_order"   ray_fire batches of >= 65536 rays: 0 caller's order,
  *                 1 sorted by (tree, direction octant, origin Morton code),
  *                 2 as 1 plus one coarse direction bit per axis (0)
@@ -99,6 +99,8 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *   "wide"        BVH4 traversal (1) or the binary LBVH (0)
  *   "defer"       mixed ray_fire traces far-origin rays in an FP64 pass (1)
  *   "leaf_size"   max triangles per leaf for trees built afterwards (2)
+ *   "refill"      persistent ray_fire: idle lanes a wave waits for before
+ *                 it claims new rays (1..64, default 40)
  * Unknown names give XDG_HIP_ERR_NOT_FOUND, out-of-range values
  * XDG_HIP_ERR_INVALID. */
 int xdg_hip_set_option(xdg_hip_ctx* ctx, const char* name, int value);
@@ -272,11 +274,12 @@ int xdg_hip_transport_create(xdg_hip_ctx* ctx, const xdg_hip_transport_problem* 
 int xdg_hip_transport_destroy(xdg_hip_ctx* ctx, int32_t handle);
 /* Launch configuration (0 keeps a value): workgroup size (64..256), traversal stack
  * entries held in LDS (the rest spill to a global area; capped at what the
- * trees need), histories claimed per lane at a time.  stack_need = entries
+ * trees need), histories claimed per lane at a time, idle lanes a wave
+ * waits for before it starts new histories (1..64).  stack_need = entries
  * the deepest tree of the problem can need. */
-int xdg_hip_transport_set_config(xdg_hip_ctx* ctx, int32_t handle, int block, int lds_stack, int chunk);
+int xdg_hip_transport_set_config(xdg_hip_ctx* ctx, int32_t handle, int block, int lds_stack, int chunk, int refill);
 int xdg_hip_transport_get_config(xdg_hip_ctx* ctx, int32_t handle, int* block, int* lds_stack, int* chunk,
-                                 int* stack_need);
+                                 int* refill, int* stack_need);
 /* Histories [first, first + n): device outputs tally_fx (nvol, per volume
  * slot, int64 two's complement of sum(llrint(len * 2^32))), and nullable
  * per-history events (int32), status (uint8), sequence hash (uint64) over
@@ -290,8 +293,9 @@ int xdg_hip_transport_capture(xdg_hip_ctx* ctx, int32_t handle, uint64_t first, 
                               uint64_t* tally_fx, int32_t* events, uint8_t* status, uint64_t* hash, void** graph);
 int xdg_hip_graph_launch(xdg_hip_ctx* ctx, void* graph, void* stream);
 int xdg_hip_graph_destroy(xdg_hip_ctx* ctx, void* graph);
-/* Times block x lds_stack x chunk candidates on histories [first, first +
- * sample) and keeps the fastest configuration (synchronous). */
+/* Times block x lds_stack x chunk x refill candidates on histories
+ * [first, first + sample) and keeps the fastest configuration
+ * (synchronous). */
 int xdg_hip_transport_autotune(xdg_hip_ctx* ctx, int32_t handle, uint64_t first, int64_t sample, int precision,
                                double* best_ms);
 

@@ -231,6 +231,29 @@ def test_sphere_ray_fire_bitexact(ctx, torch_mod, sphere5, prec, orientation, wi
     assert np.array_equal(bits(host(t)), bits(rt))
 
 
+@pytest.mark.parametrize("refill", [1, 17, 64])
+def test_persistent_refill_thresholds(ctx, torch_mod, sphere5, refill):
+    """Persistent ray_fire with lanes refilled one at a time, in groups, or
+    only once the whole wave is idle: identical to the oracle."""
+    torch = torch_mod
+    m, tree, ot = sphere5
+    n = 70000
+    org = P.ball_points(31, P.STREAM_POS, n, 9.0)
+    dirs = P.iso_dirs(31, P.STREAM_DIR, n)
+    rt, rs, rf = ot.ray_fire(org, dirs)
+    ctx.set_traversal(persistent=True)
+    ctx.set_option("refill", refill)
+    try:
+        for prec in (0, 1):
+            t, s, f = ctx.ray_fire(tree, dev(torch, org), dev(torch, dirs), precision=prec)
+            assert np.array_equal(host(s), rs)
+            assert np.array_equal(host(f), rf)
+            assert np.array_equal(bits(host(t)), bits(rt))
+    finally:
+        ctx.set_traversal()
+        ctx.set_option("refill", 40)  # the default
+
+
 @pytest.mark.parametrize("wide", [True, False])
 def test_nested_piv_occluded_variants(ctx, torch_mod, wide):
     """Binary and 4-wide traversal, plain and mixed, give identical

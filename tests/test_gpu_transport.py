@@ -75,18 +75,19 @@ def test_transport_matches_oracle(ctx, lattice, prec):
 
 def test_transport_configs_and_autotune(ctx, lattice):
     """Block size, LDS stack depth (down to one entry, the rest spilling to
-    global memory) and claim chunk change nothing in the results."""
+    global memory), claim chunk and refill threshold change nothing in the
+    results."""
     h, ref = _problem(ctx, lattice, mfp=0.7, p_absorb=0.2)
     try:
         n = 8000
         r = ref(0, n)
         need = ctx.transport_config(h)["stack_need"]
         assert need >= 2
-        for block, ls, chunk in [(64, 1, 1), (128, need, 4), (256, 2, 16),
-                                 (192, need - 1, 3)]:
-            ctx.transport_set_config(h, block, ls, chunk)
+        for block, ls, chunk, refill in [(64, 1, 1, 1), (128, need, 4, 32), (256, 2, 16, 64),
+                                         (192, need - 1, 3, 7)]:
+            ctx.transport_set_config(h, block, ls, chunk, refill)
             cfg = ctx.transport_config(h)
-            assert (cfg["block"], cfg["lds_stack"], cfg["chunk"]) == (block, ls, chunk)
+            assert (cfg["block"], cfg["lds_stack"], cfg["chunk"], cfg["refill"]) == (block, ls, chunk, refill)
             _check(ctx.transport_run(h, 0, n, len(lattice[1])), r)
         ms = ctx.transport_autotune(h, 0, 4096)
         assert ms > 0.0

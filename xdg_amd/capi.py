@@ -26,7 +26,7 @@ ERR_CAPPED = -5
 
 PLAIN = 0
 MIXED = 1
-PERSIST_DEFAULT = 1  # persistent ray_fire for the plain kernel only
+PERSIST_DEFAULT = 3  # persistent ray_fire for both precisions
 ANY = -1
 EXITING = 0
 ENTERING = 1
@@ -116,8 +116,9 @@ def lib():
                                          C.POINTER(_i32)],
             "xdg_hip_transport_destroy": [_vp, _i32],
             "xdg_hip_transport_set_config": [_vp, _i32, C.c_int, C.c_int,
-                                             C.c_int],
+                                             C.c_int, C.c_int],
             "xdg_hip_transport_get_config": [_vp, _i32, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int),
                                              C.POINTER(C.c_int),
                                              C.POINTER(C.c_int),
                                              C.POINTER(C.c_int)],
@@ -215,8 +216,8 @@ class Context:
         self._check(lib().xdg_hip_set_launch(self.h, block, stack))
 
     def set_traversal(self, wide=True, defer=True, persistent=None):
-        """persistent: None for the default (plain only), True / False for
-        both precisions, or the raw mask (bit 0 plain, bit 1 mixed)."""
+        """persistent: None for the default (both precisions), True / False
+        for both precisions, or the raw mask (bit 0 plain, bit 1 mixed)."""
         if persistent is None:
             mask = PERSIST_DEFAULT
         elif persistent is True or persistent is False:
@@ -430,15 +431,16 @@ class Context:
     def transport_destroy(self, handle):
         self._check(lib().xdg_hip_transport_destroy(self.h, handle))
 
-    def transport_set_config(self, handle, block=0, lds_stack=0, chunk=0):
+    def transport_set_config(self, handle, block=0, lds_stack=0, chunk=0,
+                             refill=0):
         self._check(lib().xdg_hip_transport_set_config(self.h, handle, block,
-                                                       lds_stack, chunk))
+                                                       lds_stack, chunk, refill))
 
     def transport_config(self, handle):
-        v = [C.c_int() for _ in range(4)]
+        v = [C.c_int() for _ in range(5)]
         self._check(lib().xdg_hip_transport_get_config(
             self.h, handle, *[C.byref(x) for x in v]))
-        return dict(zip(["block", "lds_stack", "chunk", "stack_need"],
+        return dict(zip(["block", "lds_stack", "chunk", "refill", "stack_need"],
                         [x.value for x in v]))
 
     def transport_outputs(self, nvol, n, per_history=True):

@@ -35,7 +35,7 @@ struct SurfTreeHost {
 struct TransportHost {
   TransportDev dev{};
   std::vector<DevBuf> bufs;
-  int block = 128, lds_stack = 0, chunk = 1;
+  int block = 128, lds_stack = 0, chunk = 1, refill = 1;
   int stack_need = 1;
   int32_t* spill = nullptr;
   size_t spill_entries = 0;
@@ -251,8 +251,9 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   c->cfg.defer = env_int("XDG_HIP_DEFER", 1) != 0;
   c->cfg.wide = env_int("XDG_HIP_WIDE", 1) != 0;
   c->cfg.spec = env_int("XDG_HIP_SPEC", 1) != 0;
-  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 1) & 3;
+  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 3) & 3;
   c->cfg.order = env_int("XDG_HIP_ORDER", 0);
+  c->cfg.refill = env_int("XDG_HIP_REFILL", 40);
   try {
     sync_tables(c);
   } catch (...) {
@@ -333,11 +334,13 @@ int* opt_spec(xdg_hip_ctx* c) { return &c->cfg.spec; }
 int* opt_wide(xdg_hip_ctx* c) { return &c->cfg.wide; }
 int* opt_defer(xdg_hip_ctx* c) { return &c->cfg.defer; }
 int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
+int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
 
 const OptionRef OPTIONS[] = {
     {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 3, opt_persistent},
     {"ray_order", 0, 2, opt_order},         {"speculative", 0, 1, opt_spec}, {"wide", 0, 1, opt_wide},
     {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
+    {"refill", 1, 64, opt_refill},
 };
 
 const OptionRef& find_option(const char* name)
@@ -1025,6 +1028,7 @@ void transport_launch(xdg_hip_ctx* c, TransportHost& t, const TransportLaunch& L
   R.spill = t.stack_need > t.lds_stack ? t.spill : nullptr;
   R.spill_lanes = L.grid * L.block;
   R.lds_tally = t.dev.nvol <= TRANSPORT_LDS_TALLY_MAX ? 1 : 0;
+  R.refill = t.refill;
   (void)c;
   launch_transport(t.dev, R, mixed, L, s);
 }
@@ -1111,23 +1115,25 @@ int xdg_hip_transport_destroy(xdg_hip_ctx* c, int32_t handle)
   });
 }
 
-int xdg_hip_transport_set_config(xdg_hip_ctx* c, int32_t handle, int block, int lds_stack, int chunk)
+int xdg_hip_transport_set_config(xdg_hip_ctx* c, int32_t handle, int block, int lds_stack, int chunk, int refill)
 {
   return guarded(c, [&]() {
     TransportHost& t = transport_of(c, handle);
     require(block == 0 || (block >= 64 && block <= 256 && block % 64 == 0), "block must be 64, 128, 192 or 256");
     require(lds_stack >= 0 && lds_stack <= 64, "lds_stack must be in [0, 64]");
     require(chunk >= 0 && chunk <= 4096, "chunk must be in [0, 4096]");
+    require(refill >= 0 && refill <= 64, "refill must be in [0, 64]");
     if (block) t.block = block;
     if (lds_stack) t.lds_stack = lds_stack;
     if (chunk) t.chunk = chunk;
+    if (refill) t.refill = refill;
     transport_refresh(c, t);
     return XDG_HIP_OK;
   });
 }
 
 int xdg_hip_transport_get_config(xdg_hip_ctx* c, int32_t handle, int* block, int* lds_stack, int* chunk,
-                                 int* stack_need)
+                                 int* refill, int* stack_need)
 {
   return guarded(c, [&]() {
     TransportHost& t = transport_of(c, handle);
@@ -1135,6 +1141,7 @@ int xdg_hip_transport_get_config(xdg_hip_ctx* c, int32_t handle, int* block, int
     if (block) *block = t.block;
     if (lds_stack) *lds_stack = t.lds_stack;
     if (chunk) *chunk = t.chunk;
+    if (refill) *refill = t.refill;
     if (stack_need) *stack_need = t.stack_need;
     return XDG_HIP_OK;
   });
@@ -1220,19 +1227,21 @@ int xdg_hip_transport_autotune(xdg_hip_ctx* c, int32_t handle, uint64_t first, i
     hipEvent_t e0 = nullptr, e1 = nullptr;
     XDG_HIP_CHECK(hipEventCreate(&e0));
     XDG_HIP_CHECK(hipEventCreate(&e1));
-    const int keep[3] = {t.block, t.lds_stack, t.chunk};
+    const int keep[4] = {t.block, t.lds_stack, t.chunk, t.refill};
     double best = -1.0;
-    int bb = keep[0], bs = keep[1], bc = keep[2];
+    int bb = keep[0], bs = keep[1], bc = keep[2], br = keep[3];
     try {
       std::vector<int> stacks = {std::min(t.stack_need, 8), std::min(t.stack_need, 16), t.stack_need};
       std::sort(stacks.begin(), stacks.end());
       stacks.erase(std::unique(stacks.begin(), stacks.end()), stacks.end());
       for (int block : {64, 128, 256})
         for (int ls : stacks)
-          for (int chunk : {1, 4, 16}) {
+          for (int chunk : {1, 4, 16})
+            for (int refill : {1, 32}) {
             t.block = block;
             t.lds_stack = ls;
             t.chunk = chunk;
+            t.refill = refill;
             TransportLaunch L;
             try {
               L = transport_prepare(c, t, mixed, true);
@@ -1252,12 +1261,14 @@ int xdg_hip_transport_autotune(xdg_hip_ctx* c, int32_t handle, uint64_t first, i
               bb = block;
               bs = ls;
               bc = chunk;
+              br = refill;
             }
-          }
+            }
     } catch (...) {
       t.block = keep[0];
       t.lds_stack = keep[1];
       t.chunk = keep[2];
+      t.refill = keep[3];
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       dev_free_all(tmp);
@@ -1271,6 +1282,7 @@ int xdg_hip_transport_autotune(xdg_hip_ctx* c, int32_t handle, uint64_t first, i
     t.block = bb;
     t.lds_stack = bs;
     t.chunk = bc;
+    t.refill = br;
     transport_prepare(c, t, mixed, true);
     if (best_ms) *best_ms = best;
     return XDG_HIP_OK;

@@ -114,9 +114,10 @@ struct LaunchCfg {
   int wide = 1;   // surface queries traverse the BVH4
   // ray_fire on a resident grid with per-lane ray refill, per precision:
   // bit 0 plain, bit 1 mixed
-  int persistent = 1;
+  int persistent = 3;
   int order = 0;  // ray_fire coherence ordering mode (0 = off)
   int spec = 1;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
+  int refill = 40; // persistent ray_fire: idle lanes of a wave needed before a refill
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
@@ -211,6 +212,7 @@ struct TransportRun {
   int32_t* spill;                // global stack share: (cap - lds) * spill_lanes
   int64_t spill_lanes;
   int lds_tally;                 // per-workgroup LDS tally (nvol small enough)
+  int refill;                    // idle lanes a wave waits for before starting histories
 };
 struct TransportLaunch {
   int block;

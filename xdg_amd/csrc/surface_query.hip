@@ -121,7 +121,8 @@ __device__ __forceinline__ void write_hit(const RayFireArgs& a, int64_t i, const
 // for the FP64 pass (k_ray_fire_deferred).  Every lane leaves the loop once
 // the counter passes n, so the grid always drains.
 template <bool MIXED>
-__device__ __forceinline__ void ray_fire_pt_body(const RayFireArgs& a, int cap, unsigned long long* next_ray)
+__device__ __forceinline__ void ray_fire_pt_body(const RayFireArgs& a, int cap, unsigned long long* next_ray,
+                                                 int refill_min)
 {
   extern __shared__ int32_t lds[];
   LdsStack S(lds + threadIdx.x, blockDim.x, cap);
@@ -140,12 +141,19 @@ __device__ __forceinline__ void ray_fire_pt_body(const RayFireArgs& a, int cap, 
   PlaneOfs po{0, 0, 0, 0, 0, 0};
   int32_t node = DONE;
   while (true) {
-    const bool need = node == DONE;
-    if (need && have) {
+    const bool done = node == DONE;
+    if (done && have) {
       write_hit(a, ri, h);
       have = false;
     }
-    const unsigned long long mask = __ballot(need);
+    // Refill only once enough lanes are idle (or every remaining lane is):
+    // the refill path is long, and running it for one lane at a time costs
+    // the whole wave more than the idle lanes waste.
+    const unsigned long long dmask = __ballot(done);
+    const int nact = __popcll(__ballot(1));
+    const bool go = __popcll(dmask) >= (refill_min < nact ? refill_min : nact);
+    const bool need = go && done;
+    const unsigned long long mask = go ? dmask : 0ull;
     unsigned long long base = 0;
     if (mask) {
       const int leader = __ffsll((long long)mask) - 1;
@@ -234,13 +242,14 @@ __device__ __forceinline__ void ray_fire_pt_body(const RayFireArgs& a, int cap, 
 // The mixed kernel holds five waves per SIMD (at most 102 VGPRs); left to
 // itself the compiler settles on four, which costs more than it saves.
 __global__ __attribute__((amdgpu_waves_per_eu(5))) void k_ray_fire_pt_mixed(RayFireArgs a, int cap,
-                                                                           unsigned long long* next_ray)
+                                                                           unsigned long long* next_ray,
+                                                                           int refill_min)
 {
-  ray_fire_pt_body<true>(a, cap, next_ray);
+  ray_fire_pt_body<true>(a, cap, next_ray, refill_min);
 }
-__global__ void k_ray_fire_pt_plain(RayFireArgs a, int cap, unsigned long long* next_ray)
+__global__ void k_ray_fire_pt_plain(RayFireArgs a, int cap, unsigned long long* next_ray, int refill_min)
 {
-  ray_fire_pt_body<false>(a, cap, next_ray);
+  ray_fire_pt_body<false>(a, cap, next_ray, refill_min);
 }
 
 // Instrumented ray_fire: identical traversal, plus per-ray work counters
@@ -488,12 +497,12 @@ void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipSt
     if (mixed) {
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, a.fb_count);
       const int64_t gp = std::min(need, resident_grid(k_ray_fire_pt_mixed, c.block, sh));
-      hipLaunchKernelGGL(k_ray_fire_pt_mixed, dim3((unsigned)gp), b, sh, s, a, c.stack, a.next_ray);
+      hipLaunchKernelGGL(k_ray_fire_pt_mixed, dim3((unsigned)gp), b, sh, s, a, c.stack, a.next_ray, c.refill);
       const unsigned gd = (unsigned)std::min<int64_t>(need, 1024);
       hipLaunchKernelGGL((k_ray_fire_deferred<true>), dim3(gd), b, sh, s, a, c.stack);
     } else {
       const int64_t gp = std::min(need, resident_grid(k_ray_fire_pt_plain, c.block, sh));
-      hipLaunchKernelGGL(k_ray_fire_pt_plain, dim3((unsigned)gp), b, sh, s, a, c.stack, a.next_ray);
+      hipLaunchKernelGGL(k_ray_fire_pt_plain, dim3((unsigned)gp), b, sh, s, a, c.stack, a.next_ray, c.refill);
     }
     return;
   }

@@ -139,13 +139,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   uint8_t st = TR_ALIVE;
   unsigned long long hh = HASH_SEED;
   while (true) {
-    if (!alive) {
-      if (have) {
-        if (R.events) R.events[hix] = ev;
-        if (R.status) R.status[hix] = st;
-        if (R.hash) R.hash[hix] = hh;
-        have = false;
-      }
+    // a lane whose history ended waits until at least R.refill lanes of the
+    // wave (or all remaining ones) are idle: starting a history (sampling,
+    // find_volume over every volume) is long, and the wave runs it for all
+    // lanes that start together
+    const bool idle = !alive;
+    if (idle && have) {
+      if (R.events) R.events[hix] = ev;
+      if (R.status) R.status[hix] = st;
+      if (R.hash) R.hash[hix] = hh;
+      have = false;
+    }
+    const int nact = __popcll(__ballot(1));
+    const int nidle = __popcll(__ballot(idle));
+    const bool go = nidle >= (R.refill < nact ? R.refill : nact);
+    if (idle) {
+      if (!go) continue;
       const bool need = cur >= end;
       const unsigned long long mask = __ballot(need);
       if (mask) {
