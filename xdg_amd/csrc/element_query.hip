@@ -113,11 +113,73 @@ __device__ __forceinline__ D3 face_vertex(const ElemTreeDev& T, const ElemRec& E
   return v[fo_at<FO>(T, f, j)];
 }
 
-// MeshManager::next_element (mesh_manager_interface.cpp:150-200)
+// Edge value of a face edge (i -> j) from the canonical value of the vertex
+// pair (min, max): plucker_edge_test orients every edge by the
+// lexicographic order of its endpoints (plucker.h:35-62), so the reversed
+// edge gives exactly the negated value, except that a value snapped to
+// zero stays +0.0.
+template <int I, int J>
+__device__ __forceinline__ double pair_edge(const double* e6)
+{
+  constexpr int lo = I < J ? I : J, hi = I < J ? J : I;
+  constexpr int k = lo == 0 ? hi - 1 : (lo == 1 ? hi + 1 : 5);  // (0,1)(0,2)(0,3)(1,2)(1,3)(2,3)
+  const double e = e6[k];
+  if (I < J) return e;
+  return e != 0.0 ? -e : 0.0;
+}
+
+template <int FO, int F>
+__device__ __forceinline__ void face_from_edges(const D3* v, const double* e6, const D3& u, int idx, double r_i,
+                                                double u_i, int& idx_out, double& min_dist)
+{
+  constexpr int moab[12] = {0, 1, 3, 1, 2, 3, 2, 0, 3, 0, 2, 1};
+  constexpr int mock[12] = {0, 1, 2, 0, 2, 3, 0, 3, 1, 1, 3, 2};
+  constexpr int A = FO == FO_MOAB ? moab[3 * F] : mock[3 * F];
+  constexpr int B = FO == FO_MOAB ? moab[3 * F + 1] : mock[3 * F + 1];
+  constexpr int C = FO == FO_MOAB ? moab[3 * F + 2] : mock[3 * F + 2];
+  // plucker_ray_tri_intersect with orientation 1 (plucker.h:64-161): reject
+  // on a positive coordinate, then the distance from the barycentric point
+  const double c0 = pair_edge<A, B>(e6);
+  const double c1 = pair_edge<B, C>(e6);
+  const double c2 = pair_edge<C, A>(e6);
+  double df = INFTY;
+  double t;
+  if (!(1.0 * c0 > 0) && !(1.0 * c1 > 0) && !(1.0 * c2 > 0) &&
+      plucker_finish(c0, c1, c2, v[A], v[B], v[C], r_i, u_i, idx, INFTY, 0.0, t))
+    df = (0.0 < t) ? t : 0.0;
+  if (df < min_dist) {
+    min_dist = df;
+    idx_out = F;
+  }
+}
+
+// MeshManager::next_element (mesh_manager_interface.cpp:150-200).  With a
+// fixed face order the six edges of the tet are evaluated once each (every
+// face edge is one of them, up to its sign), instead of up to twelve times
+// with per-face early exits that make neighbouring lanes diverge.
 template <int FO>
 __device__ __forceinline__ void next_element_dev(const ElemTreeDev& T, int32_t elem, const D3& r, const D3& u,
                                                  int idx, double u_i, int32_t& next, double& dist)
 {
+  if (FO != FO_TABLE) {
+    const auto E = as_global(T.elems + elem);
+    D3 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = d3(E->v[3 * j], E->v[3 * j + 1], E->v[3 * j + 2]);
+    const D3 rb = cross(u, r);
+    const double r_i = pick3(r.x, r.y, r.z, idx);
+    const double e6[6] = {edge_test(v[0], v[1], u, rb), edge_test(v[0], v[2], u, rb), edge_test(v[0], v[3], u, rb),
+                          edge_test(v[1], v[2], u, rb), edge_test(v[1], v[3], u, rb), edge_test(v[2], v[3], u, rb)};
+    int idx_out = ID_NONE;
+    double min_dist = INFTY;
+    face_from_edges<FO, 0>(v, e6, u, idx, r_i, u_i, idx_out, min_dist);
+    face_from_edges<FO, 1>(v, e6, u, idx, r_i, u_i, idx_out, min_dist);
+    face_from_edges<FO, 2>(v, e6, u, idx, r_i, u_i, idx_out, min_dist);
+    face_from_edges<FO, 3>(v, e6, u, idx, r_i, u_i, idx_out, min_dist);
+    next = idx_out == ID_NONE ? ID_NONE : E->adj[idx_out];
+    dist = min_dist;
+    return;
+  }
   const ElemRec& E = T.elems[elem];
   D3 v[4];
 #pragma unroll
