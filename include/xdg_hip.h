@@ -80,17 +80,18 @@ int xdg_hip_set_launch(xdg_hip_ctx* ctx, int block, int stack);
 int xdg_hip_get_launch(xdg_hip_ctx* ctx, int* block, int* stack);
 /* Traversal variant: wide != 0 uses the 4-wide tree (default), else the
  * binary LBVH; defer != 0 (default) traces far-origin mixed ray_fire rays in
- * a separate FP64 pass; persistent selects, per precision (bit 0 plain,
- * bit 1 mixed; default 3), ray_fire on a device-resident grid whose lanes
- * claim new rays from a counter as they finish.  Results are identical
- * either way. */
+ * a separate FP64 pass; persistent selects resident-grid kernels whose
+ * lanes claim new work from a counter as they finish (bit 0 plain
+ * ray_fire, bit 1 mixed ray_fire, bit 2 find_volume; default 3).  Results
+ * are identical either way. */
 int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent);
 /* Tuning options by name (no counterpart in the reference; results are
  * identical for every setting):
  *   "block"       workgroup size, multiple of 64 (default 64)
  *   "stack"       minimum LDS stack entries per lane (default 1; raised to
  *                 what each tree needs)
- *   "persistent"  resident-grid ray_fire, bit 0 plain, bit 1 mixed (3)
+ *   "persistent"  resident-grid kernels: bit 0 plain ray_fire, bit 1 mixed
+ *                 ray_fire, bit 2 find_volume (default 3)
  *   "ray_order"   ray_fire batches of >= 65536 rays: 0 caller's order,
  *                 1 sorted by (tree, direction octant, origin Morton code),
  *                 2 as 1 plus one coarse direction bit per axis (0)

@@ -397,6 +397,32 @@ def test_nested_find_volume_and_fire(ctx, torch_mod):
             assert np.array_equal(bits(t[sel]), bits(rt))
 
 
+@pytest.mark.parametrize("persist", [True, False])
+def test_find_volume_far_points_and_variants(ctx, torch_mod, persist):
+    """Points beyond the FP32 limit of the trees (deferred to the FP64 pass
+    by the persistent mixed kernel) mixed with near ones; persistent and
+    per-point kernels, both precisions, equal the oracle."""
+    torch = torch_mod
+    m = G.nested_spheres(2)
+    vols = list(m.volumes)
+    trees = [build_tree(ctx, m, v) for v in vols]
+    otrees = [O.Tree.from_model(m, v) for v in vols]
+    lim = min(ctx.tree_info(t)["o_limit"] for t in trees)
+    n = 30000
+    pts = P.box_points(41, P.STREAM_POS, n, [-19] * 3, [19] * 3)
+    dirs = P.iso_dirs(41, P.STREAM_DIR, n)
+    far = np.arange(n) % 5 == 0
+    pts[far] = pts[far] / 19.0 * (3.0 * lim)
+    ref = O.find_volume(otrees, vols, m.ipc, pts, dirs)
+    ctx.set_traversal(persistent=persist)
+    try:
+        for prec in (0, 1):
+            got = host(ctx.find_volume(trees, vols, m.ipc, dev(torch, pts), dev(torch, dirs), precision=prec))
+            assert np.array_equal(got, ref)
+    finally:
+        ctx.set_traversal()
+
+
 def _segments_setup(ctx, mesh):
     et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
     bm = mesh.boundary_model()

@@ -191,6 +191,20 @@ const RayOrderScratch& order_scratch(xdg_hip_ctx* c, int64_t n)
   return c->order_w;
 }
 
+// the mixed kernels' deferral list, grown to n entries on demand
+int32_t* fb_list_for(xdg_hip_ctx* c, int64_t n)
+{
+  if ((size_t)n > c->fb_cap) {
+    XDG_HIP_CHECK(hipDeviceSynchronize());
+    if (c->d_fb_list) XDG_HIP_CHECK(hipFree(c->d_fb_list));
+    c->d_fb_list = nullptr;
+    c->fb_cap = 0;
+    XDG_HIP_CHECK(hipMalloc(&c->d_fb_list, sizeof(int32_t) * (size_t)n));
+    c->fb_cap = (size_t)n;
+  }
+  return c->d_fb_list;
+}
+
 // rays below this count are traced in the caller's order
 constexpr int64_t ORDER_MIN_RAYS = 1 << 16;
 
@@ -251,7 +265,7 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   c->cfg.defer = env_int("XDG_HIP_DEFER", 1) != 0;
   c->cfg.wide = env_int("XDG_HIP_WIDE", 1) != 0;
   c->cfg.spec = env_int("XDG_HIP_SPEC", 1) != 0;
-  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 3) & 3;
+  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 3) & 7;
   c->cfg.order = env_int("XDG_HIP_ORDER", 0);
   c->cfg.refill = env_int("XDG_HIP_REFILL", 40);
   try {
@@ -311,7 +325,7 @@ int xdg_hip_set_traversal(xdg_hip_ctx* c, int wide, int defer, int persistent)
   return guarded(c, [&]() {
     c->cfg.wide = wide != 0;
     c->cfg.defer = defer != 0;
-    c->cfg.persistent = persistent & 3;
+    c->cfg.persistent = persistent & 7;
     return XDG_HIP_OK;
   });
 }
@@ -337,7 +351,7 @@ int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
 int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
 
 const OptionRef OPTIONS[] = {
-    {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 3, opt_persistent},
+    {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 7, opt_persistent},
     {"ray_order", 0, 2, opt_order},         {"speculative", 0, 1, opt_spec}, {"wide", 0, 1, opt_wide},
     {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
     {"refill", 1, 64, opt_refill},
@@ -657,15 +671,7 @@ int xdg_hip_ray_fire(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per_ray, 
     RayFireArgs a{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir, tmax, orientation,
                   excl, excl ? k : 0, t_out, surf_out, face_out, nullptr, nullptr};
     if (precision == XDG_HIP_MIXED && c->cfg.defer) {
-      if ((size_t)n > c->fb_cap) {
-        XDG_HIP_CHECK(hipDeviceSynchronize());
-        if (c->d_fb_list) XDG_HIP_CHECK(hipFree(c->d_fb_list));
-        c->d_fb_list = nullptr;
-        c->fb_cap = 0;
-        XDG_HIP_CHECK(hipMalloc(&c->d_fb_list, sizeof(int32_t) * (size_t)n));
-        c->fb_cap = (size_t)n;
-      }
-      a.fb_list = c->d_fb_list;
+      a.fb_list = fb_list_for(c, n);
       a.fb_count = c->d_fb_count;
     }
     a.next_ray = c->d_next;
@@ -790,6 +796,11 @@ int xdg_hip_find_volume(xdg_hip_ctx* c, int nvol, const int32_t* vol_trees, cons
     a.dflt[1] = 1 / std::sqrt(2.0);
     a.dflt[2] = 0.0;
     a.vol_out = vol_out;
+    a.next = c->d_next;
+    if (precision == XDG_HIP_MIXED) {
+      a.fb_list = fb_list_for(c, n);
+      a.fb_count = c->d_fb_count;
+    }
     launch_find_volume(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), s);
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;

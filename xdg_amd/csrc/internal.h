@@ -152,6 +152,11 @@ struct FindVolumeArgs {
   const double* dir;  // nullable -> default direction
   double dflt[3];
   int32_t* vol_out;
+  // persistent launch (nullable): point counter; mixed precision also queues
+  // points whose origin is beyond a tree's FP32 limit for an FP64 pass
+  unsigned long long* next;
+  int32_t* fb_list;
+  unsigned int* fb_count;
 };
 void launch_find_volume(const FindVolumeArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
 

@@ -239,11 +239,7 @@ __device__ __forceinline__ void ray_fire_pt_body(const RayFireArgs& a, int cap, 
   }
 }
 
-// The mixed kernel holds five waves per SIMD (at most 102 VGPRs); left to
-// itself the compiler settles on four, which costs more than it saves.
-__global__ __attribute__((amdgpu_waves_per_eu(5))) void k_ray_fire_pt_mixed(RayFireArgs a, int cap,
-                                                                           unsigned long long* next_ray,
-                                                                           int refill_min)
+__global__ void k_ray_fire_pt_mixed(RayFireArgs a, int cap, unsigned long long* next_ray, int refill_min)
 {
   ray_fire_pt_body<true>(a, cap, next_ray, refill_min);
 }
@@ -346,6 +342,162 @@ __global__ void k_find_volume(FindVolumeArgs a, int cap)
     }
   }
   a.vol_out[i] = found;
+}
+
+__device__ __forceinline__ void find_volume_point(const FindVolumeArgs& a, int64_t i, int32_t* stk, int cap)
+{
+  const double* d = a.dir ? a.dir + 3 * i : a.dflt;
+  RayD ray = make_ray(a.pts + 3 * i, d);
+  int32_t found = a.ipc;
+  for (int v = 0; v < a.nvol; ++v) {
+    const int32_t vid = a.vol_ids[v];
+    const int32_t tree = a.vol_trees[v];
+    if (vid == a.ipc || tree < 1 || tree > a.ntrees) continue;
+    HitState h{INFTY, INT32_MAX, ID_NONE, false};
+    trace<PIV, false, false, true>(a.table[tree], ray, ORIENT_ANY, nullptr, 0, stk, blockDim.x, cap, h);
+    if (h.surf != ID_NONE && h.inside) {
+      found = vid;
+      break;
+    }
+  }
+  a.vol_out[i] = found;
+}
+
+// FP64 pass over the points the mixed persistent kernel queued
+__global__ void k_find_volume_deferred(FindVolumeArgs a, int cap)
+{
+  extern __shared__ int32_t lds[];
+  const unsigned int cnt = *a.fb_count;
+  for (unsigned int j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x)
+    find_volume_point(a, a.fb_list[j], lds + threadIdx.x, cap);
+}
+
+// Persistent find_volume over the BVH4: a lane holds one point and runs its
+// point_in_volume traversals volume after volume (a new traversal on the
+// same ray costs only a reset); it claims a new point, lazily as in
+// ray_fire, once the answer is known.
+template <bool MIXED>
+__device__ __forceinline__ void find_volume_pt_body(const FindVolumeArgs& a, int cap, int refill_min)
+{
+  extern __shared__ int32_t lds[];
+  LdsStack S(lds + threadIdx.x, blockDim.x, cap);
+  const int lane = threadIdx.x & 63;
+  const int32_t DONE = CHILD_EMPTY;
+  int64_t pi = 0;
+  bool hasp = false, trav = false;
+  int v = 0;
+  TreeView T{nullptr, nullptr, nullptr, nullptr};
+  RayD ray;
+  BoxRayF bf{0, 0, 0, 0, 0, 0};
+  BoxRayD bd{0, 0, 0, 0, 0, 0};
+  PlaneOfs po{0, 0, 0, 0, 0, 0};
+  HitState h{INFTY, INT32_MAX, ID_NONE, false};
+  float tf = INFINITY;
+  int32_t node = DONE;
+  while (true) {
+    if (trav && node == DONE) {  // a point_in_volume traversal ended
+      trav = false;
+      if (h.surf != ID_NONE && h.inside) {
+        a.vol_out[pi] = a.vol_ids[v];
+        hasp = false;
+      } else {
+        ++v;
+      }
+    }
+    if (hasp && !trav) {  // next volume of the same point
+      while (v < a.nvol && (a.vol_ids[v] == a.ipc || a.vol_trees[v] < 1 || a.vol_trees[v] > a.ntrees)) ++v;
+      if (v >= a.nvol) {
+        a.vol_out[pi] = a.ipc;
+        hasp = false;
+      } else {
+        const SurfTreeDev& TT = a.table[a.vol_trees[v]];
+        if (MIXED && !mixed_safe(TT, ray)) {  // the whole point goes to the FP64 pass
+          const unsigned int j = atomicAdd(a.fb_count, 1u);
+          a.fb_list[j] = (int32_t)pi;
+          hasp = false;
+        } else {
+          T.nodes4_d = TT.nodes4_d;
+          T.nodes4_f = TT.nodes4_f;
+          T.tris = TT.tris;
+          T.normals = TT.normals;
+          h = HitState{INFTY, INT32_MAX, ID_NONE, false};
+          tf = INFINITY;
+          S.reset();
+          node = TT.n_prims > 0 ? TT.root4 : DONE;
+          trav = true;
+        }
+      }
+    }
+    const bool idle = !hasp;
+    const unsigned long long imask = __ballot(idle);
+    const int nact = __popcll(__ballot(1));
+    const bool go = __popcll(imask) >= (refill_min < nact ? refill_min : nact);
+    if (go && idle) {
+      const unsigned long long mask = imask;
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long b = 0;
+      if (lane == leader) b = atomicAdd(a.next, (unsigned long long)__popcll(mask));
+      b = __shfl(b, leader);
+      pi = (int64_t)(b + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
+      if (pi >= a.n) break;
+      const double* d = a.dir ? a.dir + 3 * pi : a.dflt;
+      ray = make_ray(a.pts + 3 * pi, d);
+      if (MIXED) {
+        const float ox = (float)ray.o.x, oy = (float)ray.o.y, oz = (float)ray.o.z;
+        bf.ix = safe_inv_f((float)ray.d.x);
+        bf.iy = safe_inv_f((float)ray.d.y);
+        bf.iz = safe_inv_f((float)ray.d.z);
+        bf.nx = -ox * bf.ix;
+        bf.ny = -oy * bf.iy;
+        bf.nz = -oz * bf.iz;
+      } else {
+        bd.ix = safe_inv_d(ray.d.x);
+        bd.iy = safe_inv_d(ray.d.y);
+        bd.iz = safe_inv_d(ray.d.z);
+        bd.nx = -ray.o.x * bd.ix;
+        bd.ny = -ray.o.y * bd.iy;
+        bd.nz = -ray.o.z * bd.iz;
+      }
+      po = MIXED ? plane_ofs_f(bf) : plane_ofs_d(bd);
+      v = 0;
+      hasp = true;
+      continue;  // the next iteration starts the first traversal
+    }
+    while (node >= 0) {
+      float te[4];
+      int32_t ch[4];
+      node4_test<MIXED>(T, node, bf, bd, po, tf, h.t, te, ch);
+      cswap(te[0], ch[0], te[1], ch[1]);
+      cswap(te[2], ch[2], te[3], ch[3]);
+      cswap(te[0], ch[0], te[2], ch[2]);
+      cswap(te[1], ch[1], te[3], ch[3]);
+      cswap(te[1], ch[1], te[2], ch[2]);
+      if (ch[0] != CHILD_EMPTY) {
+        if (ch[3] != CHILD_EMPTY) S.push(ch[3]);
+        if (ch[2] != CHILD_EMPTY) S.push(ch[2]);
+        if (ch[1] != CHILD_EMPTY) S.push(ch[1]);
+        node = ch[0];
+      } else {
+        node = S.empty() ? DONE : S.pop();
+      }
+    }
+    if (node != DONE) {
+      const int32_t first = leaf_first(node);
+      const int32_t cnt = leaf_count(node);
+      for (int32_t j = 0; j < cnt; ++j) tri_closest_hit<PIV>(T, first + j, ray, ORIENT_ANY, nullptr, 0, h);
+      tf = f32_up(h.t);
+      node = S.empty() ? DONE : S.pop();
+    }
+  }
+}
+
+__global__ void k_find_volume_pt_mixed(FindVolumeArgs a, int cap, int refill_min)
+{
+  find_volume_pt_body<true>(a, cap, refill_min);
+}
+__global__ void k_find_volume_pt_plain(FindVolumeArgs a, int cap, int refill_min)
+{
+  find_volume_pt_body<false>(a, cap, refill_min);
 }
 
 template <bool MIXED, bool WIDE>
@@ -551,6 +703,26 @@ void launch_point_in_volume(const PivArgs& a, bool mixed, const LaunchCfg& c, hi
 void launch_find_volume(const FindVolumeArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)
 {
   if (a.n <= 0) return;
+  // bit 2 of the persistent mask: the per-point kernel keeps a wave's lanes
+  // on the same tree volume after volume, which the resident grid loses
+  const bool persist = (c.persistent & 4) != 0;
+  if (persist && c.wide && a.next && (!mixed || (a.fb_list && a.fb_count))) {
+    const size_t sh = lds_bytes(c);
+    const int64_t need = (a.n + c.block - 1) / c.block;
+    const dim3 b(c.block);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, a.next);
+    if (mixed) {
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, a.fb_count);
+      const int64_t gp = std::min(need, resident_grid(k_find_volume_pt_mixed, c.block, sh));
+      hipLaunchKernelGGL(k_find_volume_pt_mixed, dim3((unsigned)gp), b, sh, s, a, c.stack, c.refill);
+      const unsigned gd = (unsigned)std::min<int64_t>(need, 1024);
+      hipLaunchKernelGGL(k_find_volume_deferred, dim3(gd), b, sh, s, a, c.stack);
+    } else {
+      const int64_t gp = std::min(need, resident_grid(k_find_volume_pt_plain, c.block, sh));
+      hipLaunchKernelGGL(k_find_volume_pt_plain, dim3((unsigned)gp), b, sh, s, a, c.stack, c.refill);
+    }
+    return;
+  }
   XDG_DISPATCH2(k_find_volume, mixed, c.wide, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
 }
 
