@@ -18,6 +18,8 @@
 #include "internal.h"
 #include "surface_traverse.h"
 
+#include <type_traits>
+
 namespace xdgh {
 
 using namespace xdgd;
@@ -113,71 +115,164 @@ __device__ __forceinline__ D3 face_vertex(const ElemTreeDev& T, const ElemRec& E
   return v[fo_at<FO>(T, f, j)];
 }
 
-// Edge value of a face edge (i -> j) from the canonical value of the vertex
-// pair (min, max): plucker_edge_test orients every edge by the
-// lexicographic order of its endpoints (plucker.h:35-62), so the reversed
-// edge gives exactly the negated value, except that a value snapped to
-// zero stays +0.0.
+// Edge value of a face edge (i -> j) from the value of the vertex pair
+// (min, max): plucker_edge_test orients every edge by the lexicographic
+// order of its endpoints (plucker.h:35-62), so the reversed edge gives
+// exactly the negated value, except that the reference keeps a value
+// snapped to zero at +0.0 where the negation here gives -0.0.  No result
+// depends on that sign: the face tests compare the values with 0 (equal
+// for both zeros), and a zero coordinate reaches the distance only through
+// products and sums whose zero results end in the (0 < t) ? t : 0 clamp.
 template <int I, int J>
 __device__ __forceinline__ double pair_edge(const double* e6)
 {
   constexpr int lo = I < J ? I : J, hi = I < J ? J : I;
   constexpr int k = lo == 0 ? hi - 1 : (lo == 1 ? hi + 1 : 5);  // (0,1)(0,2)(0,3)(1,2)(1,3)(2,3)
-  const double e = e6[k];
-  if (I < J) return e;
-  return e != 0.0 ? -e : 0.0;
+  return I < J ? e6[k] : -e6[k];
 }
 
+// Pluecker coordinates of face F (vertices A, B, C of the face order) from
+// the six canonical edge values, and whether the oriented test
+// (plucker.h:64-161 with orientation 1) gets past its sign and coplanar
+// rejections for that face.
 template <int FO, int F>
-__device__ __forceinline__ void face_from_edges(const D3* v, const double* e6, const D3& u, int idx, double r_i,
-                                                double u_i, int& idx_out, double& min_dist)
+__device__ __forceinline__ bool face_coords(const double* e6, double& c0, double& c1, double& c2)
 {
   constexpr int moab[12] = {0, 1, 3, 1, 2, 3, 2, 0, 3, 0, 2, 1};
   constexpr int mock[12] = {0, 1, 2, 0, 2, 3, 0, 3, 1, 1, 3, 2};
   constexpr int A = FO == FO_MOAB ? moab[3 * F] : mock[3 * F];
   constexpr int B = FO == FO_MOAB ? moab[3 * F + 1] : mock[3 * F + 1];
   constexpr int C = FO == FO_MOAB ? moab[3 * F + 2] : mock[3 * F + 2];
-  // plucker_ray_tri_intersect with orientation 1 (plucker.h:64-161): reject
-  // on a positive coordinate, then the distance from the barycentric point
-  const double c0 = pair_edge<A, B>(e6);
-  const double c1 = pair_edge<B, C>(e6);
-  const double c2 = pair_edge<C, A>(e6);
-  double df = INFTY;
-  double t;
-  if (!(1.0 * c0 > 0) && !(1.0 * c1 > 0) && !(1.0 * c2 > 0) &&
-      plucker_finish(c0, c1, c2, v[A], v[B], v[C], r_i, u_i, idx, INFTY, 0.0, t))
-    df = (0.0 < t) ? t : 0.0;
-  if (df < min_dist) {
-    min_dist = df;
-    idx_out = F;
-  }
+  c0 = pair_edge<A, B>(e6);
+  c1 = pair_edge<B, C>(e6);
+  c2 = pair_edge<C, A>(e6);
+  return !(1.0 * c0 > 0) && !(1.0 * c1 > 0) && !(1.0 * c2 > 0) && !(0.0 == c0 && 0.0 == c1 && 0.0 == c2);
 }
 
-// MeshManager::next_element (mesh_manager_interface.cpp:150-200).  With a
-// fixed face order the six edges of the tet are evaluated once each (every
-// face edge is one of them, up to its sign), instead of up to twelve times
-// with per-face early exits that make neighbouring lanes diverge.
+template <int FO, int F, int J>
+__device__ __forceinline__ double face_axis_vertex(const double* vi)
+{
+  constexpr int moab[12] = {0, 1, 3, 1, 2, 3, 2, 0, 3, 0, 2, 1};
+  constexpr int mock[12] = {0, 1, 2, 0, 2, 3, 0, 3, 1, 1, 3, 2};
+  return vi[FO == FO_MOAB ? moab[3 * F + J] : mock[3 * F + J]];
+}
+
+// x[f] for f in 0..3 by two levels of selects on the bits of f (a chain of
+// f == k tests is turned into a switch over scratch memory by the compiler)
+__device__ __forceinline__ double sel4(int f, double x0, double x1, double x2, double x3)
+{
+  const bool lo = (f & 1) != 0;
+  const double a = lo ? x1 : x0;
+  const double b = lo ? x3 : x2;
+  return (f & 2) != 0 ? b : a;
+}
+
+template <int FO, int J>
+__device__ __forceinline__ double pick_face_vertex(const double* vi, int f)
+{
+  return sel4(f, face_axis_vertex<FO, 0, J>(vi), face_axis_vertex<FO, 1, J>(vi), face_axis_vertex<FO, 2, J>(vi),
+              face_axis_vertex<FO, 3, J>(vi));
+}
+
+// One element record in registers: its vertices, neighbours and edge
+// orders (unused for FO_TABLE, whose step reads the record itself).
+struct ElemLoad {
+  D3 v[4];
+  int32_t adj[4];
+  uint32_t ef;
+  int32_t elem;
+};
+
+template <int FO>
+__device__ __forceinline__ void elem_load(const ElemTreeDev& T, int32_t elem, ElemLoad& L)
+{
+  L.elem = elem;
+  if (FO == FO_TABLE) return;
+  const auto E = as_global(T.elems + elem);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) L.v[j] = d3(E->v[3 * j], E->v[3 * j + 1], E->v[3 * j + 2]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) L.adj[j] = E->adj[j];
+  L.ef = E->edge_first;
+}
+
+template <int FO>
+__device__ __forceinline__ void next_element_dev(const ElemTreeDev& T, int32_t elem, const D3& r, const D3& u,
+                                                 int idx, double u_i, int32_t& next, double& dist);
+
+// MeshManager::next_element (mesh_manager_interface.cpp:150-200) on a
+// loaded record.  With a fixed face order the six edges of the tet are
+// evaluated once each (every face edge is one of them, up to its sign),
+// instead of up to twelve times with per-face early exits that make
+// neighbouring lanes diverge.
+template <int FO>
+__device__ __forceinline__ void elem_step(const ElemTreeDev& T, const ElemLoad& L, const D3& r, const D3& u, int idx,
+                                          double u_i, int32_t& next, double& dist)
+{
+  if (FO == FO_TABLE) {
+    next_element_dev<FO_TABLE>(T, L.elem, r, u, idx, u_i, next, dist);
+    return;
+  }
+  const D3* v = L.v;
+  const D3 rb = cross(u, r);
+  const double r_i = pick3(r.x, r.y, r.z, idx);
+  const uint32_t ef = L.ef;
+  const double e6[6] = {edge_test_f(v[0], v[1], ef & 1u, u, rb),  edge_test_f(v[0], v[2], ef & 2u, u, rb),
+                        edge_test_f(v[0], v[3], ef & 4u, u, rb),  edge_test_f(v[1], v[2], ef & 8u, u, rb),
+                        edge_test_f(v[1], v[3], ef & 16u, u, rb), edge_test_f(v[2], v[3], ef & 32u, u, rb)};
+  double c0[4], c1[4], c2[4];
+  unsigned pass = 0;
+  pass |= face_coords<FO, 0>(e6, c0[0], c1[0], c2[0]) ? 1u : 0u;
+  pass |= face_coords<FO, 1>(e6, c0[1], c1[1], c2[1]) ? 2u : 0u;
+  pass |= face_coords<FO, 2>(e6, c0[2], c1[2], c2[2]) ? 4u : 0u;
+  pass |= face_coords<FO, 3>(e6, c0[3], c1[3], c2[3]) ? 8u : 0u;
+  double vi[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) vi[j] = pick3(v[j].x, v[j].y, v[j].z, idx);
+  // A ray in general position passes the sign test on its exit face only,
+  // so the distance (two divisions) is formed once per face that passes,
+  // in face order; a failing face scores INFTY, which never replaces the
+  // running minimum, so the first strict minimum is the reference's.
+  int idx_out = ID_NONE;
+  double min_dist = INFTY;
+  while (pass) {
+    const int f = __builtin_ctz(pass);
+    pass &= pass - 1;
+    const double a0 = sel4(f, c0[0], c0[1], c0[2], c0[3]);
+    const double a1 = sel4(f, c1[0], c1[1], c1[2], c1[3]);
+    const double a2 = sel4(f, c2[0], c2[1], c2[2], c2[3]);
+    const double inverse_sum = 1.0 / (a0 + a1 + a2);
+    const double w0 = a0 * inverse_sum;
+    const double w1 = a1 * inverse_sum;
+    const double w2 = a2 * inverse_sum;
+    const double ix = w0 * pick_face_vertex<FO, 2>(vi, f) + w1 * pick_face_vertex<FO, 0>(vi, f) +
+                      w2 * pick_face_vertex<FO, 1>(vi, f);
+    double t = (ix - r_i) / u_i;
+    if (w2 < 0.0 || w0 < 0.0 || (w2 + w0) > 1.0) t = -1.0;
+    if (!(t < 0.0 || t > INFTY)) {
+      const double df = (0.0 < t) ? t : 0.0;
+      if (df < min_dist) {
+        min_dist = df;
+        idx_out = f;
+      }
+    }
+  }
+  // neighbour by select from the loaded record: no dependent load
+  const int lo = idx_out & 1;
+  const int32_t na = lo ? L.adj[1] : L.adj[0];
+  const int32_t nb = lo ? L.adj[3] : L.adj[2];
+  next = idx_out == ID_NONE ? ID_NONE : ((idx_out & 2) ? nb : na);
+  dist = min_dist;
+}
+
 template <int FO>
 __device__ __forceinline__ void next_element_dev(const ElemTreeDev& T, int32_t elem, const D3& r, const D3& u,
                                                  int idx, double u_i, int32_t& next, double& dist)
 {
   if (FO != FO_TABLE) {
-    const auto E = as_global(T.elems + elem);
-    D3 v[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = d3(E->v[3 * j], E->v[3 * j + 1], E->v[3 * j + 2]);
-    const D3 rb = cross(u, r);
-    const double r_i = pick3(r.x, r.y, r.z, idx);
-    const double e6[6] = {edge_test(v[0], v[1], u, rb), edge_test(v[0], v[2], u, rb), edge_test(v[0], v[3], u, rb),
-                          edge_test(v[1], v[2], u, rb), edge_test(v[1], v[3], u, rb), edge_test(v[2], v[3], u, rb)};
-    int idx_out = ID_NONE;
-    double min_dist = INFTY;
-    face_from_edges<FO, 0>(v, e6, u, idx, r_i, u_i, idx_out, min_dist);
-    face_from_edges<FO, 1>(v, e6, u, idx, r_i, u_i, idx_out, min_dist);
-    face_from_edges<FO, 2>(v, e6, u, idx, r_i, u_i, idx_out, min_dist);
-    face_from_edges<FO, 3>(v, e6, u, idx, r_i, u_i, idx_out, min_dist);
-    next = idx_out == ID_NONE ? ID_NONE : E->adj[idx_out];
-    dist = min_dist;
+    ElemLoad L;
+    elem_load<FO>(T, elem, L);
+    elem_step<FO>(T, L, r, u, idx, u_i, next, dist);
     return;
   }
   const ElemRec& E = T.elems[elem];
@@ -247,22 +342,30 @@ struct Sink {
   unsigned long long* tally;
 };
 
+// What a walk writes besides counts and hashes.  OUT_TALLY: per-element
+// tallies, and the tally atomic is unconditional, so the compiler knows it
+// follows the walk's prefetch of the next record and waits for that record
+// without waiting for the atomic.  OUT_SEQ: element sequences, plus
+// tallies when s.tally is non-null.
+enum WalkOut { OUT_NONE = 0, OUT_TALLY = 1, OUT_SEQ = 2 };
+
+template <int OUT>
 __device__ __forceinline__ void sink_push(Sink& s, int32_t e, double len)
 {
-  if (s.count < s.cap) {
+  if (OUT == OUT_SEQ && s.count < s.cap) {
     s.se[s.count] = e;
     s.sl[s.count] = len;
   }
   s.count++;
   s.hash = mix64(s.hash ^ (unsigned long long)(uint32_t)e);
   s.hash = mix64(s.hash ^ (unsigned long long)__double_as_longlong(len));
-  if (s.tally) {
+  if (OUT == OUT_TALLY || (OUT == OUT_SEQ && s.tally)) {
     long long q = __double2ll_rn(len * 1099511627776.0);  // len * 2^40, exact scaling
     atomicAdd(s.tally + e, (unsigned long long)q);
   }
 }
 
-template <int FO>
+template <int FO, int OUT>
 __global__ void k_segments(SegmentsArgs a, int cap)
 {
   extern __shared__ int32_t lds[];
@@ -272,7 +375,7 @@ __global__ void k_segments(SegmentsArgs a, int cap)
   s.count = 0;
   s.hash = HASH_SEED;
   s.tally = a.tally_fx;
-  if (a.seq_off) {
+  if (OUT == OUT_SEQ) {
     s.se = a.seq_elem + a.seq_off[i];
     s.sl = a.seq_len + a.seq_off[i];
     s.cap = a.seq_off[i + 1] - a.seq_off[i];
@@ -286,7 +389,10 @@ __global__ void k_segments(SegmentsArgs a, int cap)
     a.hash[i] = s.hash;
     return;
   }
-  const ElemTreeDev& T = a.etable[a.tree];
+  // a copy read before any store, so it stays in scalar registers; the loop
+  // below writes memory, after which fields read through a reference would
+  // be fetched again (and waited for) on every step
+  const ElemTreeDev T = a.etable[a.tree];
   int32_t* stk = lds + threadIdx.x;
   const int stride = blockDim.x;
   // xdg.cpp:150-153
@@ -338,21 +444,34 @@ __global__ void k_segments(SegmentsArgs a, int cap)
     D3 rr = r;
     double dist_w = distance;
     double sum = 0.0;
+    ElemLoad L;
+    elem_load<FO>(T, cur, L);  // cur is an element here
+    // Wait for the record here, so that at the loop head the only loads
+    // in flight are the ones issued by the previous step, which the tally
+    // atomic follows: the head then waits for the loads alone (one
+    // operation left outstanding) instead of for everything.
+    __builtin_amdgcn_s_waitcnt(0);
     while (dist_w > 0) {
       int32_t next;
       double ex;
-      next_element_dev<FO>(T, cur, rr, u, idx, u_i, next, ex);
+      elem_step<FO>(T, L, rr, u, idx, u_i, next, ex);
       const double len = (dist_w < ex) ? dist_w : ex;
       dist_w -= len;
-      sink_push(s, cur, len);
+      const int32_t here = cur;
       last = cur;
       sum = sum + len;
       rr.x = rr.x + len * u.x;
       rr.y = rr.y + len * u.y;
       rr.z = rr.z + len * u.z;
       cur = next;
-      if (cur == ID_NONE) break;
-      if (++steps >= MAX_WALK_STEPS) break;
+      const bool stop = cur == ID_NONE || ++steps >= MAX_WALK_STEPS;
+      // the next record is requested before this step's tally and sequence
+      // writes, so waiting for it does not wait for them; a lane that
+      // stops requests its current record again (unused, always in range)
+      // so that the loads are not under a branch
+      elem_load<FO>(T, (!stop && dist_w > 0) ? cur : here, L);
+      sink_push<OUT>(s, here, len);
+      if (stop) break;
     }
     r.x = r.x + sum * u.x;
     r.y = r.y + sum * u.y;
@@ -362,6 +481,201 @@ __global__ void k_segments(SegmentsArgs a, int cap)
   a.count[i] = s.count;
   a.hash[i] = s.hash;
   if (capped && a.status) atomicOr(a.status, 1u);
+}
+
+// Persistent tally_segments: the same walk as k_segments on a resident grid,
+// with segments claimed lazily from a counter.  A wave walks in a tight loop
+// until `refill_min` of its lanes (or all that remain) have stopped walking;
+// then lanes whose segment is done claim new ones and lanes that left the
+// walk mid-segment run the outer loop of XDG::segments (find_element, or the
+// implicit-complement re-entry) before the wave walks on.  Lanes that finish
+// short segments thus take new work instead of idling behind the wave's
+// longest segment.
+template <int FO, int OUT>
+__global__ void k_segments_pt(SegmentsArgs a, int cap, unsigned long long* next_seg, int refill_min)
+{
+  extern __shared__ int32_t lds[];
+  int32_t* stk = lds + threadIdx.x;
+  const int stride = blockDim.x;
+  const int lane = threadIdx.x & 63;
+  enum { IDLE = 0, LOCATE = 1, WALK = 2 };
+  int phase = IDLE;
+  int64_t i = 0;
+  Sink s;
+  s.count = 0;
+  s.hash = HASH_SEED;
+  s.tally = a.tally_fx;
+  s.se = nullptr;
+  s.sl = nullptr;
+  s.cap = 0;
+  D3 r = d3(0, 0, 0), u = d3(0, 0, 0), rr = d3(0, 0, 0);
+  double distance = 0.0, dist_w = 0.0, sum = 0.0, u_i = 0.0;
+  int idx = 0, steps = 0, loops = 0;
+  int32_t last = ID_NONE, cur = ID_NONE;
+  ElemLoad L;
+  bool capped = false, fresh = false;
+  const bool tree_ok = a.tree >= 1 && a.tree <= a.nelem_trees;
+  // read before any store, so it stays in scalar registers (entry 0 of the
+  // table is an empty tree)
+  const ElemTreeDev T = a.etable[tree_ok ? a.tree : 0];
+  while (true) {
+    {
+      const bool idle = phase == IDLE;
+      const unsigned long long imask = __ballot(idle);
+      const int nact = __popcll(__ballot(1));
+      if (idle && __popcll(imask) >= (refill_min < nact ? refill_min : nact)) {
+        const int leader = __ffsll((long long)imask) - 1;
+        unsigned long long b = 0;
+        if (lane == leader) b = atomicAdd(next_seg, (unsigned long long)__popcll(imask));
+        b = __shfl(b, leader);
+        i = (int64_t)(b + (unsigned long long)__popcll(imask & ((1ull << lane) - 1ull)));
+        if (i >= a.n) break;
+        s.count = 0;
+        s.hash = HASH_SEED;
+        if (OUT == OUT_SEQ) {
+          s.se = a.seq_elem + a.seq_off[i];
+          s.sl = a.seq_len + a.seq_off[i];
+          s.cap = a.seq_off[i + 1] - a.seq_off[i];
+        }
+        if (tree_ok) {
+          // xdg.cpp:150-153
+          r = d3(a.p0[3 * i], a.p0[3 * i + 1], a.p0[3 * i + 2]);
+          u = sub(d3(a.p1[3 * i], a.p1[3 * i + 1], a.p1[3 * i + 2]), r);
+          distance = sqrt(u.x * u.x + u.y * u.y + u.z * u.z);
+          u.x = u.x / distance;
+          u.y = u.y / distance;
+          u.z = u.z / distance;
+          idx = major_axis(u);
+          u_i = pick3(u.x, u.y, u.z, idx);
+          last = ID_NONE;
+          steps = 0;
+          loops = 0;
+          capped = false;
+          // the first find_element of the segment, at p0, came from the
+          // pre-pass, so refills stay cheap
+          cur = a.start_elem[i];
+          fresh = true;
+          phase = LOCATE;
+        } else {
+          a.count[i] = 0;
+          a.hash[i] = s.hash;
+        }
+      }
+    }
+    if (phase == LOCATE) {
+      // one pass of the outer loop of XDG::segments (xdg.cpp:155-193)
+      bool finish = !(distance > 0);
+      if (!finish && (++loops > MAX_SEG_LOOPS || steps >= MAX_WALK_STEPS)) {
+        capped = true;
+        finish = true;
+      }
+      if (!finish) {
+        if (!fresh) cur = find_element_dev(T, r, stk, stride, cap);
+        fresh = false;
+        if (s.count > 0 && cur == last) cur = ID_NONE;
+        if (cur == ID_NONE) {
+          // xdg.cpp:164-185: fire against the implicit complement
+          if (T.ipc_tree < 1 || T.ipc_tree > a.nsurf_trees) {
+            finish = true;
+          } else {
+            RayD ray;
+            ray.o = d3(r.x + TINY_BIT * u.x, r.y + TINY_BIT * u.y, r.z + TINY_BIT * u.z);
+            ray.d = u;
+            ray.rb = cross(ray.d, ray.o);
+            ray.idx = idx;
+            ray.oi = pick3(ray.o.x, ray.o.y, ray.o.z, idx);
+            ray.di = u_i;
+            HitState h{INFTY, INT32_MAX, ID_NONE, false};
+            trace<RAYFIRE, false>(a.stable[T.ipc_tree], ray, ORIENT_EXITING, nullptr, 0, stk, stride, cap, h);
+            if (h.surf == ID_NONE || h.t > distance) {
+              finish = true;
+            } else {
+              const double hit_dist = h.t + TINY_BIT;
+              r.x = r.x + hit_dist * u.x;
+              r.y = r.y + hit_dist * u.y;
+              r.z = r.z + hit_dist * u.z;
+              distance -= hit_dist;
+              const int32_t adjacent = T.bface_elem ? T.bface_elem[h.face] : ID_NONE;
+              if (adjacent == ID_NONE)
+                finish = true;
+              else
+                cur = adjacent;
+            }
+          }
+        }
+        // walk_elements does not step when the re-entry used up the segment
+        if (!finish && !(distance > 0)) finish = true;
+      }
+      if (finish) {
+        a.count[i] = s.count;
+        a.hash[i] = s.hash;
+        if (capped && a.status) atomicOr(a.status, 1u);
+        phase = IDLE;
+      } else {
+        // walk_elements (mesh_manager_interface.cpp:107-136) from cur
+        rr = r;
+        dist_w = distance;
+        sum = 0.0;
+        elem_load<FO>(T, cur, L);
+        phase = WALK;
+      }
+    }
+    const int nact = __popcll(__ballot(1));
+    const int need = refill_min < nact ? refill_min : nact;
+    if (phase == WALK) {
+      __builtin_amdgcn_s_waitcnt(0);  // as in k_segments: the loop head then waits for its own loads only
+      while (true) {
+        int32_t next;
+        double ex;
+        elem_step<FO>(T, L, rr, u, idx, u_i, next, ex);
+        const double len = (dist_w < ex) ? dist_w : ex;
+        dist_w -= len;
+        const int32_t here = cur;
+        last = cur;
+        sum = sum + len;
+        rr.x = rr.x + len * u.x;
+        rr.y = rr.y + len * u.y;
+        rr.z = rr.z + len * u.z;
+        cur = next;
+        const bool stop = cur == ID_NONE || ++steps >= MAX_WALK_STEPS || !(dist_w > 0);
+        // next record requested before this step's writes (see k_segments)
+        elem_load<FO>(T, stop ? here : cur, L);
+        sink_push<OUT>(s, here, len);
+        if (stop) {
+          r.x = r.x + sum * u.x;
+          r.y = r.y + sum * u.y;
+          r.z = r.z + sum * u.z;
+          distance -= sum;
+          if (distance > 0) {
+            phase = LOCATE;
+          } else {
+            a.count[i] = s.count;
+            a.hash[i] = s.hash;
+            phase = IDLE;
+          }
+        }
+        const int walking = __popcll(__ballot(phase == WALK));
+        if (phase != WALK || nact - walking >= need) break;
+      }
+    }
+  }
+}
+
+__global__ void k_zero_seg(unsigned long long* p) { *p = 0ull; }
+
+// find_element at every segment's start point (xdg.cpp:150-157: the first
+// pass of the outer loop locates r = p0), one lane per segment
+__global__ void k_segment_starts(SegmentsArgs a, int cap)
+{
+  extern __shared__ int32_t lds[];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  int32_t e = ID_NONE;
+  if (a.tree >= 1 && a.tree <= a.nelem_trees) {
+    const D3 p = d3(a.p0[3 * i], a.p0[3 * i + 1], a.p0[3 * i + 2]);
+    e = find_element_dev(a.etable[a.tree], p, lds + threadIdx.x, blockDim.x, cap);
+  }
+  a.start_elem[i] = e;
 }
 
 __global__ void k_elem_boxes(const ElemRec* __restrict__ elems, int64_t n, double* box6)
@@ -401,15 +715,44 @@ void launch_next_element(const NextElementArgs& a, hipStream_t s)
   hipLaunchKernelGGL(k_next_element, grid1(a.n, 256), dim3(256), 0, s, a);
 }
 
+template <int FO, int OUT>
+void launch_segments_t(const SegmentsArgs& a, const LaunchCfg& c, hipStream_t s)
+{
+  if ((c.persistent & 8) && a.next && a.start_elem) {
+    const size_t sh = lds_bytes(c);
+    const int64_t need = (a.n + c.block - 1) / c.block;
+    int dev = 0, cu = 256, nb = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_segments_pt<FO, OUT>, c.block, sh);
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)std::max(nb, 1) * cu));
+    hipLaunchKernelGGL(k_zero_seg, dim3(1), dim3(1), 0, s, a.next);
+    hipLaunchKernelGGL(k_segment_starts, grid1(a.n, c.block), dim3(c.block), sh, s, a, c.stack);
+    hipLaunchKernelGGL((k_segments_pt<FO, OUT>), dim3(g), dim3(c.block), sh, s, a, c.stack, a.next, c.seg_refill);
+    return;
+  }
+  hipLaunchKernelGGL((k_segments<FO, OUT>), grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+}
+
 void launch_segments(const SegmentsArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s)
 {
   if (a.n <= 0) return;
+  const int out = a.seq_off ? OUT_SEQ : (a.tally_fx ? OUT_TALLY : OUT_NONE);
+  auto go = [&](auto fo) {
+    constexpr int FO = decltype(fo)::value;
+    if (out == OUT_SEQ)
+      launch_segments_t<FO, OUT_SEQ>(a, c, s);
+    else if (out == OUT_TALLY)
+      launch_segments_t<FO, OUT_TALLY>(a, c, s);
+    else
+      launch_segments_t<FO, OUT_NONE>(a, c, s);
+  };
   if (face_order_kind == FO_MOAB)
-    hipLaunchKernelGGL(k_segments<FO_MOAB>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+    go(std::integral_constant<int, FO_MOAB>{});
   else if (face_order_kind == FO_MOCK)
-    hipLaunchKernelGGL(k_segments<FO_MOCK>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+    go(std::integral_constant<int, FO_MOCK>{});
   else
-    hipLaunchKernelGGL(k_segments<FO_TABLE>, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+    go(std::integral_constant<int, FO_TABLE>{});
 }
 
 int face_order_kind(const int32_t* fo)

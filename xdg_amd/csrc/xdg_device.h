@@ -86,21 +86,28 @@ __device__ __forceinline__ bool first(const D3& a, const D3& b)
   return lx || (!gx && (ly || (!gy && lz)));
 }
 
-// plucker_edge_test plucker.h:45-62.  The canonical endpoint is chosen by
-// selects and the one expression is evaluated for both orientations, so a
-// wave whose lanes disagree on the order does not run both branches; the
-// operations and their order are those of the reference in either case.
-__device__ __forceinline__ double edge_test(const D3& va, const D3& vb, const D3& ray, const D3& rn)
+// plucker_edge_test plucker.h:45-62.  The reference evaluates
+//   f:  dot(ray, cross(b - a, a)) + dot(rn, b - a)
+//   !f: -(dot(ray, cross(a - b, b)) + dot(rn, a - b))
+// with f = first(a, b).  a - b is exactly -(b - a) and negation passes
+// exactly through every product and sum of the second form, so it equals
+// dot(ray, cross(b - a, b)) + dot(rn, b - a) bit for bit (up to the sign of
+// a zero, which the tolerance snap turns into +0.0 either way).  One
+// expression with the endpoint chosen by select therefore serves both
+// orders: a wave whose lanes disagree on the order runs no second branch.
+// edge_test_f takes first(a, b) precomputed (element records carry it).
+__device__ __forceinline__ double edge_test_f(const D3& va, const D3& vb, bool f, const D3& ray, const D3& rn)
 {
-  const bool f = first(va, vb);
   const D3 p = f ? va : vb;
-  const D3 q = f ? vb : va;
-  const D3 e = sub(q, p);
+  const D3 e = sub(vb, va);
   const D3 en = cross(e, p);
   double pip = dot(ray, en) + dot(rn, e);
-  pip = f ? pip : -pip;
   if (PLUCKER_ZERO_TOL > fabs(pip)) pip = 0.0;
   return pip;
+}
+__device__ __forceinline__ double edge_test(const D3& va, const D3& vb, const D3& ray, const D3& rn)
+{
+  return edge_test_f(va, vb, first(va, vb), ray, rn);
 }
 
 // largest-|dir| axis, first wins ties (plucker.h:137-144)
@@ -116,9 +123,9 @@ __device__ __forceinline__ int major_axis(const D3& d)
 
 // Distance from the three Pluecker coordinates (plucker.h:123-160).
 // Returns false for the coplanar / barycentric / range rejections.  o_i and
-// d_i are the ray's origin and direction on the major axis idx.  The three
-// axis candidates are formed and one is selected, so no value is indexed
-// dynamically (which would force the vertices into scratch memory).
+// d_i are the ray's origin and direction on the major axis idx.  Values on
+// that axis are chosen by select, never by a dynamic index (which would
+// force the vertices into scratch memory).
 __device__ __forceinline__ bool plucker_finish(double c0, double c1, double c2, const D3& v0, const D3& v1,
                                                const D3& v2, double o_i, double d_i, int idx, double tmax,
                                                double tmin, double& t)
@@ -128,10 +135,10 @@ __device__ __forceinline__ bool plucker_finish(double c0, double c1, double c2, 
   const double w0 = c0 * inverse_sum;
   const double w1 = c1 * inverse_sum;
   const double w2 = c2 * inverse_sum;
-  const double ix_x = w0 * v2.x + w1 * v0.x + w2 * v1.x;
-  const double ix_y = w0 * v2.y + w1 * v0.y + w2 * v1.y;
-  const double ix_z = w0 * v2.z + w1 * v0.z + w2 * v1.z;
-  const double ix = idx == 0 ? ix_x : (idx == 1 ? ix_y : ix_z);
+  // only the major-axis component of the intersection point is used, so
+  // the vertices' components on that axis are selected first
+  const double ix = w0 * pick3(v2.x, v2.y, v2.z, idx) + w1 * pick3(v0.x, v0.y, v0.z, idx) +
+                    w2 * pick3(v1.x, v1.y, v1.z, idx);
   double dist = (ix - o_i) / d_i;
   if (w2 < 0.0 || w0 < 0.0 || (w2 + w0) > 1.0) dist = -1.0;
   if (dist < tmin || dist > tmax) return false;
@@ -432,10 +439,13 @@ struct alignas(16) TriRec {
 
 // Tetrahedron record indexed by element index, 128 B.
 struct alignas(16) ElemRec {
-  double v[12];     // vertices in connectivity order
-  int32_t adj[4];   // neighbour across face i, -1 on the boundary
-  int32_t conn[4];  // global vertex ids
+  double v[12];         // vertices in connectivity order
+  int32_t adj[4];       // neighbour across face i, -1 on the boundary
+  uint32_t edge_first;  // bit k: first(v_a, v_b) for the k-th vertex pair (a, b) of
+                        // (0,1) (0,2) (0,3) (1,2) (1,3) (2,3) (plucker.h:35-43)
+  int32_t conn[3];      // global ids of vertices 0..2 (diagnostics only)
 };
+static_assert(sizeof(ElemRec) == 128, "one element record per 128-byte cache line");
 
 __host__ __device__ __forceinline__ int32_t leaf_code(int32_t first, int32_t count)
 {
