@@ -477,50 +477,6 @@ def test_mock_segments_sequences(ctx, torch_mod, golden_dir):
         assert np.array_equal(bits(gl[a:b]), bits(sl[a:b]))
 
 
-@pytest.mark.parametrize("refill", [1, 8, 64])
-def test_persistent_segments(ctx, torch_mod, golden_dir, refill):
-    """Resident-grid tally_segments (lazy per-lane segment refill) gives the
-    same counts, hashes, tallies and sequences as one segment per lane."""
-    torch = torch_mod
-    ctx.set_traversal(persistent=8 | 3)
-    ctx.set_option("segment_refill", refill)
-    try:
-        mesh = G.kuhn_cube(12, 0.2, seed=7)
-        et = _segments_setup(ctx, mesh)
-        om = O.TetMesh(mesh)
-        n = 9000
-        p0 = P.box_points(5, P.STREAM_POS, n, [-0.2] * 3, [1.2] * 3)
-        p1 = P.box_points(5, P.STREAM_DIR, n, [-0.2] * 3, [1.2] * 3)
-        p1[::97] = p0[::97]  # zero-length segments
-        tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device="cuda")
-        cnt, hsh = ctx.tally_segments(et, dev(torch, p0), dev(torch, p1), tally=tally)
-        rc, rh, rt = om.segments(p0, p1, tally=True)
-        assert np.array_equal(host(cnt), rc)
-        assert np.array_equal(host(hsh).view(np.uint64), rh)
-        assert np.array_equal(host(tally), rt)
-        assert ctx.status() == 0
-        mm = G.mock_tetmesh()
-        et = _segments_setup(ctx, mm)
-        om = O.TetMesh(mm)
-        tracks = np.load(os.path.join(golden_dir, "golden_mock_tracks.npz"))["tracks"]
-        q0, q1 = tracks[:, :3], tracks[:, 3:]
-        rc, rh, (off, se, sl) = om.segments(q0, q1, sequences=np.full(len(q0), 64, np.int64))
-        seq_e = torch.zeros(int(off[-1]), dtype=torch.int32, device="cuda")
-        seq_l = torch.zeros(int(off[-1]), dtype=torch.float64, device="cuda")
-        cnt, hsh = ctx.tally_segments(et, dev(torch, q0), dev(torch, q1), seq_off=dev(torch, off),
-                                      seq_elem=seq_e, seq_len=seq_l)
-        assert np.array_equal(host(cnt), rc)
-        assert np.array_equal(host(hsh).view(np.uint64), rh)
-        ge, gl = host(seq_e), host(seq_l)
-        for i in range(len(q0)):
-            a, b = off[i], off[i] + rc[i]
-            assert np.array_equal(ge[a:b], se[a:b])
-            assert np.array_equal(bits(gl[a:b]), bits(sl[a:b]))
-    finally:
-        ctx.set_traversal()
-        ctx.set_option("segment_refill", 8)  # the default
-
-
 def test_fresh_context_status_is_clear(torch_mod):
     """The sticky status word starts at zero on every new context, also when
     its memory is recycled from contexts that were destroyed."""

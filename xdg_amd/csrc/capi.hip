@@ -92,9 +92,7 @@ struct xdg_hip_ctx {
   int32_t* d_fb_list = nullptr;  // mixed ray_fire: rays deferred to the FP64 pass
   size_t fb_cap = 0;
   unsigned int* d_fb_count = nullptr;
-  unsigned long long* d_next = nullptr;  // persistent kernels' work counters: [0] ray_fire, [1] find_volume, [2] segments
-  int32_t* d_seg_start = nullptr;  // persistent tally_segments: element of each segment's start point
-  size_t seg_start_cap = 0;
+  unsigned long long* d_next = nullptr;  // persistent kernels' work counters: [0] ray_fire, [1] find_volume
   void* d_order = nullptr;  // ray ordering: keys, indices and sort storage
   int64_t order_cap = 0;
   RayOrderScratch order_w{};
@@ -207,20 +205,6 @@ int32_t* fb_list_for(xdg_hip_ctx* c, int64_t n)
   return c->d_fb_list;
 }
 
-// the persistent segment walk's start-element buffer, grown to n entries
-int32_t* seg_start_for(xdg_hip_ctx* c, int64_t n)
-{
-  if ((size_t)n > c->seg_start_cap) {
-    XDG_HIP_CHECK(hipDeviceSynchronize());
-    if (c->d_seg_start) XDG_HIP_CHECK(hipFree(c->d_seg_start));
-    c->d_seg_start = nullptr;
-    c->seg_start_cap = 0;
-    XDG_HIP_CHECK(hipMalloc(&c->d_seg_start, sizeof(int32_t) * (size_t)n));
-    c->seg_start_cap = (size_t)n;
-  }
-  return c->d_seg_start;
-}
-
 // rays below this count are traced in the caller's order
 constexpr int64_t ORDER_MIN_RAYS = 1 << 16;
 
@@ -266,7 +250,7 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->d_status, sizeof(uint32_t) * 4) != hipSuccess ||
       hipMalloc(&c->d_fb_count, sizeof(unsigned int) * 4) != hipSuccess ||
-      hipMalloc(&c->d_next, sizeof(unsigned long long) * 4) != hipSuccess ||
+      hipMalloc(&c->d_next, sizeof(unsigned long long) * 2) != hipSuccess ||
       hipMemset(c->d_status, 0, sizeof(uint32_t) * 4) != hipSuccess) {  // status flags start clear
     if (c->d_next) (void)hipFree(c->d_next);
     if (c->d_fb_count) (void)hipFree(c->d_fb_count);
@@ -281,10 +265,9 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   c->cfg.defer = env_int("XDG_HIP_DEFER", 1) != 0;
   c->cfg.wide = env_int("XDG_HIP_WIDE", 1) != 0;
   c->cfg.spec = env_int("XDG_HIP_SPEC", 1) != 0;
-  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 3) & 15;
+  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 3) & 7;
   c->cfg.order = env_int("XDG_HIP_ORDER", 0);
   c->cfg.refill = env_int("XDG_HIP_REFILL", 40);
-  c->cfg.seg_refill = env_int("XDG_HIP_SEG_REFILL", 8);
   try {
     sync_tables(c);
   } catch (...) {
@@ -318,7 +301,6 @@ int xdg_hip_destroy(xdg_hip_ctx* c)
     if (c->d_fb_count) (void)hipFree(c->d_fb_count);
     if (c->d_next) (void)hipFree(c->d_next);
     if (c->d_order) (void)hipFree(c->d_order);
-    if (c->d_seg_start) (void)hipFree(c->d_seg_start);
     if (c->stream) (void)hipStreamDestroy(c->stream);
   }
   delete c;
@@ -343,7 +325,7 @@ int xdg_hip_set_traversal(xdg_hip_ctx* c, int wide, int defer, int persistent)
   return guarded(c, [&]() {
     c->cfg.wide = wide != 0;
     c->cfg.defer = defer != 0;
-    c->cfg.persistent = persistent & 15;
+    c->cfg.persistent = persistent & 7;
     return XDG_HIP_OK;
   });
 }
@@ -367,13 +349,12 @@ int* opt_wide(xdg_hip_ctx* c) { return &c->cfg.wide; }
 int* opt_defer(xdg_hip_ctx* c) { return &c->cfg.defer; }
 int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
 int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
-int* opt_seg_refill(xdg_hip_ctx* c) { return &c->cfg.seg_refill; }
 
 const OptionRef OPTIONS[] = {
-    {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 15, opt_persistent},
+    {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 7, opt_persistent},
     {"ray_order", 0, 2, opt_order},         {"speculative", 0, 1, opt_spec}, {"wide", 0, 1, opt_wide},
     {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
-    {"refill", 1, 64, opt_refill},          {"segment_refill", 1, 64, opt_seg_refill},
+    {"refill", 1, 64, opt_refill},
 };
 
 const OptionRef& find_option(const char* name)
@@ -924,8 +905,6 @@ int xdg_hip_tally_segments(xdg_hip_ctx* c, int32_t element_tree, int64_t n, cons
     a.seq_elem = seq_elem;
     a.seq_len = seq_len;
     a.status = c->d_status;
-    a.next = c->d_next + 2;
-    if (c->cfg.persistent & 8) a.start_elem = seg_start_for(c, n);
     launch_segments(a, face_order_kind(t.dev.face_order), cfg_for(c, depth), pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;

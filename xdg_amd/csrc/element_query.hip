@@ -483,201 +483,6 @@ __global__ void k_segments(SegmentsArgs a, int cap)
   if (capped && a.status) atomicOr(a.status, 1u);
 }
 
-// Persistent tally_segments: the same walk as k_segments on a resident grid,
-// with segments claimed lazily from a counter.  A wave walks in a tight loop
-// until `refill_min` of its lanes (or all that remain) have stopped walking;
-// then lanes whose segment is done claim new ones and lanes that left the
-// walk mid-segment run the outer loop of XDG::segments (find_element, or the
-// implicit-complement re-entry) before the wave walks on.  Lanes that finish
-// short segments thus take new work instead of idling behind the wave's
-// longest segment.
-template <int FO, int OUT>
-__global__ void k_segments_pt(SegmentsArgs a, int cap, unsigned long long* next_seg, int refill_min)
-{
-  extern __shared__ int32_t lds[];
-  int32_t* stk = lds + threadIdx.x;
-  const int stride = blockDim.x;
-  const int lane = threadIdx.x & 63;
-  enum { IDLE = 0, LOCATE = 1, WALK = 2 };
-  int phase = IDLE;
-  int64_t i = 0;
-  Sink s;
-  s.count = 0;
-  s.hash = HASH_SEED;
-  s.tally = a.tally_fx;
-  s.se = nullptr;
-  s.sl = nullptr;
-  s.cap = 0;
-  D3 r = d3(0, 0, 0), u = d3(0, 0, 0), rr = d3(0, 0, 0);
-  double distance = 0.0, dist_w = 0.0, sum = 0.0, u_i = 0.0;
-  int idx = 0, steps = 0, loops = 0;
-  int32_t last = ID_NONE, cur = ID_NONE;
-  ElemLoad L;
-  bool capped = false, fresh = false;
-  const bool tree_ok = a.tree >= 1 && a.tree <= a.nelem_trees;
-  // read before any store, so it stays in scalar registers (entry 0 of the
-  // table is an empty tree)
-  const ElemTreeDev T = a.etable[tree_ok ? a.tree : 0];
-  while (true) {
-    {
-      const bool idle = phase == IDLE;
-      const unsigned long long imask = __ballot(idle);
-      const int nact = __popcll(__ballot(1));
-      if (idle && __popcll(imask) >= (refill_min < nact ? refill_min : nact)) {
-        const int leader = __ffsll((long long)imask) - 1;
-        unsigned long long b = 0;
-        if (lane == leader) b = atomicAdd(next_seg, (unsigned long long)__popcll(imask));
-        b = __shfl(b, leader);
-        i = (int64_t)(b + (unsigned long long)__popcll(imask & ((1ull << lane) - 1ull)));
-        if (i >= a.n) break;
-        s.count = 0;
-        s.hash = HASH_SEED;
-        if (OUT == OUT_SEQ) {
-          s.se = a.seq_elem + a.seq_off[i];
-          s.sl = a.seq_len + a.seq_off[i];
-          s.cap = a.seq_off[i + 1] - a.seq_off[i];
-        }
-        if (tree_ok) {
-          // xdg.cpp:150-153
-          r = d3(a.p0[3 * i], a.p0[3 * i + 1], a.p0[3 * i + 2]);
-          u = sub(d3(a.p1[3 * i], a.p1[3 * i + 1], a.p1[3 * i + 2]), r);
-          distance = sqrt(u.x * u.x + u.y * u.y + u.z * u.z);
-          u.x = u.x / distance;
-          u.y = u.y / distance;
-          u.z = u.z / distance;
-          idx = major_axis(u);
-          u_i = pick3(u.x, u.y, u.z, idx);
-          last = ID_NONE;
-          steps = 0;
-          loops = 0;
-          capped = false;
-          // the first find_element of the segment, at p0, came from the
-          // pre-pass, so refills stay cheap
-          cur = a.start_elem[i];
-          fresh = true;
-          phase = LOCATE;
-        } else {
-          a.count[i] = 0;
-          a.hash[i] = s.hash;
-        }
-      }
-    }
-    if (phase == LOCATE) {
-      // one pass of the outer loop of XDG::segments (xdg.cpp:155-193)
-      bool finish = !(distance > 0);
-      if (!finish && (++loops > MAX_SEG_LOOPS || steps >= MAX_WALK_STEPS)) {
-        capped = true;
-        finish = true;
-      }
-      if (!finish) {
-        if (!fresh) cur = find_element_dev(T, r, stk, stride, cap);
-        fresh = false;
-        if (s.count > 0 && cur == last) cur = ID_NONE;
-        if (cur == ID_NONE) {
-          // xdg.cpp:164-185: fire against the implicit complement
-          if (T.ipc_tree < 1 || T.ipc_tree > a.nsurf_trees) {
-            finish = true;
-          } else {
-            RayD ray;
-            ray.o = d3(r.x + TINY_BIT * u.x, r.y + TINY_BIT * u.y, r.z + TINY_BIT * u.z);
-            ray.d = u;
-            ray.rb = cross(ray.d, ray.o);
-            ray.idx = idx;
-            ray.oi = pick3(ray.o.x, ray.o.y, ray.o.z, idx);
-            ray.di = u_i;
-            HitState h{INFTY, INT32_MAX, ID_NONE, false};
-            trace<RAYFIRE, false>(a.stable[T.ipc_tree], ray, ORIENT_EXITING, nullptr, 0, stk, stride, cap, h);
-            if (h.surf == ID_NONE || h.t > distance) {
-              finish = true;
-            } else {
-              const double hit_dist = h.t + TINY_BIT;
-              r.x = r.x + hit_dist * u.x;
-              r.y = r.y + hit_dist * u.y;
-              r.z = r.z + hit_dist * u.z;
-              distance -= hit_dist;
-              const int32_t adjacent = T.bface_elem ? T.bface_elem[h.face] : ID_NONE;
-              if (adjacent == ID_NONE)
-                finish = true;
-              else
-                cur = adjacent;
-            }
-          }
-        }
-        // walk_elements does not step when the re-entry used up the segment
-        if (!finish && !(distance > 0)) finish = true;
-      }
-      if (finish) {
-        a.count[i] = s.count;
-        a.hash[i] = s.hash;
-        if (capped && a.status) atomicOr(a.status, 1u);
-        phase = IDLE;
-      } else {
-        // walk_elements (mesh_manager_interface.cpp:107-136) from cur
-        rr = r;
-        dist_w = distance;
-        sum = 0.0;
-        elem_load<FO>(T, cur, L);
-        phase = WALK;
-      }
-    }
-    const int nact = __popcll(__ballot(1));
-    const int need = refill_min < nact ? refill_min : nact;
-    if (phase == WALK) {
-      __builtin_amdgcn_s_waitcnt(0);  // as in k_segments: the loop head then waits for its own loads only
-      while (true) {
-        int32_t next;
-        double ex;
-        elem_step<FO>(T, L, rr, u, idx, u_i, next, ex);
-        const double len = (dist_w < ex) ? dist_w : ex;
-        dist_w -= len;
-        const int32_t here = cur;
-        last = cur;
-        sum = sum + len;
-        rr.x = rr.x + len * u.x;
-        rr.y = rr.y + len * u.y;
-        rr.z = rr.z + len * u.z;
-        cur = next;
-        const bool stop = cur == ID_NONE || ++steps >= MAX_WALK_STEPS || !(dist_w > 0);
-        // next record requested before this step's writes (see k_segments)
-        elem_load<FO>(T, stop ? here : cur, L);
-        sink_push<OUT>(s, here, len);
-        if (stop) {
-          r.x = r.x + sum * u.x;
-          r.y = r.y + sum * u.y;
-          r.z = r.z + sum * u.z;
-          distance -= sum;
-          if (distance > 0) {
-            phase = LOCATE;
-          } else {
-            a.count[i] = s.count;
-            a.hash[i] = s.hash;
-            phase = IDLE;
-          }
-        }
-        const int walking = __popcll(__ballot(phase == WALK));
-        if (phase != WALK || nact - walking >= need) break;
-      }
-    }
-  }
-}
-
-__global__ void k_zero_seg(unsigned long long* p) { *p = 0ull; }
-
-// find_element at every segment's start point (xdg.cpp:150-157: the first
-// pass of the outer loop locates r = p0), one lane per segment
-__global__ void k_segment_starts(SegmentsArgs a, int cap)
-{
-  extern __shared__ int32_t lds[];
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  int32_t e = ID_NONE;
-  if (a.tree >= 1 && a.tree <= a.nelem_trees) {
-    const D3 p = d3(a.p0[3 * i], a.p0[3 * i + 1], a.p0[3 * i + 2]);
-    e = find_element_dev(a.etable[a.tree], p, lds + threadIdx.x, blockDim.x, cap);
-  }
-  a.start_elem[i] = e;
-}
-
 __global__ void k_elem_boxes(const ElemRec* __restrict__ elems, int64_t n, double* box6)
 {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -718,19 +523,6 @@ void launch_next_element(const NextElementArgs& a, hipStream_t s)
 template <int FO, int OUT>
 void launch_segments_t(const SegmentsArgs& a, const LaunchCfg& c, hipStream_t s)
 {
-  if ((c.persistent & 8) && a.next && a.start_elem) {
-    const size_t sh = lds_bytes(c);
-    const int64_t need = (a.n + c.block - 1) / c.block;
-    int dev = 0, cu = 256, nb = 1;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_segments_pt<FO, OUT>, c.block, sh);
-    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)std::max(nb, 1) * cu));
-    hipLaunchKernelGGL(k_zero_seg, dim3(1), dim3(1), 0, s, a.next);
-    hipLaunchKernelGGL(k_segment_starts, grid1(a.n, c.block), dim3(c.block), sh, s, a, c.stack);
-    hipLaunchKernelGGL((k_segments_pt<FO, OUT>), dim3(g), dim3(c.block), sh, s, a, c.stack, a.next, c.seg_refill);
-    return;
-  }
   hipLaunchKernelGGL((k_segments<FO, OUT>), grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
 }
 

@@ -113,12 +113,11 @@ struct LaunchCfg {
   int defer = 1;  // mixed ray_fire: FP64 pass for far-origin rays
   int wide = 1;   // surface queries traverse the BVH4
   // resident-grid kernels with per-lane work refill: bit 0 plain ray_fire,
-  // bit 1 mixed ray_fire, bit 2 find_volume, bit 3 tally_segments
+  // bit 1 mixed ray_fire, bit 2 find_volume
   int persistent = 3;
   int order = 0;  // ray_fire coherence ordering mode (0 = off)
   int spec = 1;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
   int refill = 40; // persistent ray_fire / find_volume: idle lanes of a wave needed before a refill
-  int seg_refill = 8; // persistent tally_segments: the same for segments
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
@@ -290,8 +289,6 @@ struct SegmentsArgs {
   int32_t* seq_elem;
   double* seq_len;
   uint32_t* status;  // device word: bit0 = step cap hit
-  unsigned long long* next;  // persistent launch: segment counter (nullable)
-  int32_t* start_elem;       // persistent launch: per-segment scratch for the element holding p0
 };
 void launch_segments(const SegmentsArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s);
 // 0 = MOAB canonical, 1 = MeshMock, 2 = other (read from the tree)
