@@ -425,7 +425,7 @@ int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
         require(vi >= 0 && vi < n_verts, "triangle vertex index out of range");
         for (int k = 0; k < 3; ++k) v9[9 * i + 3 * j + k] = verts[3 * (int64_t)vi + k];
       }
-      require(surf_ids[i] >= 0 && surf_ids[i] < (1 << 30), "surface id out of range [0, 2^30)");
+      require(surf_ids[i] >= 0 && surf_ids[i] < (1 << 27), "surface id out of range [0, 2^27)");
     }
     int32_t min_face = 0, max_face = -1;
     if (n_tris > 0) {

@@ -822,7 +822,8 @@ __global__ void k_write_tris(const double* __restrict__ v9, const int32_t* __res
   TriRec r;
   for (int k = 0; k < 9; ++k) r.v[k] = v9[9 * p + k];
   r.face = face[p];
-  r.surf2 = (surf[p] << 1) | (flip[p] ? 1 : 0);
+  const unsigned fb = tri_first_bits(d3(r.v[0], r.v[1], r.v[2]), d3(r.v[3], r.v[4], r.v[5]), d3(r.v[6], r.v[7], r.v[8]));
+  r.surf2 = (surf[p] << TRI_SURF_SHIFT) | (int32_t)(fb << 1) | (flip[p] ? 1 : 0);
   out[s] = r;
 }
 

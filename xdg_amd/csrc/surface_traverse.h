@@ -170,10 +170,11 @@ __device__ __forceinline__ void tri_closest_hit(const TT& T, int32_t p, const Ra
     st->F += 1;
     plucker_count(v0, v1, v2, ray.d, ray.rb, *st);
   }
-  if (!plucker_any(v0, v1, v2, ray.d, ray.rb, ray.idx, ray.oi, ray.di, h.t, 0.0, t)) return;
+  const int32_t s2 = r->surf2;
+  if (!plucker_any_f(v0, v1, v2, (unsigned)(s2 >> 1) & 7u, ray.d, ray.rb, ray.idx, ray.oi, ray.di, h.t, 0.0, t))
+    return;
   const int32_t face = r->face;
   if (t == h.t && face >= h.face) return;  // tie rule
-  const int32_t s2 = r->surf2;
   if (STATS) st->N += 1;
   const auto pn = as_global(T.normals + 3 * (int64_t)p);
   D3 n = d3(pn[0], pn[1], pn[2]);
@@ -187,7 +188,7 @@ __device__ __forceinline__ void tri_closest_hit(const TT& T, int32_t p, const Ra
     if (ex[j] == face) return;
   h.t = t;
   h.face = face;
-  h.surf = s2 >> 1;
+  h.surf = s2 >> TRI_SURF_SHIFT;
   if (MODE == PIV) h.inside = dot(ray.d, n) > 0.0;
 }
 
@@ -199,7 +200,8 @@ __device__ __forceinline__ bool tri_any_hit(const TriRec* __restrict__ tris, int
   const D3 v1 = d3(r->v[3], r->v[4], r->v[5]);
   const D3 v2 = d3(r->v[6], r->v[7], r->v[8]);
   double t;
-  return plucker_any(v0, v1, v2, ray.d, ray.rb, ray.idx, ray.oi, ray.di, INFTY, 0.0, t);
+  return plucker_any_f(v0, v1, v2, (unsigned)(r->surf2 >> 1) & 7u, ray.d, ray.rb, ray.idx, ray.oi, ray.di, INFTY,
+                       0.0, t);
 }
 
 // Generic traversal.  Returns true for OCCL when any hit is found.

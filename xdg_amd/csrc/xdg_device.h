@@ -149,6 +149,25 @@ __device__ __forceinline__ bool plucker_finish(double c0, double c1, double c2, 
 // plucker_ray_tri_intersect, unoriented mode (useOrientation = false), with
 // the ray moment rb = cross(d, o), the major axis and its components
 // precomputed per ray.
+// first_bits: bit 0 first(v0, v1), bit 1 first(v1, v2), bit 2 first(v2, v0).
+__device__ __forceinline__ bool plucker_any_f(const D3& v0, const D3& v1, const D3& v2, unsigned first_bits,
+                                              const D3& d, const D3& rb, int idx, double o_i, double d_i,
+                                              double tmax, double tmin, double& t)
+{
+  const double c0 = edge_test_f(v0, v1, first_bits & 1u, d, rb);
+  const double c1 = edge_test_f(v1, v2, first_bits & 2u, d, rb);
+  if ((0.0 < c0 && 0.0 > c1) || (0.0 > c0 && 0.0 < c1)) return false;
+  const double c2 = edge_test_f(v2, v0, first_bits & 4u, d, rb);
+  if ((0.0 < c1 && 0.0 > c2) || (0.0 > c1 && 0.0 < c2) || (0.0 < c0 && 0.0 > c2) || (0.0 > c0 && 0.0 < c2))
+    return false;
+  return plucker_finish(c0, c1, c2, v0, v1, v2, o_i, d_i, idx, tmax, tmin, t);
+}
+
+__device__ __forceinline__ unsigned tri_first_bits(const D3& v0, const D3& v1, const D3& v2)
+{
+  return (first(v0, v1) ? 1u : 0u) | (first(v1, v2) ? 2u : 0u) | (first(v2, v0) ? 4u : 0u);
+}
+
 __device__ __forceinline__ bool plucker_any(const D3& v0, const D3& v1, const D3& v2, const D3& d, const D3& rb,
                                             int idx, double o_i, double d_i, double tmax, double tmin, double& t)
 {
@@ -434,8 +453,10 @@ struct alignas(128) Node4D {  // plain FP64 layout, 208 B used, 256 B stride
 struct alignas(16) TriRec {
   double v[9];
   int32_t face;   // mesh face id (the primitive id returned by queries)
-  int32_t surf2;  // (surface id << 1) | reverse-sense flag for this tree
+  int32_t surf2;  // (surface id << 4) | first(v2, v0) << 3 | first(v1, v2) << 2 |
+                  // first(v0, v1) << 1 | reverse-sense flag for this tree
 };
+constexpr int TRI_SURF_SHIFT = 4;
 
 // Tetrahedron record indexed by element index, 128 B.
 struct alignas(16) ElemRec {
