@@ -113,7 +113,7 @@ int xdg_hip_synchronize(xdg_hip_ctx* ctx, void* stream);
 /* RayTracer::create_surface_tree (ray_tracing_interface.h:51; Embree:
  * ray_tracer.cpp:50-105).  One tree = all triangles of one volume, in the
  * caller's order: tris[3*i..] index verts; face_ids[i] is the mesh face id
- * returned by queries; surf_ids[i] the surface id; flip[i] = 1 when this
+ * returned by queries; surf_ids[i] the surface id (0 <= id < 2^27); flip[i] = 1 when this
  * volume is the surface's reverse-sense parent (triangle_intersect.cpp:84).
  * bump: box dilation, normally max(1e-6 * volume box chord, 1e-3)
  * (ray_tracing_interface.cpp:20-24). */
