@@ -335,6 +335,32 @@ def test_sphere_limits_and_exclusions(ctx, torch_mod, sphere5):
         assert np.array_equal(bits(host(t)), bits(et))
 
 
+def test_surface_id_range(ctx, torch_mod):
+    """Surface ids up to 2^27 - 1 come back intact beside the edge-order
+    bits packed into the triangle record; larger ids are rejected."""
+    from xdg_amd import capi
+    torch = torch_mod
+    m = G.single_sphere(3, 10.0)
+    v9, f, s, fl = m.tree_arrays(1)
+    nt = v9.shape[0]
+    verts = v9.reshape(-1, 3)
+    tris = np.arange(3 * nt, dtype=np.int32).reshape(-1, 3)
+    big = np.full(nt, (1 << 27) - 1, np.int32)
+    tree = ctx.build_surface_tree(verts, tris, f, big, fl, m.bump(1))
+    org = P.ball_points(12, P.STREAM_POS, 4096, 9.0)
+    dirs = P.iso_dirs(12, P.STREAM_DIR, 4096)
+    ot = O.Tree.from_model(m, 1)
+    rt, rs, rf = ot.ray_fire(org, dirs)
+    for prec in (0, 1):
+        t, s_, f_ = ctx.ray_fire(tree, dev(torch, org), dev(torch, dirs), precision=prec)
+        hit = rs >= 0
+        assert np.array_equal(host(s_)[hit], np.full(hit.sum(), (1 << 27) - 1, np.int32))
+        assert np.array_equal(host(f_), rf)
+        assert np.array_equal(bits(host(t)), bits(rt))
+    with pytest.raises(capi.XdgHipError):
+        ctx.build_surface_tree(verts, tris, f, np.full(nt, 1 << 27, np.int32), fl, m.bump(1))
+
+
 def test_sphere_occluded_closest_piv(ctx, torch_mod, sphere5):
     torch = torch_mod
     m, tree, ot = sphere5
