@@ -212,14 +212,29 @@ int xdg_hip_next_element(xdg_hip_ctx* ctx, int32_t element_tree, int64_t n, cons
  * h = mix(h ^ elem); h = mix(h ^ bits(len)); mix = splitmix64 finaliser).
  * tally_fx (device, nullable, one per element): += llrint(len * 2^40).
  * seq_off (device, nullable, n+1 prefix offsets) with seq_elem/seq_len:
- * explicit sequences.  Returns XDG_HIP_ERR_CAPPED after synchronizing if a
- * walk hit its step bound. */
+ * explicit sequences.  Asynchronous on stream; a walk that hits its step
+ * bound (2^16 steps) sets bit 0 of the status word (xdg_hip_status). */
 int xdg_hip_tally_segments(xdg_hip_ctx* ctx, int32_t element_tree, int64_t n, const double* p0, const double* p1,
                            int64_t* tally_fx, int32_t* count_out, uint64_t* hash_out, const int64_t* seq_off,
                            int32_t* seq_elem, double* seq_len, void* stream);
 
+/* XDG::segments(volume, start, end) (src/xdg.cpp:200-227), outputs as in
+ * xdg_hip_tally_segments.  element_tree is the volume's element tree and
+ * surface_tree its surface tree.  A start outside the mesh enters it along
+ * the ENTERING hit on the volume's surface (no hit: no entries); the walk
+ * is MeshManager::walk_elements(elem, start, end)
+ * (mesh_manager_interface.cpp:139-147) from the entry point, with no
+ * implicit-complement re-entry.  When no element lies behind the entry hit
+ * (the reference warns) the segment has no entries and bit 1 of the status
+ * word is set. */
+int xdg_hip_tally_segments_volume(xdg_hip_ctx* ctx, int32_t element_tree, int32_t surface_tree, int64_t n,
+                                  const double* p0, const double* p1, int64_t* tally_fx, int32_t* count_out,
+                                  uint64_t* hash_out, const int64_t* seq_off, int32_t* seq_elem, double* seq_len,
+                                  void* stream);
+
 /* Synchronize the device, return and clear the status word (bit 0: a walk of
- * xdg_hip_tally_segments hit its step bound since the last call). */
+ * the segment calls hit its step bound since the last call; bit 1: a
+ * volume segment found no element behind its entry hit). */
 int xdg_hip_status(xdg_hip_ctx* ctx, uint32_t* flags);
 
 /* Evaluate one device geometry primitive per item, for parity checks

@@ -70,6 +70,9 @@ def lib():
                                        C.c_int64, _i32, _d]
         L.xo_segments.argtypes = [_vp, _vp, _i32, C.c_int64, _d, _d, C.c_int,
                                   _i32, _u64, _i64, _i64, _i32, _d]
+        L.xo_segments_volume.argtypes = [_vp, _vp, C.c_int64, _d, _d, C.c_int,
+                                         _i32, _u64, _i64, _i64, _i32, _d,
+                                         C.POINTER(C.c_uint32)]
         L.xo_mix64.restype = C.c_uint64
         L.xo_mix64.argtypes = [C.c_uint64]
         L.xo_philox4x32_10.argtypes = [C.POINTER(C.c_uint32),
@@ -329,6 +332,36 @@ class TetMesh:
                           _p(tal, _i64), _p(off, _i64), _p(se, _i32),
                           _p(sl, _d))
         out = [cnt, hsh]
+        if tally:
+            out.append(tal)
+        if sequences is not None:
+            out.append((off, se, sl))
+        return tuple(out)
+
+    def segments_volume(self, vol_tree, p0, p1, tally=False, sequences=None,
+                        nthreads=0):
+        """XDG::segments(volume, start, end).  vol_tree: the volume's
+        surface Tree (None: starts outside the mesh give no entries).
+        Returns (count, hash, flags[, tally_fx][, (offsets, elems, lens)])."""
+        p0, p1 = _f64(p0).reshape(-1, 3), _f64(p1).reshape(-1, 3)
+        n = p0.shape[0]
+        cnt = np.zeros(n, np.int32)
+        hsh = np.zeros(n, np.uint64)
+        flg = np.zeros(n, np.uint32)
+        tal = np.zeros(self.tets.shape[0], np.int64) if tally else None
+        off = se = sl = None
+        if sequences is not None:
+            cap = np.asarray(sequences, dtype=np.int64)
+            off = np.zeros(n + 1, np.int64)
+            off[1:] = np.cumsum(cap)
+            se = np.zeros(int(off[-1]), np.int32)
+            sl = np.zeros(int(off[-1]))
+        lib().xo_segments_volume(self.h, vol_tree.h if vol_tree else None, n,
+                                 _p(p0, _d), _p(p1, _d), nthreads,
+                                 _p(cnt, _i32), _p(hsh, _u64), _p(tal, _i64),
+                                 _p(off, _i64), _p(se, _i32), _p(sl, _d),
+                                 flg.ctypes.data_as(C.POINTER(C.c_uint32)))
+        out = [cnt, hsh, flg]
         if tally:
             out.append(tal)
         if sequences is not None:

@@ -16,6 +16,8 @@
 //   mock_next : 7 doubles/case (elem r u) on MeshMock -> 2 (next, dist)
 //   mock_walk : 8 doubles/case (elem r u distance) -> 1 + 2*64 doubles
 //               (count, then up to 64 (elem, len) pairs, zero padded)
+//   mock_walk_se : 7 doubles/case (elem start end) -> 1 + 2*64 doubles,
+//               MeshManager::walk_elements(elem, start, end)
 //   mock_face_normal : 1 double/case (face id) -> 3 doubles
 //   mock_tracks : 1 double (count) -> 6 doubles/track: the std::mt19937(42)
 //               start/end points of tests/test_tracks.cpp:170-180
@@ -117,6 +119,22 @@ int main(int argc, char** argv)
     for (size_t i = 0; i + 8 <= in.size(); i += 8) {
       const double* c = &in[i];
       auto segs = mm->walk_elements((MeshID)c[0], v3(c + 1), v3(c + 4), c[7]);
+      out.push_back((double)segs.size());
+      for (size_t j = 0; j < 64; ++j) {
+        if (j < segs.size()) {
+          out.push_back((double)segs[j].first);
+          out.push_back(segs[j].second);
+        } else {
+          out.push_back(0.0);
+          out.push_back(0.0);
+        }
+      }
+    }
+  } else if (mode == "mock_walk_se") {
+    auto mm = std::make_shared<MeshMock>();
+    for (size_t i = 0; i + 7 <= in.size(); i += 7) {
+      const double* c = &in[i];
+      auto segs = mm->walk_elements((MeshID)c[0], v3(c + 1), v3(c + 4));
       out.push_back((double)segs.size());
       for (size_t j = 0; j < 64; ++j) {
         if (j < segs.size()) {

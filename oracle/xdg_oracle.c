@@ -1204,6 +1204,86 @@ void xo_segments(const xo_tetmesh* m, const xo_tree* ipc,
   }
 }
 
+/* XDG::segments(volume, start, end) (src/xdg.cpp:200-227).  vol: the
+ * volume's surface tree (senses as seen from the volume).  A start outside
+ * the element tree enters through the first ENTERING hit (xdg.cpp:211-222);
+ * the walk is MeshManager::walk_elements(elem, start, end)
+ * (mesh_manager_interface.cpp:139-147): direction and length are taken
+ * again from the moved start, with Vec3da::normalize.  flags (nullable):
+ * bit 1 = surface hit with no element behind it (the reference warns and
+ * returns no segments). */
+void xo_segments_volume(const xo_tetmesh* m, const xo_tree* vol, int64_t n,
+                        const double* p0, const double* p1, int nthreads,
+                        int32_t* count, uint64_t* hash, int64_t* tally_fx,
+                        const int64_t* seq_off, int32_t* seq_elem,
+                        double* seq_len, uint32_t* flags)
+{
+  int nt = resolve_threads(nthreads);
+  (void)nt;
+  OMP_FOR
+  for (int64_t i = 0; i < n; ++i) {
+    seg_sink s;
+    s.count = 0;
+    s.hash = XO_HASH_SEED;
+    s.tally = tally_fx;
+    if (seq_off) {
+      s.se = seq_elem + seq_off[i];
+      s.sl = seq_len + seq_off[i];
+      s.cap = seq_off[i + 1] - seq_off[i];
+    } else {
+      s.se = NULL;
+      s.sl = NULL;
+      s.cap = 0;
+    }
+    uint32_t f = 0;
+    /* xdg.cpp:205-208 */
+    const double* a = p0 + 3 * i;
+    double start[3], u[3];
+    v_copy(a, start);
+    v_sub(p1 + 3 * i, a, u);
+    v_normalize(u);
+    int32_t e = find_element_one(m, a, 0);
+    int ok = 1;
+    if (e == XO_ID_NONE) {
+      /* xdg.cpp:211-222 */
+      ok = vol != NULL;
+      if (ok) {
+        hit_state h;
+        trace_one(vol, a, u, INFTY, XO_ENTERING, NULL, 0, 0, &h);
+        ok = h.surf != XO_ID_NONE;
+        if (ok) {
+          const double into = h.t + TINY_BIT;
+          double q[3] = {a[0] + u[0] * into, a[1] + u[1] * into,
+                         a[2] + u[2] * into};
+          e = find_element_one(m, q, 0);
+          if (e == XO_ID_NONE) {
+            ok = 0;
+            f |= 2u;
+          } else {
+            start[0] = start[0] + u[0] * h.t;
+            start[1] = start[1] + u[1] * h.t;
+            start[2] = start[2] + u[2] * h.t;
+          }
+        }
+      }
+    }
+    if (ok) {
+      /* mesh_manager_interface.cpp:139-147 */
+      double w[3];
+      v_sub(p1 + 3 * i, start, w);
+      const double distance = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+      v_normalize(w);
+      int32_t last = XO_ID_NONE;
+      int steps = 0;
+      walk_sink(m, e, start, w, distance, &s, &last, &steps);
+      if (steps >= MAX_WALK_STEPS) f |= 1u;
+    }
+    count[i] = s.count;
+    hash[i] = s.hash;
+    if (flags) flags[i] = f;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) and uniforms            */
 /* ------------------------------------------------------------------------ */

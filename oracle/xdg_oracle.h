@@ -183,6 +183,15 @@ void xo_segments(const xo_tetmesh* m, const xo_tree* ipc,
                  uint64_t* hash, int64_t* tally_fx, const int64_t* seq_off,
                  int32_t* seq_elem, double* seq_len);
 
+/* XDG::segments(volume, start, end) (src/xdg.cpp:200-227).  vol: the
+ * volume's surface tree; outputs as in xo_segments; flags[i] (nullable):
+ * bit 0 = step bound hit, bit 1 = entering hit with no element behind it. */
+void xo_segments_volume(const xo_tetmesh* m, const xo_tree* vol, int64_t n,
+                        const double* p0, const double* p1, int nthreads,
+                        int32_t* count, uint64_t* hash, int64_t* tally_fx,
+                        const int64_t* seq_off, int32_t* seq_elem,
+                        double* seq_len, uint32_t* flags);
+
 /* Sequence hash used for per-segment parity: h0 = XO_HASH_SEED; per entry
  * h = mix64(h ^ (uint32)elem); h = mix64(h ^ bits(len)).  mix64 is the
  * splitmix64 finaliser. */

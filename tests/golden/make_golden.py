@@ -223,5 +223,54 @@ def main():
           "max segs", int(walk_out[:, 0].max()))
 
 
+def walk_start_end():
+    """MeshManager::walk_elements(elem, start, end)
+    (mesh_manager_interface.cpp:139-147) on MeshMock, as XDG::segments(volume,
+    start, end) (src/xdg.cpp:200-227) calls it: the MeshMock tracks with the
+    start element from find_element, plus tracks that start outside the mock
+    box, entered through the first ENTERING hit on its boundary (the element
+    behind the hit and the moved start are inputs here, computed with the
+    oracle)."""
+    import pyoracle as O  # test-infrastructure oracle
+    rng = np.random.default_rng(20240607)
+    mesh = G.mock_tetmesh()
+    tm = O.TetMesh(mesh)
+    bm = mesh.boundary_model()
+    vol = O.Tree.from_model(bm, 1)
+    tracks = np.load(os.path.join(HERE, "golden_mock_tracks.npz"))["tracks"]
+    starts, ends = tracks[:, :3].copy(), tracks[:, 3:].copy()
+    elems = tm.find_element(starts)
+    lo = np.array([-2.0, -3.0, -4.0])
+    hi = np.array([5.0, 6.0, 7.0])
+    # outside starts: a shell around the box; ends inside it
+    o = rng.uniform(lo - 4.0, hi + 4.0, (600, 3))
+    inside = ((o > lo) & (o < hi)).all(1)
+    o = o[~inside][:300]
+    e = rng.uniform(lo, hi, (o.shape[0], 3))
+    u = e - o
+    ln = np.sqrt(u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1] + u[:, 2] * u[:, 2])
+    ln = np.where(ln < G.MIN_RCP_INPUT, G.MIN_RCP_INPUT, ln)
+    u = np.stack([u[:, 0] / ln, u[:, 1] / ln, u[:, 2] / ln], 1)
+    t, surf, _ = vol.ray_fire(o, u, orientation=1)
+    assert (surf >= 0).all()
+    into = t + 1e-10
+    q = np.stack([o[:, k] + u[:, k] * into for k in range(3)], 1)
+    oe = tm.find_element(q)
+    assert (oe >= 0).all()
+    moved = np.stack([o[:, k] + u[:, k] * t for k in range(3)], 1)
+    se_in = np.concatenate([
+        np.concatenate([elems[:, None].astype(np.float64), starts, ends], 1),
+        np.concatenate([oe[:, None].astype(np.float64), moved, e], 1)])
+    se_out = run("mock_walk_se", se_in, 129)
+    np.savez_compressed(os.path.join(HERE, "golden_mock_walk_se.npz"),
+                        walk_se_in=se_in, walk_se_out=se_out, outside_start=o)
+    print("mock walk start/end", se_in.shape, "max segs",
+          int(se_out[:, 0].max()))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["walk_se"]:
+        walk_start_end()
+    else:
+        main()
+        walk_start_end()

@@ -503,6 +503,80 @@ def test_mock_segments_sequences(ctx, torch_mod, golden_dir):
         assert np.array_equal(bits(gl[a:b]), bits(sl[a:b]))
 
 
+def _segments_volume_cases(n, seed):
+    """Segments for XDG::segments(volume, start, end) on the unit cube:
+    inside -> inside, outside -> inside, outside -> beyond the cube, outside
+    -> away from it (no entry), and outside -> short of the cube (the end
+    lies before the entry, so the reference walks back from the entry)."""
+    rng = np.random.default_rng(seed)
+    k = n // 5
+    p0 = [rng.uniform(0.01, 0.99, (k, 3))]
+    p1 = [rng.uniform(0.01, 0.99, (k, 3))]
+    o = rng.uniform(-1.0, 2.0, (8 * k, 3))
+    o = o[~((o > 0) & (o < 1)).all(1)][:4 * k]
+    oa, ob, oc, od = np.split(o, 4)
+    e_in = rng.uniform(0.02, 0.98, (k, 3))
+    p0 += [oa, ob, oc, od]
+    p1 += [e_in[:oa.shape[0]], 0.5 + (0.5 - ob) * 1.5, oc + (oc - 0.5) * 3.0,
+           od + (0.5 - od) * 0.05]
+    return np.concatenate(p0), np.concatenate(p1)
+
+
+def test_kuhn_segments_volume(ctx, torch_mod):
+    torch = torch_mod
+    mesh = G.kuhn_cube(16, 0.2, seed=4)
+    et = _segments_setup(ctx, mesh)
+    bm = mesh.boundary_model()
+    st = build_tree(ctx, bm, 1)
+    om = O.TetMesh(mesh)
+    vol = O.Tree.from_model(bm, 1)
+    p0, p1 = _segments_volume_cases(20000, 5)
+    n = p0.shape[0]
+    rc, rh, rf, rt = om.segments_volume(vol, p0, p1, tally=True)
+    tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device="cuda")
+    cnt, hsh = ctx.tally_segments_volume(et, st, dev(torch, p0), dev(torch, p1),
+                                         tally=tally)
+    assert np.array_equal(host(cnt), rc)
+    assert np.array_equal(host(hsh).view(np.uint64), rh)
+    assert np.array_equal(host(tally), rt)
+    want = (2 if (rf & 2).any() else 0) | (1 if (rf & 1).any() else 0)
+    assert ctx.status() == want
+    # each case kind produced entries where the closed form says it must
+    k = n // 5
+    assert (rc[:2 * k] > 0).all() and (rc[3 * k:4 * k] == 0).sum() > 0
+
+
+def test_mock_segments_volume_sequences(ctx, torch_mod, golden_dir):
+    torch = torch_mod
+    mesh = G.mock_tetmesh()
+    et = _segments_setup(ctx, mesh)
+    bm = mesh.boundary_model()
+    st = build_tree(ctx, bm, 1)
+    g = np.load(os.path.join(golden_dir, "golden_mock_walk_se.npz"))
+    tracks = np.load(os.path.join(golden_dir, "golden_mock_tracks.npz"))["tracks"]
+    nin = tracks.shape[0]
+    p0 = np.concatenate([tracks[:, :3], g["outside_start"]])
+    p1 = np.concatenate([tracks[:, 3:], g["walk_se_in"][nin:, 4:7]])
+    sout = g["walk_se_out"]
+    cap = np.full(len(p0), 64, np.int64)
+    off = np.zeros(len(p0) + 1, np.int64)
+    off[1:] = np.cumsum(cap)
+    seq_e = torch.zeros(int(off[-1]), dtype=torch.int32, device="cuda")
+    seq_l = torch.zeros(int(off[-1]), dtype=torch.float64, device="cuda")
+    cnt, hsh = ctx.tally_segments_volume(et, st, dev(torch, p0), dev(torch, p1),
+                                         seq_off=dev(torch, off), seq_elem=seq_e,
+                                         seq_len=seq_l)
+    cnt = host(cnt)
+    assert np.array_equal(cnt, sout[:, 0].astype(np.int32))
+    ge, gl = host(seq_e), host(seq_l)
+    for i in range(len(p0)):
+        pairs = sout[i, 1:1 + 2 * cnt[i]].reshape(-1, 2)
+        a, b = off[i], off[i] + cnt[i]
+        assert np.array_equal(ge[a:b], pairs[:, 0].astype(np.int32))
+        assert np.array_equal(bits(gl[a:b]), bits(pairs[:, 1]))
+    assert ctx.status() == 0
+
+
 def test_fresh_context_status_is_clear(torch_mod):
     """The sticky status word starts at zero on every new context, also when
     its memory is recycled from contexts that were destroyed."""

@@ -110,6 +110,9 @@ def lib():
                                      _vp],
             "xdg_hip_tally_segments": [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp,
                                        _vp, _vp, _vp, _vp],
+            "xdg_hip_tally_segments_volume": [_vp, _i32, _i32, _i64, _vp, _vp,
+                                              _vp, _vp, _vp, _vp, _vp, _vp,
+                                              _vp],
             "xdg_hip_status": [_vp, C.POINTER(C.c_uint32)],
             "xdg_hip_eval_primitive": [_vp, C.c_int, _i64, _vp, _vp, _vp],
             "xdg_hip_transport_create": [_vp, C.POINTER(TransportProblem),
@@ -395,6 +398,23 @@ class Context:
         self._done(lib().xdg_hip_tally_segments(
             self.h, etree, n, _ptr(p0), _ptr(p1), _ptr(tally), _ptr(cnt),
             _ptr(hsh), _ptr(seq_off), _ptr(seq_elem), _ptr(seq_len), stream), stream)
+        return cnt, hsh
+
+    @_ordered
+    def tally_segments_volume(self, etree, stree, p0, p1, tally=None,
+                              seq_off=None, seq_elem=None, seq_len=None,
+                              out=None, stream=None):
+        """XDG::segments(volume, start, end) (src/xdg.cpp:200-227): etree and
+        stree are the volume's element and surface trees."""
+        n = p0.shape[0]
+        if out is None:
+            out = (torch.empty(n, dtype=torch.int32, device=p0.device),
+                   torch.empty(n, dtype=torch.int64, device=p0.device))
+        cnt, hsh = out
+        self._done(lib().xdg_hip_tally_segments_volume(
+            self.h, etree, stree, n, _ptr(p0), _ptr(p1), _ptr(tally),
+            _ptr(cnt), _ptr(hsh), _ptr(seq_off), _ptr(seq_elem),
+            _ptr(seq_len), stream), stream)
         return cnt, hsh
 
     # ---- batched particle random walk ------------------------------------

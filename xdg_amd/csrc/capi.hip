@@ -911,6 +911,43 @@ int xdg_hip_tally_segments(xdg_hip_ctx* c, int32_t element_tree, int64_t n, cons
   });
 }
 
+int xdg_hip_tally_segments_volume(xdg_hip_ctx* c, int32_t element_tree, int32_t surface_tree, int64_t n,
+                                  const double* p0, const double* p1, int64_t* tally_fx, int32_t* count_out,
+                                  uint64_t* hash_out, const int64_t* seq_off, int32_t* seq_elem, double* seq_len,
+                                  void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    const ElemTreeHost& t = elem_tree(c, element_tree);
+    int depth = t.dev.depth;
+    depth = std::max(depth, surf_tree(c, surface_tree).dev.depth);
+    if (n == 0) return XDG_HIP_OK;
+    require(p0 && p1 && count_out && hash_out, "null array");
+    require(!seq_off || (seq_elem && seq_len), "seq_off given without seq_elem/seq_len");
+    SegmentsArgs a{};
+    a.etable = c->d_elem;
+    a.nelem_trees = (int32_t)c->elem.size();
+    a.stable = c->d_surf;
+    a.nsurf_trees = (int32_t)c->surf.size();
+    a.tree = element_tree;
+    a.n = n;
+    a.p0 = p0;
+    a.p1 = p1;
+    a.tally_fx = (unsigned long long*)tally_fx;
+    a.count = count_out;
+    a.hash = hash_out;
+    a.seq_off = seq_off;
+    a.seq_elem = seq_elem;
+    a.seq_len = seq_len;
+    a.status = c->d_status;
+    a.volume_mode = 1;
+    a.surf_tree = surface_tree;
+    launch_segments(a, face_order_kind(t.dev.face_order), cfg_for(c, depth), pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
 int xdg_hip_status(xdg_hip_ctx* c, uint32_t* flags)
 {
   return guarded(c, [&]() {

@@ -365,6 +365,49 @@ __device__ __forceinline__ void sink_push(Sink& s, int32_t e, double len)
   }
 }
 
+// MeshManager::walk_elements(elem, start, u, distance)
+// (mesh_manager_interface.cpp:107-136) from element cur, fused with the
+// sink; returns the std::accumulate sum of the lengths (xdg.cpp:190-191).
+// `last` is the last element pushed; `steps` counts steps against
+// MAX_WALK_STEPS over the whole segment.
+template <int FO, int OUT>
+__device__ __forceinline__ double walk_dev(const ElemTreeDev& T, int32_t cur, const D3& r, const D3& u, int idx,
+                                           double u_i, double dist_w, Sink& s, int32_t& last, int& steps)
+{
+  D3 rr = r;
+  double sum = 0.0;
+  ElemLoad L;
+  elem_load<FO>(T, cur, L);  // cur is an element here
+  // Wait for the record here, so that at the loop head the only loads in
+  // flight are the ones issued by the previous step, which the tally atomic
+  // follows: the head then waits for the loads alone (one operation left
+  // outstanding) instead of for everything.
+  __builtin_amdgcn_s_waitcnt(0);
+  while (dist_w > 0) {
+    int32_t next;
+    double ex;
+    elem_step<FO>(T, L, rr, u, idx, u_i, next, ex);
+    const double len = (dist_w < ex) ? dist_w : ex;
+    dist_w -= len;
+    const int32_t here = cur;
+    last = cur;
+    sum = sum + len;
+    rr.x = rr.x + len * u.x;
+    rr.y = rr.y + len * u.y;
+    rr.z = rr.z + len * u.z;
+    cur = next;
+    const bool stop = cur == ID_NONE || ++steps >= MAX_WALK_STEPS;
+    // the next record is requested before this step's tally and sequence
+    // writes, so waiting for it does not wait for them; a lane that stops
+    // requests its current record again (unused, always in range) so that
+    // the loads are not under a branch
+    elem_load<FO>(T, (!stop && dist_w > 0) ? cur : here, L);
+    sink_push<OUT>(s, here, len);
+    if (stop) break;
+  }
+  return sum;
+}
+
 template <int FO, int OUT>
 __global__ void k_segments(SegmentsArgs a, int cap)
 {
@@ -441,38 +484,7 @@ __global__ void k_segments(SegmentsArgs a, int cap)
       cur = adjacent;
     }
     // walk_elements (mesh_manager_interface.cpp:107-136) from cur
-    D3 rr = r;
-    double dist_w = distance;
-    double sum = 0.0;
-    ElemLoad L;
-    elem_load<FO>(T, cur, L);  // cur is an element here
-    // Wait for the record here, so that at the loop head the only loads
-    // in flight are the ones issued by the previous step, which the tally
-    // atomic follows: the head then waits for the loads alone (one
-    // operation left outstanding) instead of for everything.
-    __builtin_amdgcn_s_waitcnt(0);
-    while (dist_w > 0) {
-      int32_t next;
-      double ex;
-      elem_step<FO>(T, L, rr, u, idx, u_i, next, ex);
-      const double len = (dist_w < ex) ? dist_w : ex;
-      dist_w -= len;
-      const int32_t here = cur;
-      last = cur;
-      sum = sum + len;
-      rr.x = rr.x + len * u.x;
-      rr.y = rr.y + len * u.y;
-      rr.z = rr.z + len * u.z;
-      cur = next;
-      const bool stop = cur == ID_NONE || ++steps >= MAX_WALK_STEPS;
-      // the next record is requested before this step's tally and sequence
-      // writes, so waiting for it does not wait for them; a lane that
-      // stops requests its current record again (unused, always in range)
-      // so that the loads are not under a branch
-      elem_load<FO>(T, (!stop && dist_w > 0) ? cur : here, L);
-      sink_push<OUT>(s, here, len);
-      if (stop) break;
-    }
+    const double sum = walk_dev<FO, OUT>(T, cur, r, u, idx, u_i, distance, s, last, steps);
     r.x = r.x + sum * u.x;
     r.y = r.y + sum * u.y;
     r.z = r.z + sum * u.z;
@@ -481,6 +493,89 @@ __global__ void k_segments(SegmentsArgs a, int cap)
   a.count[i] = s.count;
   a.hash[i] = s.hash;
   if (capped && a.status) atomicOr(a.status, 1u);
+}
+
+// XDG::segments(volume, start, end) (xdg.cpp:200-227) over the volume's
+// element tree, fused with the same sink as k_segments.  A start outside
+// the mesh enters it through the volume's surface (ENTERING ray_fire, no
+// exclusions); the walk then runs MeshManager::walk_elements(elem, start,
+// end) (mesh_manager_interface.cpp:139-147), which takes its direction and
+// length again from the moved start.
+template <int FO, int OUT>
+__global__ void k_segments_vol(SegmentsArgs a, int cap)
+{
+  extern __shared__ int32_t lds[];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  Sink s;
+  s.count = 0;
+  s.hash = HASH_SEED;
+  s.tally = a.tally_fx;
+  if (OUT == OUT_SEQ) {
+    s.se = a.seq_elem + a.seq_off[i];
+    s.sl = a.seq_len + a.seq_off[i];
+    s.cap = a.seq_off[i + 1] - a.seq_off[i];
+  } else {
+    s.se = nullptr;
+    s.sl = nullptr;
+    s.cap = 0;
+  }
+  if (a.tree < 1 || a.tree > a.nelem_trees) {
+    a.count[i] = 0;
+    a.hash[i] = s.hash;
+    return;
+  }
+  const ElemTreeDev T = a.etable[a.tree];  // before any store: scalar registers
+  int32_t* stk = lds + threadIdx.x;
+  const int stride = blockDim.x;
+  const D3 p0 = d3(a.p0[3 * i], a.p0[3 * i + 1], a.p0[3 * i + 2]);
+  const D3 p1 = d3(a.p1[3 * i], a.p1[3 * i + 1], a.p1[3 * i + 2]);
+  // xdg.cpp:205-207
+  D3 start = p0;
+  const D3 u = normalize(sub(p1, p0));
+  int32_t e = find_element_dev(T, p0, stk, stride, cap);
+  bool ok = true;
+  uint32_t flags = 0;
+  if (e == ID_NONE) {
+    // xdg.cpp:211-222: the first entering hit on the volume's surface
+    ok = a.surf_tree >= 1 && a.surf_tree <= a.nsurf_trees;
+    if (ok) {
+      RayD ray;
+      ray.o = p0;
+      ray.d = u;
+      ray.rb = cross(u, p0);
+      ray.idx = major_axis(u);
+      ray.oi = pick3(p0.x, p0.y, p0.z, ray.idx);
+      ray.di = pick3(u.x, u.y, u.z, ray.idx);
+      HitState h{INFTY, INT32_MAX, ID_NONE, false};
+      trace<RAYFIRE, false>(a.stable[a.surf_tree], ray, ORIENT_ENTERING, nullptr, 0, stk, stride, cap, h);
+      ok = h.surf != ID_NONE;
+      if (ok) {
+        const double into = h.t + TINY_BIT;
+        e = find_element_dev(T, d3(p0.x + u.x * into, p0.y + u.y * into, p0.z + u.z * into), stk, stride, cap);
+        if (e == ID_NONE) {
+          ok = false;
+          flags |= 2u;  // the reference warns: surface hit, no element behind it
+        } else {
+          start = d3(start.x + u.x * h.t, start.y + u.y * h.t, start.z + u.z * h.t);
+        }
+      }
+    }
+  }
+  if (ok) {
+    // MeshManager::walk_elements(elem, start, end)
+    const D3 w0 = sub(p1, start);
+    const double distance = sqrt(w0.x * w0.x + w0.y * w0.y + w0.z * w0.z);
+    const D3 w = normalize(w0);
+    const int idx = major_axis(w);
+    int32_t last = ID_NONE;
+    int steps = 0;
+    walk_dev<FO, OUT>(T, e, start, w, idx, pick3(w.x, w.y, w.z, idx), distance, s, last, steps);
+    if (steps >= MAX_WALK_STEPS) flags |= 1u;
+  }
+  a.count[i] = s.count;
+  a.hash[i] = s.hash;
+  if (flags && a.status) atomicOr(a.status, flags);
 }
 
 __global__ void k_elem_boxes(const ElemRec* __restrict__ elems, int64_t n, double* box6)
@@ -523,7 +618,10 @@ void launch_next_element(const NextElementArgs& a, hipStream_t s)
 template <int FO, int OUT>
 void launch_segments_t(const SegmentsArgs& a, const LaunchCfg& c, hipStream_t s)
 {
-  hipLaunchKernelGGL((k_segments<FO, OUT>), grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+  if (a.volume_mode)
+    hipLaunchKernelGGL((k_segments_vol<FO, OUT>), grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+  else
+    hipLaunchKernelGGL((k_segments<FO, OUT>), grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
 }
 
 void launch_segments(const SegmentsArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s)

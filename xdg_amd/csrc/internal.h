@@ -288,7 +288,11 @@ struct SegmentsArgs {
   const int64_t* seq_off;  // nullable
   int32_t* seq_elem;
   double* seq_len;
-  uint32_t* status;  // device word: bit0 = step cap hit
+  uint32_t* status;  // device word: bit0 = step cap hit, bit1 = no element behind the entry hit
+  // XDG::segments(volume, start, end): walks only the element tree and
+  // enters it through surface tree surf_tree (ENTERING)
+  int volume_mode;
+  int32_t surf_tree;
 };
 void launch_segments(const SegmentsArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s);
 // 0 = MOAB canonical, 1 = MeshMock, 2 = other (read from the tree)
