@@ -258,7 +258,14 @@ int xdg_hip_eval_primitive(xdg_hip_ctx* ctx, int kind, int64_t n, const double* 
  * tally in the current volume, and either a collision (absorbed with
  * probability p_absorb, else a new isotropic direction) or a surface
  * crossing (MeshManager::next_volume, mesh_manager_interface.cpp:202-214).
- * A particle entering volume ipc leaks (ipc_graveyard).  Random numbers:
+ * A particle entering volume ipc leaks (ipc_graveyard).  A surface
+ * crossing first reads the surface's boundary condition
+ * (MeshManager::get_surface_property, mesh_manager_interface.cpp:99-104;
+ * Particle::cross_surface, particle_sim.h:85-130): vacuum ends the history,
+ * reflecting mirrors the direction about face_normal of the hit face
+ * (u - n * (2 (n.u) / |n|), normalized; XDG::surface_normal, xdg.cpp:272-290)
+ * and keeps only that face in the history, transmission (the default)
+ * moves to the next volume.  Random numbers:
  * Philox4x32-10, key (seed, stream), counter (history, 0, draw, 0).
  * Additions to the reference driver: Philox, sampled starts, absorption,
  * the event cap.  One persistent launch runs all histories; lanes claim
@@ -276,7 +283,14 @@ typedef struct {
   double mfp;                 /* mean free path */
   double p_absorb;            /* absorption probability per collision */
   int32_t max_events;         /* event cap per history */
+  const uint8_t* surf_bc;     /* host, nullable, nsurf_ids: XDG_HIP_BC_* per
+                                 surface id (NULL: all transmission) */
 } xdg_hip_transport_problem;
+
+/* Surface boundary conditions */
+#define XDG_HIP_BC_TRANSMISSION 0
+#define XDG_HIP_BC_VACUUM 1
+#define XDG_HIP_BC_REFLECTING 2
 
 /* Per-history status codes */
 #define XDG_HIP_TR_ABSORBED 1
@@ -285,6 +299,7 @@ typedef struct {
 #define XDG_HIP_TR_STUCK 4    /* hit at distance 0 (reference: fatal_error) */
 #define XDG_HIP_TR_CAPPED 5   /* max_events reached */
 #define XDG_HIP_TR_ERROR 6    /* topology error (volume not a parent, ...) */
+#define XDG_HIP_TR_VACUUM 7   /* crossed a vacuum surface */
 
 int xdg_hip_transport_create(xdg_hip_ctx* ctx, const xdg_hip_transport_problem* p, int32_t* handle);
 int xdg_hip_transport_destroy(xdg_hip_ctx* ctx, int32_t handle);

@@ -98,7 +98,11 @@ class TransportProblem(C.Structure):
                 ("sense_rev", C.POINTER(C.c_int32)), ("seed", C.c_uint32),
                 ("stream", C.c_uint32), ("lo", C.c_double * 3),
                 ("hi", C.c_double * 3), ("mfp", C.c_double),
-                ("p_absorb", C.c_double), ("max_events", C.c_int32)]
+                ("p_absorb", C.c_double), ("max_events", C.c_int32),
+                ("surf_bc", C.POINTER(C.c_uint8))]
+
+BC_TRANSMISSION, BC_VACUUM, BC_REFLECTING = 0, 1, 2
+TR_ABSORBED, TR_LEAKED, TR_LOST, TR_STUCK, TR_CAPPED, TR_ERROR, TR_VACUUM = range(1, 8)
 
 
 def _p(a, t):
@@ -382,13 +386,30 @@ def sense_tables(model):
     return fwd, rev
 
 
+def bc_table(model, bc):
+    """Per-surface-id boundary condition codes from {surface id: code}
+    (None: every surface transmits)."""
+    if not bc:
+        return None
+    ns = max(max(model.senses), max(bc)) + 1
+    t = np.zeros(ns, np.uint8)
+    for sid, code in bc.items():
+        t[sid] = code
+    return t
+
+
 def transport(model, trees, vol_ids, seed, stream, lo, hi, mfp, p_absorb,
-              max_events, first, n, nthreads=0):
+              max_events, first, n, nthreads=0, bc=None):
     """xo_transport over histories first .. first+n-1.  trees: Tree per
-    entry of vol_ids (find_volume order).  Returns tally_fx (int64 per
-    slot, scale 2^32), events, status, hash."""
+    entry of vol_ids (find_volume order).  bc: {surface id: BC_*} (others
+    transmit).  Returns tally_fx (int64 per slot, scale 2^32), events,
+    status, hash."""
     ids = _i32a(vol_ids)
     fwd, rev = sense_tables(model)
+    bct = bc_table(model, bc)
+    if bct is not None and bct.size > fwd.size:
+        pad = np.full(bct.size - fwd.size, -1, np.int32)
+        fwd, rev = np.concatenate([fwd, pad]), np.concatenate([rev, pad])
     arr = (_vp * len(trees))(*[t.h for t in trees])
     pr = TransportProblem()
     pr.nvol = len(trees)
@@ -405,6 +426,7 @@ def transport(model, trees, vol_ids, seed, stream, lo, hi, mfp, p_absorb,
     pr.mfp = float(mfp)
     pr.p_absorb = float(p_absorb)
     pr.max_events = int(max_events)
+    pr.surf_bc = _p(bct, _u8)
     tally = np.zeros(len(trees), np.int64)
     events = np.zeros(n, np.int32)
     status = np.zeros(n, np.uint8)

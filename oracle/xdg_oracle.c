@@ -634,6 +634,7 @@ typedef struct {
   double t;
   int32_t face, surf;
   double n[3];
+  int64_t prim; /* tree position of the committed triangle */
 } hit_state;
 
 /* Leaf test for ray_fire / point_in_volume: TriangleIntersectionFunc
@@ -669,6 +670,7 @@ static inline void leaf_intersect(const xo_tree* tr, int64_t p,
   h->face = face;
   h->surf = tr->surf[p];
   v_copy(n, h->n);
+  h->prim = p;
 }
 
 static void trace_one(const xo_tree* tr, const double* o, const double* d,
@@ -679,6 +681,7 @@ static void trace_one(const xo_tree* tr, const double* o, const double* d,
   h->face = INT32_MAX;
   h->surf = XO_ID_NONE;
   h->n[0] = h->n[1] = h->n[2] = 0.0;
+  h->prim = -1;
   if (brute || !tr->bvh) {
     for (int64_t p = 0; p < tr->n; ++p)
       leaf_intersect(tr, p, o, d, orientation, ex, k, h);
@@ -1500,9 +1503,38 @@ void xo_transport(const xo_transport_problem* p, uint64_t first, int64_t n,
         }
         hn = 0;
       } else {
-        /* cross_surface -> MeshManager::next_volume
-         * (mesh_manager_interface.cpp:202-214) */
+        /* cross_surface (particle_sim.h:85-130) */
         const int32_t sid = h.surf;
+        const int bc = (p->surf_bc && sid >= 0 && sid < p->nsurf_ids)
+                           ? p->surf_bc[sid]
+                           : XO_BC_TRANSMISSION;
+        if (bc == XO_BC_VACUUM) {
+          st = XO_TR_VACUUM;
+          break;
+        }
+        if (bc == XO_BC_REFLECTING) {
+          /* particle_sim.h:90-104 with XDG::surface_normal (xdg.cpp:272-290)
+           * = face_normal of the last history entry, the face just hit */
+          double nrm[3];
+          xo_face_normal(p->trees[slot]->v + 9 * h.prim, nrm);
+          const double proj = nrm[0] * u[0] + nrm[1] * u[1] + nrm[2] * u[2];
+          const double mag =
+              sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+          const double sc = 2.0 * proj / mag;
+          u[0] = u[0] - sc * nrm[0];
+          u[1] = u[1] - sc * nrm[1];
+          u[2] = u[2] - sc * nrm[2];
+          v_normalize(u);
+          hist[0] = h.face; /* history_ = {history_.back()} */
+          hn = 1;
+          if (ev >= p->max_events) {
+            st = XO_TR_CAPPED;
+            break;
+          }
+          continue;
+        }
+        /* transmission: MeshManager::next_volume
+         * (mesh_manager_interface.cpp:202-214) */
         int32_t nv = XO_ID_NONE;
         if (sid >= 0 && sid < p->nsurf_ids) {
           if (p->sense_fwd[sid] == vol)

@@ -228,8 +228,12 @@ double xo_log(double x);
  *              one block: dcol = -log(1 - ua) * mfp (xo_log); s = min(dcol,
  *              t_hit); r += s * u; tally[vol] += s; if dcol < t_hit:
  *              absorbed when ub < p_absorb, else new direction, history
- *              cleared; otherwise vol = next_volume(vol, surf), leaked when
- *              that is ipc.
+ *              cleared; otherwise the hit surface's boundary condition
+ *              (particle_sim.h:85-130): vacuum ends the history
+ *              (XO_TR_VACUUM); reflecting mirrors u about face_normal of the
+ *              hit face (u - n * (2 (n.u) / |n|), then normalize) and keeps
+ *              only that face in the history; transmission moves to
+ *              vol = next_volume(vol, surf), leaked when that is ipc.
  * Outputs per history: event count, status (XO_TR_*), sequence hash over
  * (vol, s) as in xo_segments.  tally_fx[slot] += llrint(s * 2^32) (int64,
  * order independent). */
@@ -241,6 +245,12 @@ double xo_log(double x);
 #define XO_TR_STUCK 4
 #define XO_TR_CAPPED 5
 #define XO_TR_ERROR 6
+#define XO_TR_VACUUM 7
+/* surface boundary conditions (MeshManager::get_surface_property,
+ * mesh_manager_interface.cpp:99-104; default "transmission") */
+#define XO_BC_TRANSMISSION 0
+#define XO_BC_VACUUM 1
+#define XO_BC_REFLECTING 2
 
 typedef struct {
   int nvol;                   /* volumes with surface trees, find order */
@@ -254,6 +264,8 @@ typedef struct {
   double lo[3], hi[3];
   double mfp, p_absorb;
   int32_t max_events;
+  const uint8_t* surf_bc;     /* per surface id (nsurf_ids), NULL = all
+                                 transmission */
 } xo_transport_problem;
 
 void xo_transport(const xo_transport_problem* p, uint64_t first, int64_t n,

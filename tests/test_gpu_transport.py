@@ -73,6 +73,31 @@ def test_transport_matches_oracle(ctx, lattice, prec):
         ctx.transport_destroy(h)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_transport_boundary_conditions(ctx, lattice, prec):
+    """Reflecting box and a mix of vacuum, reflecting and transmitting
+    sphere surfaces (particle_sim.h:85-130): every history and the tallies
+    equal the oracle's."""
+    m, vols, trees, otrees = lattice
+    box_sid = max(m.surfaces)
+    bc = {box_sid: 2}
+    for k, s in enumerate(sorted(x for x in m.surfaces if x != box_sid)):
+        bc[s] = (0, 1, 2)[k % 3]
+    h = ctx.transport_create(vols, trees, m.ipc, m.senses, SEED, STREAM,
+                             [0.0] * 3, [6.0] * 3, 2.0, 0.3, 10000, bc=bc)
+    try:
+        n, first = 20000, 77
+        r = O.transport(m, otrees, vols, SEED, STREAM, [0.0] * 3, [6.0] * 3,
+                        2.0, 0.3, 10000, first, n, bc=bc)
+        st = set(np.unique(r["status"]))
+        assert st <= {O.TR_ABSORBED, O.TR_VACUUM}
+        assert (r["status"] == O.TR_VACUUM).sum() > 1000
+        out = ctx.transport_run(h, first, n, len(vols), precision=prec)
+        _check(out, r)
+    finally:
+        ctx.transport_destroy(h)
+
+
 def test_transport_configs_and_autotune(ctx, lattice):
     """Block size, LDS stack depth (down to one entry, the rest spilling to
     global memory), claim chunk and refill threshold change nothing in the

@@ -17,6 +17,7 @@ import numpy as np
 
 import pyoracle as O
 from xdg_amd import geometry as G
+from xdg_amd import philox as P
 
 
 def _sphere_problem(mfp, p_absorb, n, first=0, nthreads=0):
@@ -70,3 +71,73 @@ def test_tallies_do_not_depend_on_threads():
     b = _sphere_problem(0.5, 0.3, 4000, first=123, nthreads=8)
     for k in a:
         assert np.array_equal(a[k], b[k])
+
+
+# ---- surface boundary conditions (particle_sim.h:85-130) -------------------
+
+def _unit_box_model():
+    v, f = G.box_surface([0.0] * 3, [1.0] * 3)
+    ts = np.ones(f.shape[0], np.int32)
+    return G.SurfaceModel(v, f, ts, [1], {1: (1, 2)}, [1, 2], ipc=2)
+
+
+def test_reflecting_box_unfolds_to_straight_lines():
+    """Specular reflection in the unit cube unfolds every history into one
+    straight line from the start along the start direction (mirror images
+    of the cube tile space).  With absorption at the first collision the
+    history's events are its wall hits plus one, and the wall hits are the
+    integer planes that the unfolded line of length T (the history's tally)
+    crosses: sum_k |floor(r0_k + T u0_k) - floor(r0_k)|.  A wrong mirror
+    formula bends the unfolded line and changes the counts."""
+    m = _unit_box_model()
+    tree = O.Tree.from_model(m, 1)
+    seed, stream, mfp = 21, 3, 2.5
+    r0 = P.box_points(seed, stream, 200, [0.0] * 3, [1.0] * 3)
+    u0 = P.iso_dirs(seed, stream, 200, base=2)
+    walls = []
+    for i in range(200):
+        r = O.transport(m, [tree], [1], seed, stream, [0.0] * 3, [1.0] * 3, mfp,
+                        1.0, 100000, i, 1, bc={1: O.BC_REFLECTING})
+        assert r["status"][0] == O.TR_ABSORBED
+        T = r["tally_fx"][0] / 2.0 ** 32
+        end = r0[i] + T * u0[i]
+        want = int(np.abs(np.floor(end) - np.floor(r0[i])).sum())
+        assert r["events"][0] == want + 1
+        walls.append(want)
+    assert np.mean(walls) > 2.0  # the check exercised many reflections
+
+
+def test_reflecting_sphere_keeps_every_particle():
+    """A reflecting outer surface keeps the medium infinite in effect: no
+    history leaks and the mean track per history is mfp / p."""
+    n, mfp, p = 20000, 4.0, 0.3
+    m = G.single_sphere(3, 10.0)
+    tree = O.Tree.from_model(m, 1)
+    r = O.transport(m, [tree], [1], 5, 9, [-2.0] * 3, [2.0] * 3, mfp, p,
+                    100000, 0, n, bc={1: O.BC_REFLECTING})
+    assert (r["status"] == O.TR_ABSORBED).all()
+    total = r["tally_fx"].sum() / 2.0 ** 32 / n
+    mean = mfp / p
+    assert abs(total - mean) < 4.0 * mean / np.sqrt(n)
+
+
+def test_vacuum_spheres_in_reflecting_box():
+    """Vacuum sphere surfaces end a history on entry, so a history that
+    starts outside the spheres never tallies inside one; the reflecting box
+    lets nothing leak."""
+    m = G.sphere_lattice(3, 2, 0.8, 2.0)
+    vols = [v for v in m.volumes if v != m.ipc]
+    trees = [O.Tree.from_model(m, v) for v in vols]
+    box_sid = max(m.surfaces)
+    bc = {s: O.BC_VACUUM for s in m.surfaces if s != box_sid}
+    bc[box_sid] = O.BC_REFLECTING
+    n = 5000
+    # starts in the slab 1.85 < x < 2.15 between the sphere layers (spheres
+    # span [0.2, 1.8] and [2.2, 3.8] in x): every history starts in the box
+    # region
+    r = O.transport(m, trees, vols, 11, 2, [1.85, 0.0, 0.0], [2.15, 6.0, 6.0],
+                    1.5, 0.2, 1000, 0, n, bc=bc)
+    st = r["status"]
+    assert set(np.unique(st)) <= {O.TR_ABSORBED, O.TR_VACUUM}
+    assert (st == O.TR_VACUUM).sum() > 0.1 * n
+    assert (r["tally_fx"][:-1] == 0).all() and r["tally_fx"][-1] > 0

@@ -60,11 +60,14 @@ class TransportProblem(C.Structure):
                 ("sense_rev", _vp), ("seed", C.c_uint32),
                 ("stream", C.c_uint32), ("lo", C.c_double * 3),
                 ("hi", C.c_double * 3), ("mfp", C.c_double),
-                ("p_absorb", C.c_double), ("max_events", _i32)]
+                ("p_absorb", C.c_double), ("max_events", _i32),
+                ("surf_bc", _vp)]
 
 
 # per-history transport status codes (XDG_HIP_TR_*)
-TR_ABSORBED, TR_LEAKED, TR_LOST, TR_STUCK, TR_CAPPED, TR_ERROR = 1, 2, 3, 4, 5, 6
+TR_ABSORBED, TR_LEAKED, TR_LOST, TR_STUCK, TR_CAPPED, TR_ERROR, TR_VACUUM = range(1, 8)
+# surface boundary conditions (XDG_HIP_BC_*)
+BC_TRANSMISSION, BC_VACUUM, BC_REFLECTING = 0, 1, 2
 
 _LIB = None
 
@@ -419,12 +422,18 @@ class Context:
 
     # ---- batched particle random walk ------------------------------------
     def transport_create(self, vol_ids, vol_trees, ipc, senses, seed, stream,
-                         lo, hi, mfp, p_absorb, max_events=10000):
+                         lo, hi, mfp, p_absorb, max_events=10000, bc=None):
         """Register a transport problem.  senses: {surface id: (forward
-        volume, reverse volume)}.  Returns the handle."""
+        volume, reverse volume)}; bc: {surface id: BC_*} (others transmit).
+        Returns the handle."""
         ids = np.ascontiguousarray(vol_ids, dtype=np.int32)
         trees = np.ascontiguousarray(vol_trees, dtype=np.int32)
-        ns = max(senses) + 1
+        ns = max(max(senses), max(bc) if bc else 0) + 1
+        bct = None
+        if bc:
+            bct = np.zeros(ns, np.uint8)
+            for sid, code in bc.items():
+                bct[sid] = code
         fwd = np.full(ns, -1, np.int32)
         rev = np.full(ns, -1, np.int32)
         for sid, (f, r) in senses.items():
@@ -445,6 +454,7 @@ class Context:
         p.mfp = float(mfp)
         p.p_absorb = float(p_absorb)
         p.max_events = int(max_events)
+        p.surf_bc = bct.ctypes.data if bct is not None else None
         h = _i32()
         self._check(lib().xdg_hip_transport_create(self.h, C.byref(p), C.byref(h)))
         return h.value

@@ -1144,6 +1144,14 @@ int xdg_hip_transport_create(xdg_hip_ctx* c, const xdg_hip_transport_problem* p,
       t->dev.mfp = p->mfp;
       t->dev.p_absorb = p->p_absorb;
       t->dev.max_events = p->max_events;
+      t->dev.surf_bc = nullptr;
+      if (p->surf_bc) {
+        for (int i = 0; i < p->nsurf_ids; ++i)
+          require(p->surf_bc[i] <= XDG_HIP_BC_REFLECTING, "unknown boundary condition code");
+        uint8_t* d = (uint8_t*)dev_alloc((size_t)p->nsurf_ids, t->bufs);
+        XDG_HIP_CHECK(hipMemcpy(d, p->surf_bc, (size_t)p->nsurf_ids, hipMemcpyHostToDevice));
+        t->dev.surf_bc = d;
+      }
       t->d_next = (unsigned long long*)dev_alloc(64, t->bufs);
       t->host_trees.assign(p->vol_trees, p->vol_trees + p->nvol);
       t->block = c->cfg.block > 256 ? 256 : c->cfg.block;

@@ -201,6 +201,7 @@ struct TransportDev {
   double lo[3], hi[3];
   double mfp, p_absorb;
   int32_t max_events;
+  const uint8_t* surf_bc;   // nullable: boundary condition per surface id [0, nsurf)
 };
 // One launch over histories [first, first + n).
 struct TransportRun {

@@ -103,6 +103,7 @@ struct HitState {
   int32_t face;
   int32_t surf;
   bool inside;  // PIV: dot(dir, sense-flipped normal) > 0 of the committed hit
+  int32_t leaf;  // tree position (leaf order) of the committed triangle
 };
 
 // Work counters of the instrumented traversal (roofline accounting): V node
@@ -188,6 +189,7 @@ __device__ __forceinline__ void tri_closest_hit(const TT& T, int32_t p, const Ra
     if (ex[j] == face) return;
   h.t = t;
   h.face = face;
+  h.leaf = p;
   h.surf = s2 >> TRI_SURF_SHIFT;
   if (MODE == PIV) h.inside = dot(ray.d, n) > 0.0;
 }

@@ -30,7 +30,8 @@ namespace xdgh {
 namespace {
 
 constexpr int HIST = 8;  // exclusion history: last hit faces since the last collision
-enum : uint8_t { TR_ALIVE = 0, TR_ABSORBED, TR_LEAKED, TR_LOST, TR_STUCK, TR_CAPPED, TR_ERROR };
+enum : uint8_t { TR_ALIVE = 0, TR_ABSORBED, TR_LEAKED, TR_LOST, TR_STUCK, TR_CAPPED, TR_ERROR, TR_VACUUM };
+enum : uint8_t { BC_TRANSMISSION = 0, BC_VACUUM = 1, BC_REFLECTING = 2 };
 
 // Marsaglia (1972) isotropic direction as tools/ray_benchmark.h:24-40: one
 // Philox block per attempt.
@@ -254,20 +255,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           hn = 0;
         }
       } else {
-        // cross_surface -> MeshManager::next_volume (mesh_manager_interface.cpp:202-214)
+        // cross_surface (particle_sim.h:85-130): the hit surface's boundary
+        // condition (MeshManager::get_surface_property, default transmission)
         const int32_t sid = h.surf;
-        int32_t nv = ID_NONE;
-        if (sid >= 0 && sid < P.nsurf) {
-          if (P.sense_fwd[sid] == vol)
-            nv = P.sense_rev[sid];
-          else if (P.sense_rev[sid] == vol)
-            nv = P.sense_fwd[sid];
-        }
-        if (nv == ID_NONE) {
-          st = TR_ERROR;
+        const bool known = sid >= 0 && sid < P.nsurf;
+        const uint8_t bc = (P.surf_bc && known) ? P.surf_bc[sid] : (uint8_t)BC_TRANSMISSION;
+        if (bc == BC_VACUUM) {
+          st = TR_VACUUM;
+        } else if (bc == BC_REFLECTING) {
+          // particle_sim.h:90-104; XDG::surface_normal (xdg.cpp:272-290) is
+          // face_normal of the last history entry, the face just hit, whose
+          // unflipped normal the tree stores in leaf order
+          const double* pn = T.normals + 3 * (int64_t)h.leaf;
+          const D3 nrm = d3(pn[0], pn[1], pn[2]);
+          const double proj = nrm.x * u.x + nrm.y * u.y + nrm.z * u.z;
+          const double mag = sqrt(nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z);
+          const double sc = 2.0 * proj / mag;
+          u = normalize(d3(u.x - sc * nrm.x, u.y - sc * nrm.y, u.z - sc * nrm.z));
+          hist[0] = h.face;  // history_ = {history_.back()}
+          hn = 1;
         } else {
-          vol = nv;
-          if (vol == P.ipc) st = TR_LEAKED;
+          // MeshManager::next_volume (mesh_manager_interface.cpp:202-214)
+          int32_t nv = ID_NONE;
+          if (known) {
+            if (P.sense_fwd[sid] == vol)
+              nv = P.sense_rev[sid];
+            else if (P.sense_rev[sid] == vol)
+              nv = P.sense_fwd[sid];
+          }
+          if (nv == ID_NONE) {
+            st = TR_ERROR;
+          } else {
+            vol = nv;
+            if (vol == P.ipc) st = TR_LEAKED;
+          }
         }
       }
       if (st == TR_ALIVE && ev >= P.max_events) st = TR_CAPPED;
