@@ -183,6 +183,30 @@ int xdg_hip_find_volume(xdg_hip_ctx* ctx, int nvol, const int32_t* vol_trees, co
                         int32_t ipc_volume, int64_t n, const double* pts, const double* dir, int precision,
                         int32_t* vol_out, void* stream);
 
+/* ---- overlap check (tools/overlap.cpp) ------------------------------------
+ * check_location_for_overlap (overlap.cpp:9-61) asks point_in_volume of
+ * every volume of the model at each location.  For each location i (device
+ * pts[3n], dir[3n]: its own ray direction, no exclusions):
+ * count_out[i] = number of volumes in the list whose point_in_volume is
+ * true, ids_out[max_k*i ...] = the first max_k of them (volume ids, list
+ * order).  vol_trees/vol_ids are host arrays of nvol entries (the model's
+ * volumes, the implicit complement included as MeshManager::volumes() has
+ * it). */
+int xdg_hip_point_in_volumes(xdg_hip_ctx* ctx, int nvol, const int32_t* vol_trees, const int32_t* vol_ids, int64_t n,
+                             const double* pts, const double* dir, int max_k, int precision, int32_t* count_out,
+                             int32_t* ids_out, void* stream);
+
+/* check_along_edge (overlap.cpp:229-265): for each edge ray i (device
+ * org[3n], dir[3n], len[n]), XDG::ray_fire(vol, org, dir, len) (EXITING, no
+ * exclusions) over the volumes of the list in order, skipping the two
+ * volume ids skip[2i], skip[2i+1] (the parents of the edge's surface); the
+ * first volume with a hit ends the search.  Outputs: t_out (INFTY when no
+ * volume is hit), surf_out (hit surface or ID_NONE), vol_out (the volume
+ * whose tree was hit, or ID_NONE). */
+int xdg_hip_edge_rays(xdg_hip_ctx* ctx, int nvol, const int32_t* vol_trees, const int32_t* vol_ids, int64_t n,
+                      const double* org, const double* dir, const double* len, const int32_t* skip, int precision,
+                      double* t_out, int32_t* surf_out, int32_t* vol_out, void* stream);
+
 /* RayTracer::occluded (ray_tracing_interface.h:112-115; Embree
  * ray_tracer.cpp:347-370).  dist_out (nullable): -INFTY on hit, INFTY on
  * miss, as the reference's `dist` argument. */

@@ -454,3 +454,53 @@ def adjoint(verts, tris, org, dirs, t, face, dldt):
                      _p(org, _d), _p(dirs, _d), _p(t, _d), _p(face, _i32),
                      _p(dldt, _d), _p(gv, _d), _p(go, _d), _p(gd, _d))
     return gv, go, gd
+
+
+# ---- overlap check (tools/overlap.cpp) ----------------------------------------
+
+class OverlapQueries:
+    """The overlap check's two batched queries composed from the oracle's
+    point_in_volume and ray_fire (test infrastructure: the checker of
+    xdg_hip_point_in_volumes / xdg_hip_edge_rays).  trees = {volume id:
+    Tree} for every volume of the model, in model order."""
+
+    def __init__(self, model, trees, max_k=4):
+        self.vols = [v for v in model.volumes if v in trees]
+        self.trees = [trees[v] for v in self.vols]
+        self.max_k = max_k
+
+    def inside_matrix(self, pts, dirs):
+        """(n, nvol) booleans: point_in_volume of each location."""
+        return np.stack([t.point_in_volume(pts, dirs) for t in self.trees], 1)
+
+    def point_in_volumes(self, pts, dirs):
+        m = self.inside_matrix(pts, dirs)
+        cnt = m.sum(1).astype(np.int32)
+        ids = np.full((pts.shape[0], self.max_k), -1, np.int32)
+        full = {}
+        vols = np.asarray(self.vols, np.int32)
+        for i in np.nonzero(cnt)[0]:
+            s = vols[m[i]]
+            ids[i, :min(cnt[i], self.max_k)] = s[:self.max_k]
+            if cnt[i] > self.max_k:
+                full[int(i)] = s
+        return cnt, ids, full
+
+    def edge_rays(self, org, dirs, length, skip):
+        n = org.shape[0]
+        t = np.full(n, np.finfo(np.float64).max)
+        surf = np.full(n, -1, np.int32)
+        vol = np.full(n, -1, np.int32)
+        todo = np.ones(n, bool)
+        for v, tr in zip(self.vols, self.trees):
+            sel = np.nonzero(todo & (skip[:, 0] != v) & (skip[:, 1] != v))[0]
+            if sel.size == 0:
+                continue
+            tt, ss, _ = tr.ray_fire(org[sel], dirs[sel], tmax=length[sel],
+                                    orientation=0)
+            hit = ss >= 0
+            t[sel[hit]] = tt[hit]
+            surf[sel[hit]] = ss[hit]
+            vol[sel[hit]] = v
+            todo[sel[hit]] = False
+        return t, surf, vol

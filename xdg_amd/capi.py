@@ -107,6 +107,11 @@ def lib():
                                     _vp, C.c_int, _vp, _vp],
             "xdg_hip_occluded": [_vp, _i32, _i64, _vp, _vp, C.c_int, _vp, _vp,
                                  _vp],
+            "xdg_hip_point_in_volumes": [_vp, C.c_int, _vp, _vp, _i64, _vp,
+                                         _vp, C.c_int, C.c_int, _vp, _vp,
+                                         _vp],
+            "xdg_hip_edge_rays": [_vp, C.c_int, _vp, _vp, _i64, _vp, _vp, _vp,
+                                  _vp, C.c_int, _vp, _vp, _vp, _vp],
             "xdg_hip_closest": [_vp, _i32, _i64, _vp, _vp, _vp, _vp],
             "xdg_hip_find_element": [_vp, _i32, _i64, _vp, _vp, _vp],
             "xdg_hip_next_element": [_vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp,
@@ -352,6 +357,39 @@ class Context:
             self.h, vt.size, vtp, vip, ipc, n, _ptr(pts), _ptr(dirs), precision,
             _ptr(out), stream), stream)
         return out
+
+    @_ordered
+    def point_in_volumes(self, vol_trees, vol_ids, pts, dirs, max_k=4,
+                         precision=PLAIN, stream=None):
+        """point_in_volume of every point against every listed volume
+        (tools/overlap.cpp:9-61).  Returns (count (n,), ids (n, max_k))."""
+        n = pts.shape[0]
+        cnt = torch.empty(n, dtype=torch.int32, device=pts.device)
+        ids = torch.full((n, max_k), -1, dtype=torch.int32, device=pts.device)
+        vt, vtp = _host(vol_trees, np.int32)
+        vi, vip = _host(vol_ids, np.int32)
+        self._done(lib().xdg_hip_point_in_volumes(
+            self.h, vt.size, vtp, vip, n, _ptr(pts), _ptr(dirs), max_k,
+            precision, _ptr(cnt), _ptr(ids), stream), stream)
+        return cnt, ids
+
+    @_ordered
+    def edge_rays(self, vol_trees, vol_ids, org, dirs, length, skip,
+                  precision=PLAIN, stream=None):
+        """First EXITING hit within length over the listed volumes, skipping
+        two volume ids per ray (tools/overlap.cpp:229-265).  Returns
+        (t, surface, volume)."""
+        n = org.shape[0]
+        t = torch.empty(n, dtype=torch.float64, device=org.device)
+        surf = torch.empty(n, dtype=torch.int32, device=org.device)
+        vol = torch.empty(n, dtype=torch.int32, device=org.device)
+        vt, vtp = _host(vol_trees, np.int32)
+        vi, vip = _host(vol_ids, np.int32)
+        self._done(lib().xdg_hip_edge_rays(
+            self.h, vt.size, vtp, vip, n, _ptr(org), _ptr(dirs), _ptr(length),
+            _ptr(skip), precision, _ptr(t), _ptr(surf), _ptr(vol), stream),
+            stream)
+        return t, surf, vol
 
     @_ordered
     def occluded(self, tree, org, dirs, precision=PLAIN, stream=None):

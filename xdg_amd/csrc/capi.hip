@@ -761,6 +761,28 @@ int xdg_hip_point_in_volume(xdg_hip_ctx* c, int32_t tree, int64_t n, const doubl
   });
 }
 
+// Device copy of a (trees, ids) volume list, cached across calls with the
+// same list; a changed list waits for queued work that may still read the
+// old one.
+static const int32_t* volume_lists(xdg_hip_ctx* c, int nvol, const int32_t* vol_trees, const int32_t* vol_ids)
+{
+  std::vector<int32_t> lists(vol_trees, vol_trees + nvol);
+  lists.insert(lists.end(), vol_ids, vol_ids + nvol);
+  if (lists != c->small_cache) {
+    XDG_HIP_CHECK(hipDeviceSynchronize());
+    if (lists.size() > c->small_cap) {
+      if (c->d_small) XDG_HIP_CHECK(hipFree(c->d_small));
+      c->d_small = nullptr;
+      c->small_cap = std::max<size_t>(lists.size(), 64);
+      XDG_HIP_CHECK(hipMalloc(&c->d_small, c->small_cap * sizeof(int32_t)));
+    }
+    if (!lists.empty())
+      XDG_HIP_CHECK(hipMemcpy(c->d_small, lists.data(), lists.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    c->small_cache = lists;
+  }
+  return c->d_small;
+}
+
 int xdg_hip_find_volume(xdg_hip_ctx* c, int nvol, const int32_t* vol_trees, const int32_t* vol_ids,
                         int32_t ipc_volume, int64_t n, const double* pts, const double* dir, int precision,
                         int32_t* vol_out, void* stream)
@@ -776,21 +798,8 @@ int xdg_hip_find_volume(xdg_hip_ctx* c, int nvol, const int32_t* vol_trees, cons
     }
     if (n == 0) return XDG_HIP_OK;
     require(pts && vol_out, "null array");
-    std::vector<int32_t> lists(vol_trees, vol_trees + nvol);
-    lists.insert(lists.end(), vol_ids, vol_ids + nvol);
     hipStream_t s = pick(c, stream);
-    if (lists != c->small_cache) {
-      XDG_HIP_CHECK(hipDeviceSynchronize());
-      if (lists.size() > c->small_cap) {
-        if (c->d_small) XDG_HIP_CHECK(hipFree(c->d_small));
-        c->d_small = nullptr;
-        c->small_cap = std::max<size_t>(lists.size(), 64);
-        XDG_HIP_CHECK(hipMalloc(&c->d_small, c->small_cap * sizeof(int32_t)));
-      }
-      if (!lists.empty())
-        XDG_HIP_CHECK(hipMemcpy(c->d_small, lists.data(), lists.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-      c->small_cache = lists;
-    }
+    volume_lists(c, nvol, vol_trees, vol_ids);
     FindVolumeArgs a{};
     a.table = c->d_surf;
     a.ntrees = (int32_t)c->surf.size();
@@ -811,6 +820,71 @@ int xdg_hip_find_volume(xdg_hip_ctx* c, int nvol, const int32_t* vol_trees, cons
       a.fb_count = c->d_fb_count;
     }
     launch_find_volume(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), s);
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_point_in_volumes(xdg_hip_ctx* c, int nvol, const int32_t* vol_trees, const int32_t* vol_ids, int64_t n,
+                             const double* pts, const double* dir, int max_k, int precision, int32_t* count_out,
+                             int32_t* ids_out, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0 && nvol >= 0, "negative size");
+    require(max_k >= 1, "max_k must be >= 1");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    require(nvol == 0 || (vol_trees && vol_ids), "null volume lists");
+    int depth = 1;
+    for (int v = 0; v < nvol; ++v) depth = std::max(depth, surf_need(c, &surf_tree(c, vol_trees[v])));
+    if (n == 0) return XDG_HIP_OK;
+    require(pts && dir && count_out && ids_out, "null array");
+    const int32_t* d_lists = volume_lists(c, nvol, vol_trees, vol_ids);
+    VolumesArgs a{};
+    a.table = c->d_surf;
+    a.ntrees = (int32_t)c->surf.size();
+    a.nvol = nvol;
+    a.vol_trees = d_lists;
+    a.vol_ids = d_lists + nvol;
+    a.n = n;
+    a.pts = pts;
+    a.dir = dir;
+    a.max_k = max_k;
+    a.count_out = count_out;
+    a.ids_out = ids_out;
+    launch_point_in_volumes(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_edge_rays(xdg_hip_ctx* c, int nvol, const int32_t* vol_trees, const int32_t* vol_ids, int64_t n,
+                      const double* org, const double* dir, const double* len, const int32_t* skip, int precision,
+                      double* t_out, int32_t* surf_out, int32_t* vol_out, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0 && nvol >= 0, "negative size");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    require(nvol == 0 || (vol_trees && vol_ids), "null volume lists");
+    int depth = 1;
+    for (int v = 0; v < nvol; ++v) depth = std::max(depth, surf_need(c, &surf_tree(c, vol_trees[v])));
+    if (n == 0) return XDG_HIP_OK;
+    require(org && dir && len && skip && t_out && surf_out && vol_out, "null array");
+    const int32_t* d_lists = volume_lists(c, nvol, vol_trees, vol_ids);
+    VolumesArgs a{};
+    a.table = c->d_surf;
+    a.ntrees = (int32_t)c->surf.size();
+    a.nvol = nvol;
+    a.vol_trees = d_lists;
+    a.vol_ids = d_lists + nvol;
+    a.n = n;
+    a.pts = org;
+    a.dir = dir;
+    a.len = len;
+    a.skip = skip;
+    a.t_out = t_out;
+    a.surf_out = surf_out;
+    a.vol_out = vol_out;
+    launch_edge_rays(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
   });

@@ -160,6 +160,31 @@ struct FindVolumeArgs {
 };
 void launch_find_volume(const FindVolumeArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
 
+// Overlap check (tools/overlap.cpp): every volume of a list, no early exit
+// for points; first hit over the list for edge rays.
+struct VolumesArgs {
+  const xdgd::SurfTreeDev* table;
+  int32_t ntrees;
+  int nvol;
+  const int32_t* vol_trees;  // device, nvol
+  const int32_t* vol_ids;    // device, nvol
+  int64_t n;
+  const double* pts;         // points / edge-ray origins
+  const double* dir;         // one direction per item
+  // point_in_volumes
+  int max_k;
+  int32_t* count_out;
+  int32_t* ids_out;          // max_k per point
+  // edge rays
+  const double* len;         // distance limit per ray
+  const int32_t* skip;       // two volume ids per ray that are not tested
+  double* t_out;
+  int32_t* surf_out;
+  int32_t* vol_out;
+};
+void launch_point_in_volumes(const VolumesArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
+void launch_edge_rays(const VolumesArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
+
 struct OccludedArgs {
   const xdgd::SurfTreeDev* table;
   int32_t ntrees;

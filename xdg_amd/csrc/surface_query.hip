@@ -726,6 +726,79 @@ void launch_find_volume(const FindVolumeArgs& a, bool mixed, const LaunchCfg& c,
   XDG_DISPATCH2(k_find_volume, mixed, c.wide, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
 }
 
+// check_location_for_overlap (tools/overlap.cpp:9-61): point_in_volume of
+// each location, with its own direction and no exclusions, against every
+// volume of the list in order (no early exit).  The first max_k volumes
+// that contain it are written, and their count.
+template <bool MIXED, bool WIDE>
+__global__ void k_point_in_volumes(VolumesArgs a, int cap)
+{
+  extern __shared__ int32_t lds[];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const RayD ray = make_ray(a.pts + 3 * i, a.dir + 3 * i);
+  int32_t* ids = a.ids_out + (int64_t)a.max_k * i;
+  int cnt = 0;
+  for (int v = 0; v < a.nvol; ++v) {
+    const int32_t tree = a.vol_trees[v];
+    if (tree < 1 || tree > a.ntrees) continue;
+    HitState h{INFTY, INT32_MAX, ID_NONE, false};
+    trace<PIV, MIXED, false, WIDE>(a.table[tree], ray, ORIENT_ANY, nullptr, 0, lds + threadIdx.x, blockDim.x, cap,
+                                   h);
+    if (h.surf != ID_NONE && h.inside) {
+      if (cnt < a.max_k) ids[cnt] = a.vol_ids[v];
+      ++cnt;
+    }
+  }
+  a.count_out[i] = cnt;
+}
+
+// check_along_edge (tools/overlap.cpp:229-265): XDG::ray_fire(vol, origin,
+// direction, edge length) (EXITING, no exclusions) against each volume of the
+// list in order except the ray's two skipped volumes (the parents of the
+// edge's surface); the first volume with a hit ends the search.
+template <bool MIXED, bool WIDE>
+__global__ void k_edge_rays(VolumesArgs a, int cap)
+{
+  extern __shared__ int32_t lds[];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const RayD ray = make_ray(a.pts + 3 * i, a.dir + 3 * i);
+  const int32_t s0 = a.skip[2 * i], s1 = a.skip[2 * i + 1];
+  const double lim = a.len[i];
+  double t = INFTY;
+  int32_t surf = ID_NONE, vol = ID_NONE;
+  for (int v = 0; v < a.nvol; ++v) {
+    const int32_t vid = a.vol_ids[v];
+    const int32_t tree = a.vol_trees[v];
+    if (vid == s0 || vid == s1 || tree < 1 || tree > a.ntrees) continue;
+    HitState h{lim, INT32_MAX, ID_NONE, false};
+    trace<RAYFIRE, MIXED, false, WIDE>(a.table[tree], ray, ORIENT_EXITING, nullptr, 0, lds + threadIdx.x, blockDim.x,
+                                       cap, h);
+    if (h.surf != ID_NONE) {
+      t = h.t;
+      surf = h.surf;
+      vol = vid;
+      break;
+    }
+  }
+  a.t_out[i] = t;
+  a.surf_out[i] = surf;
+  a.vol_out[i] = vol;
+}
+
+void launch_point_in_volumes(const VolumesArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)
+{
+  if (a.n <= 0) return;
+  XDG_DISPATCH2(k_point_in_volumes, mixed, c.wide, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+}
+
+void launch_edge_rays(const VolumesArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)
+{
+  if (a.n <= 0) return;
+  XDG_DISPATCH2(k_edge_rays, mixed, c.wide, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+}
+
 void launch_occluded(const OccludedArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)
 {
   if (a.n <= 0) return;

@@ -202,6 +202,56 @@ def _assemble(parts):
             np.concatenate(ts).astype(np.int32), np.concatenate(ss))
 
 
+def _two_volume_model(parts, senses) -> SurfaceModel:
+    verts, tris, ts = _assemble(parts)
+    surfs = sorted(senses)
+    return SurfaceModel(verts, tris, ts, surfs, dict(senses), [1, 2, 3], ipc=3)
+
+
+def overlapping_boxes() -> SurfaceModel:
+    """Two boxes that overlap (the analogue of overlap.h5m in the
+    reference's tests/test_overlap_check.cpp): A = [0,2] x [0,1.9] x [0,2.2]
+    and B = [1,3] x [1.3,2.75] x [0.7,3.1]; A's corner (2,1.9,2.2) lies inside
+    B and B's corner (1,1.3,0.7) inside A.  Volumes 1, 2, implicit
+    complement 3.  Side lengths are chosen so that no ray along +-(1,1,1)
+    from a vertex meets a box edge or corner."""
+    av, at = box_surface([0.0] * 3, [2.0, 1.9, 2.2])
+    bv, bt = box_surface([1.0, 1.3, 0.7], [3.0, 2.75, 3.1])
+    return _two_volume_model([(av, at, 1), (bv, bt, 2)], {1: (1, 3), 2: (2, 3)})
+
+
+def imprinted_boxes() -> SurfaceModel:
+    """Two boxes [0,1.1] x [0,0.7] x [0,0.45] and [1.1,2.3] x [0,0.7] x
+    [0,0.45] sharing the face x = 1.1 as one surface with senses (1, 2)
+    (no_overlap_imp.h5m): no overlap."""
+    av, at = box_surface([0.0] * 3, [1.1, 0.7, 0.45])
+    bv, bt = box_surface([1.1, 0.0, 0.0], [2.3, 0.7, 0.45])
+    # box_surface rows 8,9: x = lo; rows 10,11: x = hi
+    a_out = at[np.r_[0:10]]
+    shared = at[10:12]
+    b_out = np.concatenate([bt[0:8], bt[10:12]])
+    return _two_volume_model([(av, a_out, 1), (av, shared, 2), (bv, b_out, 3)],
+                             {1: (1, 3), 2: (1, 2), 3: (2, 3)})
+
+
+def enclosed_boxes() -> SurfaceModel:
+    """A box [1.05,2] x [1.2,2.13] x [0.9,1.61] (volume 2) inside a box
+    [0,4] x [0,3.7] x [0,4.4] (volume 1) whose surface does not bound it
+    (enclosed.h5m): volume 2 overlaps volume 1."""
+    av, at = box_surface([0.0] * 3, [4.0, 3.7, 4.4])
+    bv, bt = box_surface([1.05, 1.2, 0.9], [2.0, 2.13, 1.61])
+    return _two_volume_model([(av, at, 1), (bv, bt, 2)], {1: (1, 3), 2: (2, 3)})
+
+
+def crossed_bars() -> SurfaceModel:
+    """Two bars crossing like a plus sign (overlap-edge.h5m): A = [-2,2] x
+    [-0.5,0.5] x [-0.1,0.1], B = [-0.5,0.5] x [-2,2] x [-0.2,0.2].  No vertex
+    of one lies inside the other, but A's long edges pass through B."""
+    av, at = box_surface([-2.0, -0.5, -0.1], [2.0, 0.5, 0.1])
+    bv, bt = box_surface([-0.5, -2.0, -0.2], [0.5, 2.0, 0.2])
+    return _two_volume_model([(av, at, 1), (bv, bt, 2)], {1: (1, 3), 2: (2, 3)})
+
+
 def single_sphere(subdiv: int = 5, radius: float = 10.0) -> SurfaceModel:
     """One closed icosphere volume (id 1) plus its implicit complement (2)."""
     v, t = icosphere(subdiv, radius)
