@@ -449,6 +449,30 @@ def test_find_volume_far_points_and_variants(ctx, torch_mod, persist):
         ctx.set_traversal()
 
 
+def test_global_surface_tree(ctx, torch_mod):
+    """A tree over every surface, never flipped (create_global_surface_tree,
+    ray_tracer.cpp:190-206): ray_fire in each orientation and
+    point_in_volume equal the oracle on the same tree."""
+    torch = torch_mod
+    m = G.nested_spheres(3)
+    v9, f, s, fl, bump = m.global_tree_arrays()
+    n = v9.shape[0]
+    gt = ctx.build_surface_tree(v9.reshape(-1, 3), np.arange(3 * n, dtype=np.int32).reshape(-1, 3),
+                                f, s, fl, bump)
+    ot = O.Tree(v9, f, s, fl, bump)
+    pts = P.box_points(12, 0, 20000, [-18.0] * 3, [18.0] * 3)
+    dirs = P.iso_dirs(12, 1, 20000)
+    for orient in (-1, 0, 1):
+        for prec in (0, 1):
+            t, su, fa = ctx.ray_fire(gt, dev(torch, pts), dev(torch, dirs), orientation=orient,
+                                     precision=prec)
+            rt, rs, rf = ot.ray_fire(pts, dirs, orientation=orient)
+            assert np.array_equal(host(su), rs) and np.array_equal(host(fa), rf)
+            assert np.array_equal(bits(host(t)), bits(rt))
+    ins = host(ctx.point_in_volume(gt, dev(torch, pts), dev(torch, dirs))).astype(bool)
+    assert np.array_equal(ins, ot.point_in_volume(pts, dirs))
+
+
 def _segments_setup(ctx, mesh):
     et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
     bm = mesh.boundary_model()

@@ -89,6 +89,18 @@ class SurfaceModel:
             self.verts[self.tris[faces]].reshape(-1, 9), dtype=np.float64)
         return v9, faces.astype(np.int32), surfs, flips
 
+    def global_tree_arrays(self):
+        """Triangles of every surface once, never flipped
+        (EmbreeRayTracer::create_global_surface_tree, ray_tracer.cpp:190-206;
+        the global tree is no volume's reverse tree), with the largest
+        volume bump for the boxes."""
+        faces = [self.surface_faces(s) for s in self.surfaces]
+        faces = np.concatenate(faces) if faces else np.zeros(0, np.int32)
+        v9 = np.ascontiguousarray(self.verts[self.tris[faces]].reshape(-1, 9), dtype=np.float64)
+        bump = max([self.bump(v) for v in self.volumes if v != self.ipc] or [1e-3])
+        return (v9, faces.astype(np.int32), self.tri_surf[faces].astype(np.int32),
+                np.zeros(faces.size, np.uint8), bump)
+
     def volume_bbox(self, vol: int):
         """MeshManager::volume_bounding_box (mesh_manager_interface.cpp:237-246):
         starts from an all-zero box, so it always contains the origin."""
