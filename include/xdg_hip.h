@@ -256,6 +256,27 @@ int xdg_hip_tally_segments_volume(xdg_hip_ctx* ctx, int32_t element_tree, int32_
                                   uint64_t* hash_out, const int64_t* seq_off, int32_t* seq_elem, double* seq_len,
                                   void* stream);
 
+/* The element-by-element particle walk of tools/walk_elements.h:53-117
+ * (walk_elements), one particle per lane, particles [first, first + n).
+ * Start: points uniform in [lo, hi] (host arrays of 3) until find_element
+ * finds an element (1000 tries), an isotropic direction, normalized.  Each
+ * event: MeshManager::next_element, a collision distance -log(1-u)*mfp; a
+ * collision scatters isotropically, otherwise the particle moves into the
+ * next element; a particle leaving the mesh fires at the implicit
+ * complement (EXITING, that event's hit faces excluded) and re-enters at
+ * find_element(hit + u*TINY_BIT), or ends.  Random numbers: Philox4x32-10,
+ * key (seed, stream_id), counter (particle, 0, draw, 0), in place of the
+ * tool's drand48 and rand_dir.  Nullable device outputs per particle:
+ * distance travelled (f64, re-entry flights included), events (i32),
+ * status (XDG_HIP_TR_LEAKED: left the mesh; XDG_HIP_TR_CAPPED: max_events
+ * reached; XDG_HIP_TR_LOST: more than 64 re-entry rays in one event;
+ * XDG_HIP_TR_ERROR: no start found), a hash over the (element, length)
+ * pieces; tally_fx (nullable, per element) += llrint(len * 2^40). */
+int xdg_hip_mesh_walk(xdg_hip_ctx* ctx, int32_t element_tree, uint64_t first, int64_t n, uint32_t seed,
+                      uint32_t stream_id, const double* lo, const double* hi, double mfp, int32_t max_events,
+                      double* distance_out, int32_t* events_out, uint8_t* status_out, uint64_t* hash_out,
+                      int64_t* tally_fx, void* stream);
+
 /* Synchronize the device, return and clear the status word (bit 0: a walk of
  * the segment calls hit its step bound since the last call; bit 1: a
  * volume segment found no element behind its entry hit). */

@@ -73,6 +73,10 @@ def lib():
         L.xo_segments_volume.argtypes = [_vp, _vp, C.c_int64, _d, _d, C.c_int,
                                          _i32, _u64, _i64, _i64, _i32, _d,
                                          C.POINTER(C.c_uint32)]
+        L.xo_mesh_walk.argtypes = [_vp, _vp, _d, _d, C.c_uint32, C.c_uint32,
+                                   C.c_double, C.c_int32, C.c_uint64,
+                                   C.c_int64, C.c_int, _d, _i32, _u8, _u64,
+                                   _i64]
         L.xo_mix64.restype = C.c_uint64
         L.xo_mix64.argtypes = [C.c_uint64]
         L.xo_philox4x32_10.argtypes = [C.POINTER(C.c_uint32),
@@ -340,6 +344,26 @@ class TetMesh:
             out.append(tal)
         if sequences is not None:
             out.append((off, se, sl))
+        return tuple(out)
+
+    def mesh_walk(self, first, n, seed, stream, lo, hi, mfp, max_events=100000,
+                  tally=False, nthreads=0):
+        """xo_mesh_walk (tools/walk_elements.h:53-117): per particle
+        distance, events, status, hash[, per-element tally_fx]."""
+        dist = np.zeros(n)
+        ev = np.zeros(n, np.int32)
+        st = np.zeros(n, np.uint8)
+        hs = np.zeros(n, np.uint64)
+        tal = np.zeros(self.tets.shape[0], np.int64) if tally else None
+        lo_a, hi_a = _f64(lo), _f64(hi)
+        lib().xo_mesh_walk(self.h, self.ipc.h if self.ipc else None,
+                           _p(lo_a, _d), _p(hi_a, _d), seed, stream, float(mfp),
+                           int(max_events), int(first), int(n), nthreads,
+                           _p(dist, _d), _p(ev, _i32), _p(st, _u8),
+                           _p(hs, _u64), _p(tal, _i64))
+        out = [dist, ev, st, hs]
+        if tally:
+            out.append(tal)
         return tuple(out)
 
     def segments_volume(self, vol_tree, p0, p1, tally=False, sequences=None,

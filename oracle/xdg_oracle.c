@@ -1618,3 +1618,117 @@ void xo_adjoint(int64_t nv, const double* verts, int64_t ntri,
     }
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* element-by-element particle walk (tools/walk_elements.h:53-117)          */
+/* ------------------------------------------------------------------------ */
+
+static inline void prim_push(int32_t* prims, int* n, int32_t face)
+{
+  if (*n == XO_TRANSPORT_HISTORY) {
+    for (int j = 1; j < XO_TRANSPORT_HISTORY; ++j) prims[j - 1] = prims[j];
+    *n -= 1;
+  }
+  prims[(*n)++] = face;
+}
+
+void xo_mesh_walk(const xo_tetmesh* m, const xo_tree* ipc, const double lo[3],
+                  const double hi[3], uint32_t seed, uint32_t stream,
+                  double mfp, int32_t max_events, uint64_t first, int64_t n,
+                  int nthreads, double* distance, int32_t* events,
+                  uint8_t* status, uint64_t* hash, int64_t* tally_fx)
+{
+  int nt = resolve_threads(nthreads);
+  (void)nt;
+  OMP_FOR
+  for (int64_t i = 0; i < n; ++i) {
+    const uint64_t item = first + (uint64_t)i;
+    uint32_t dc = 0;
+    uint8_t st = XO_TR_ALIVE;
+    double r[3], u[3];
+    int32_t e = XO_ID_NONE;
+    /* sample a location within the model (walk_elements.h:64-68) */
+    for (int a = 0; a < XO_MESH_WALK_STARTS && e == XO_ID_NONE; ++a) {
+      double ua, ub, uc, unused;
+      xo_uniform2(seed, stream, item, dc++, &ua, &ub);
+      xo_uniform2(seed, stream, item, dc++, &uc, &unused);
+      r[0] = lo[0] + (hi[0] - lo[0]) * ua;
+      r[1] = lo[1] + (hi[1] - lo[1]) * ub;
+      r[2] = lo[2] + (hi[2] - lo[2]) * uc;
+      e = find_element_one(m, r, 0);
+    }
+    if (e == XO_ID_NONE) st = XO_TR_ERROR;
+    /* u = rand_dir(); u.normalize() (:70-71) */
+    if (st == XO_TR_ALIVE) {
+      if (!iso_dir_draw(seed, stream, item, &dc, u))
+        st = XO_TR_ERROR;
+      else
+        v_normalize(u);
+    }
+    double dist = 0.0;
+    int32_t nev = 0;
+    uint64_t hh = XO_HASH_SEED;
+    int32_t prims[XO_TRANSPORT_HISTORY];
+    int np = 0;
+    while (st == XO_TR_ALIVE && e != XO_ID_NONE) {
+      int32_t next;
+      double ex;
+      xo_next_element(m, e, r, u, &next, &ex);
+      double ua, ub;
+      xo_uniform2(seed, stream, item, dc++, &ua, &ub);
+      const double coll = -xo_log(1.0 - ua) * mfp;
+      double s;
+      const int32_t here = e;
+      if (coll < ex) {
+        s = coll;
+        r[0] = r[0] + u[0] * coll;
+        r[1] = r[1] + u[1] * coll;
+        r[2] = r[2] + u[2] * coll;
+        dist += coll;
+        if (!iso_dir_draw(seed, stream, item, &dc, u)) st = XO_TR_ERROR;
+      } else {
+        s = ex;
+        r[0] = r[0] + u[0] * ex;
+        r[1] = r[1] + u[1] * ex;
+        r[2] = r[2] + u[2] * ex;
+        dist += ex;
+        e = next;
+      }
+      hh = xo_mix64(hh ^ (uint64_t)(uint32_t)here);
+      hh = xo_mix64(hh ^ dbits(s));
+      if (tally_fx) {
+        long long q = llrint(ldexp(s, 40));
+#pragma omp atomic
+        tally_fx[here] += q;
+      }
+      /* re-entry through the implicit complement (:92-102) */
+      int tries = 0;
+      while (st == XO_TR_ALIVE && e == XO_ID_NONE && ipc) {
+        if (++tries > XO_MESH_WALK_REENTRIES) {
+          st = XO_TR_LOST;
+          break;
+        }
+        hit_state h;
+        trace_one(ipc, r, u, INFTY, XO_EXITING, prims, np, 0, &h);
+        if (h.surf == XO_ID_NONE) break;
+        prim_push(prims, &np, h.face); /* ray_tracer.cpp:324 */
+        r[0] = r[0] + u[0] * h.t;
+        r[1] = r[1] + u[1] * h.t;
+        r[2] = r[2] + u[2] * h.t;
+        dist += h.t;
+        double q[3] = {r[0] + u[0] * TINY_BIT, r[1] + u[1] * TINY_BIT,
+                       r[2] + u[2] * TINY_BIT};
+        e = find_element_one(m, q, 0);
+      }
+      np = 0;
+      ++nev;
+      if (st == XO_TR_ALIVE && e != XO_ID_NONE && nev >= max_events)
+        st = XO_TR_CAPPED;
+    }
+    if (st == XO_TR_ALIVE) st = XO_TR_LEAKED;
+    if (distance) distance[i] = dist;
+    if (events) events[i] = nev;
+    if (status) status[i] = st;
+    if (hash) hash[i] = hh;
+  }
+}

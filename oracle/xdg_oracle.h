@@ -272,6 +272,33 @@ void xo_transport(const xo_transport_problem* p, uint64_t first, int64_t n,
                   int nthreads, int64_t* tally_fx, int32_t* events,
                   uint8_t* status, uint64_t* hash);
 
+/* Element-by-element particle walk: tools/walk_elements.h:53-117 with
+ * Philox draws in place of drand48 / rand_dir.  Per particle (item = first
+ * + i, key (seed, stream), draw counter from 0):
+ *   start      blocks (2 per attempt): r = lo + (hi-lo)*(ua0, ub0, ua1) until
+ *              find_element(r) finds an element (at most
+ *              XO_MESH_WALK_STARTS attempts, else XO_TR_ERROR)
+ *   direction  Marsaglia rejection, one block per attempt, then
+ *              Vec3da::normalize
+ *   event      (next, exit) = next_element(e, r, u); one block: coll =
+ *              -log(1 - ua) * mfp; coll < exit: r += u*coll, new direction
+ *              (not normalized); else r += u*exit, e = next.  While e is
+ *              ID_NONE: ray_fire(ipc, r, u, INFTY, EXITING, prims) (the hit
+ *              face is appended), stop on a miss, else r += u*t and e =
+ *              find_element(r + u*TINY_BIT); prims cleared after the event.
+ * Outputs per particle: total distance, events, status (XO_TR_LEAKED: left
+ * the mesh; XO_TR_CAPPED at max_events; XO_TR_LOST: more than
+ * XO_MESH_WALK_REENTRIES re-entry rays in one event), a hash over the
+ * (element, length) pieces as in xo_segments; tally_fx (nullable) +=
+ * llrint(len * 2^40) per element. */
+#define XO_MESH_WALK_STARTS 1000
+#define XO_MESH_WALK_REENTRIES 64
+void xo_mesh_walk(const xo_tetmesh* m, const xo_tree* ipc, const double lo[3],
+                  const double hi[3], uint32_t seed, uint32_t stream,
+                  double mfp, int32_t max_events, uint64_t first, int64_t n,
+                  int nthreads, double* distance, int32_t* events,
+                  uint8_t* status, uint64_t* hash, int64_t* tally_fx);
+
 /* ---- adjoint of the ray distance ----------------------------------------- */
 
 /* For rays that hit triangle f = face[i] of the mesh (verts, tris) at

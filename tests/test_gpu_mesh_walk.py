@@ -1,0 +1,76 @@
+"""Element-by-element particle walk (xdg_hip_mesh_walk) vs the oracle's
+replay of the same Philox streams: distance bits, events, status, the
+(element, length) hash and per-element tallies, bit for bit."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpu_util import bits, build_tree, host
+from xdg_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+SEED, STREAM = 13, 4
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(torch_mod):
+    from xdg_amd import capi
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def _setup(ctx, mesh):
+    et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
+    bm = mesh.boundary_model()
+    st = build_tree(ctx, bm, bm.ipc)
+    ctx.attach_boundary(et, st, mesh.bface_elem)
+    return et
+
+
+CASES = {
+    "kuhn16": (lambda: G.kuhn_cube(16, 0.2, seed=4), [0.0] * 3, [1.0] * 3, 0.05),
+    "two_blocks": (lambda: G.two_blocks(8, 0.5), [0.0] * 3, [2.5, 1.0, 1.0], 0.3),
+    "mock": (G.mock_tetmesh, [-2.0, -3.0, -4.0], [5.0, 6.0, 7.0], 2.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mesh_walk_matches_oracle(ctx, torch_mod, name):
+    torch = torch_mod
+    make, lo, hi, mfp = CASES[name]
+    mesh = make()
+    et = _setup(ctx, mesh)
+    om = O.TetMesh(mesh)
+    n, first = 20000, 321
+    rd, re, rs, rh, rt = om.mesh_walk(first, n, SEED, STREAM, lo, hi, mfp, tally=True)
+    tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device="cuda")
+    d, e, s, h = ctx.mesh_walk(et, first, n, SEED, STREAM, lo, hi, mfp, tally=tally)
+    assert np.array_equal(host(s), rs)
+    assert np.array_equal(host(e), re)
+    assert np.array_equal(bits(host(d)), bits(rd))
+    assert np.array_equal(host(h).view(np.uint64), rh)
+    assert np.array_equal(host(tally), rt)
+    assert set(np.unique(rs)) == {O.TR_LEAKED}
+    assert re.mean() > 3
+
+
+def test_mesh_walk_event_cap(ctx, torch_mod):
+    mesh = G.kuhn_cube(8, 0.2, seed=4)
+    et = _setup(ctx, mesh)
+    om = O.TetMesh(mesh)
+    rd, re, rs, rh = om.mesh_walk(0, 4000, SEED, STREAM, [0.0] * 3, [1.0] * 3, 0.01, max_events=20)
+    d, e, s, h = ctx.mesh_walk(et, 0, 4000, SEED, STREAM, [0.0] * 3, [1.0] * 3, 0.01, max_events=20)
+    assert (rs == O.TR_CAPPED).sum() > 1000
+    assert np.array_equal(host(s), rs) and np.array_equal(host(e), re)
+    assert np.array_equal(bits(host(d)), bits(rd))
+    assert np.array_equal(host(h).view(np.uint64), rh)

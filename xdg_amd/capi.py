@@ -122,6 +122,9 @@ def lib():
                                               _vp, _vp, _vp, _vp, _vp, _vp,
                                               _vp],
             "xdg_hip_status": [_vp, C.POINTER(C.c_uint32)],
+            "xdg_hip_mesh_walk": [_vp, _i32, C.c_uint64, _i64, C.c_uint32,
+                                  C.c_uint32, _vp, _vp, C.c_double, _i32, _vp,
+                                  _vp, _vp, _vp, _vp, _vp],
             "xdg_hip_eval_primitive": [_vp, C.c_int, _i64, _vp, _vp, _vp],
             "xdg_hip_transport_create": [_vp, C.POINTER(TransportProblem),
                                          C.POINTER(_i32)],
@@ -457,6 +460,25 @@ class Context:
             _ptr(cnt), _ptr(hsh), _ptr(seq_off), _ptr(seq_elem),
             _ptr(seq_len), stream), stream)
         return cnt, hsh
+
+    @_ordered
+    def mesh_walk(self, etree, first, n, seed, stream_id, lo, hi, mfp,
+                  max_events=100000, tally=None, stream=None):
+        """Element-by-element particle walk (tools/walk_elements.h:53-117).
+        Returns (distance, events, status, hash) device tensors."""
+        dev = self.dev
+        dist = torch.empty(n, dtype=torch.float64, device=dev)
+        ev = torch.empty(n, dtype=torch.int32, device=dev)
+        st = torch.empty(n, dtype=torch.uint8, device=dev)
+        hs = torch.empty(n, dtype=torch.int64, device=dev)
+        lo_a = (C.c_double * 3)(*[float(x) for x in lo])
+        hi_a = (C.c_double * 3)(*[float(x) for x in hi])
+        self._done(lib().xdg_hip_mesh_walk(
+            self.h, etree, int(first), int(n), int(seed), int(stream_id),
+            C.cast(lo_a, _vp), C.cast(hi_a, _vp), float(mfp), int(max_events),
+            _ptr(dist), _ptr(ev), _ptr(st), _ptr(hs), _ptr(tally), stream),
+            stream)
+        return dist, ev, st, hs
 
     # ---- batched particle random walk ------------------------------------
     def transport_create(self, vol_ids, vol_trees, ipc, senses, seed, stream,

@@ -1022,6 +1022,48 @@ int xdg_hip_tally_segments_volume(xdg_hip_ctx* c, int32_t element_tree, int32_t 
   });
 }
 
+int xdg_hip_mesh_walk(xdg_hip_ctx* c, int32_t element_tree, uint64_t first, int64_t n, uint32_t seed,
+                      uint32_t stream_id, const double* lo, const double* hi, double mfp, int32_t max_events,
+                      double* distance_out, int32_t* events_out, uint8_t* status_out, uint64_t* hash_out,
+                      int64_t* tally_fx, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    require(lo && hi, "null start box");
+    for (int k = 0; k < 3; ++k) require(std::isfinite(lo[k]) && std::isfinite(hi[k]), "start box not finite");
+    require(std::isfinite(mfp) && mfp > 0.0, "mfp must be finite and > 0");
+    require(max_events >= 1, "max_events must be >= 1");
+    const ElemTreeHost& t = elem_tree(c, element_tree);
+    int depth = t.dev.depth;
+    if (t.dev.ipc_tree >= 1) depth = std::max(depth, surf_tree(c, t.dev.ipc_tree).dev.depth);
+    if (n == 0) return XDG_HIP_OK;
+    MeshWalkArgs a{};
+    a.etable = c->d_elem;
+    a.nelem_trees = (int32_t)c->elem.size();
+    a.stable = c->d_surf;
+    a.nsurf_trees = (int32_t)c->surf.size();
+    a.tree = element_tree;
+    a.first = first;
+    a.n = n;
+    a.seed = seed;
+    a.stream = stream_id;
+    for (int k = 0; k < 3; ++k) {
+      a.lo[k] = lo[k];
+      a.hi[k] = hi[k];
+    }
+    a.mfp = mfp;
+    a.max_events = max_events;
+    a.distance = distance_out;
+    a.events = events_out;
+    a.status = status_out;
+    a.hash = hash_out;
+    a.tally_fx = (unsigned long long*)tally_fx;
+    launch_mesh_walk(a, face_order_kind(t.dev.face_order), cfg_for(c, depth), pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
+    return XDG_HIP_OK;
+  });
+}
+
 int xdg_hip_status(xdg_hip_ctx* c, uint32_t* flags)
 {
   return guarded(c, [&]() {

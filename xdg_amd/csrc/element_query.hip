@@ -578,6 +578,114 @@ __global__ void k_segments_vol(SegmentsArgs a, int cap)
   if (flags && a.status) atomicOr(a.status, flags);
 }
 
+// tools/walk_elements.h:53-117 (walk_elements): one particle per lane.
+// Start: uniform samples in [lo, hi] until find_element finds an element;
+// isotropic direction, normalized.  Each event: next_element, an
+// exponential collision distance; a collision scatters isotropically, else
+// the particle moves to the next element; a particle that leaves the mesh
+// fires at the implicit complement (EXITING, the event's hit faces
+// excluded) and re-enters at the element behind the hit.  Philox draws
+// replace drand48 / rand_dir (the oracle replays the same streams).
+enum : uint8_t { MW_ALIVE = 0, MW_LOST = 3, MW_LEFT = 2, MW_CAPPED = 5, MW_ERROR = 6 };
+constexpr int MW_PRIMS = 8;        // re-entry exclusions kept per event
+constexpr int MW_STARTS = 1000;    // start samples tried per particle
+constexpr int MW_REENTRIES = 64;   // re-entry rays per event
+
+template <int FO>
+__global__ void k_mesh_walk(MeshWalkArgs a, int cap)
+{
+  extern __shared__ int32_t lds[];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  int32_t* stk = lds + threadIdx.x;
+  int32_t* prims = lds + (size_t)cap * blockDim.x + MW_PRIMS * threadIdx.x;
+  const ElemTreeDev T = a.etable[a.tree];
+  const bool has_ipc = T.ipc_tree >= 1 && T.ipc_tree <= a.nsurf_trees;
+  const uint64_t item = a.first + (uint64_t)i;
+  uint32_t dc = 0;
+  uint8_t st = MW_ALIVE;
+  D3 r = d3(0.0, 0.0, 0.0), u = d3(0.0, 0.0, 0.0);
+  int32_t e = ID_NONE;
+  // sample a location within the model (walk_elements.h:64-68)
+  for (int att = 0; att < MW_STARTS && e == ID_NONE; ++att) {
+    double ua, ub, uc, unused;
+    uniform2(a.seed, a.stream, item, dc++, ua, ub);
+    uniform2(a.seed, a.stream, item, dc++, uc, unused);
+    r = d3(a.lo[0] + (a.hi[0] - a.lo[0]) * ua, a.lo[1] + (a.hi[1] - a.lo[1]) * ub,
+           a.lo[2] + (a.hi[2] - a.lo[2]) * uc);
+    e = find_element_dev(T, r, stk, blockDim.x, cap);
+  }
+  if (e == ID_NONE) st = MW_ERROR;
+  // u = rand_dir(); u.normalize() (walk_elements.h:70-71)
+  if (st == MW_ALIVE) {
+    if (!iso_dir(a.seed, a.stream, item, dc, u))
+      st = MW_ERROR;
+    else
+      u = normalize(u);
+  }
+  double dist = 0.0;
+  int32_t nev = 0;
+  unsigned long long hh = HASH_SEED;
+  int np = 0;
+  while (st == MW_ALIVE && e != ID_NONE) {
+    const int idx = major_axis(u);
+    int32_t next;
+    double ex;
+    next_element_dev<FO>(T, e, r, u, idx, pick3(u.x, u.y, u.z, idx), next, ex);
+    double ua, ub;
+    uniform2(a.seed, a.stream, item, dc++, ua, ub);
+    const double coll = -log_series(1.0 - ua) * a.mfp;
+    const int32_t here = e;
+    const double s = coll < ex ? coll : ex;
+    r = d3(r.x + u.x * s, r.y + u.y * s, r.z + u.z * s);
+    dist += s;
+    if (coll < ex) {
+      if (!iso_dir(a.seed, a.stream, item, dc, u)) st = MW_ERROR;
+    } else {
+      e = next;
+    }
+    hh = mix64(hh ^ (unsigned long long)(uint32_t)here);
+    hh = mix64(hh ^ (unsigned long long)__double_as_longlong(s));
+    if (a.tally_fx) atomicAdd(a.tally_fx + here, (unsigned long long)__double2ll_rn(s * 1099511627776.0));
+    // re-entry through the implicit complement (walk_elements.h:92-102)
+    int tries = 0;
+    while (st == MW_ALIVE && e == ID_NONE && has_ipc) {
+      if (++tries > MW_REENTRIES) {
+        st = MW_LOST;
+        break;
+      }
+      RayD ray;
+      ray.o = r;
+      ray.d = u;
+      ray.rb = cross(u, r);
+      ray.idx = major_axis(u);
+      ray.oi = pick3(r.x, r.y, r.z, ray.idx);
+      ray.di = pick3(u.x, u.y, u.z, ray.idx);
+      HitState h{INFTY, INT32_MAX, ID_NONE, false};
+      trace<RAYFIRE, false>(a.stable[T.ipc_tree], ray, ORIENT_EXITING, prims, np, stk, blockDim.x, cap, h);
+      if (h.surf == ID_NONE) break;
+      // the hit face joins the exclusions (ray_tracer.cpp:324)
+      if (np == MW_PRIMS) {
+        for (int j = 1; j < MW_PRIMS; ++j) prims[j - 1] = prims[j];
+        --np;
+      }
+      prims[np++] = h.face;
+      r = d3(r.x + u.x * h.t, r.y + u.y * h.t, r.z + u.z * h.t);
+      dist += h.t;
+      e = find_element_dev(T, d3(r.x + u.x * TINY_BIT, r.y + u.y * TINY_BIT, r.z + u.z * TINY_BIT), stk,
+                           blockDim.x, cap);
+    }
+    np = 0;
+    ++nev;
+    if (st == MW_ALIVE && e != ID_NONE && nev >= a.max_events) st = MW_CAPPED;
+  }
+  if (st == MW_ALIVE) st = MW_LEFT;
+  if (a.distance) a.distance[i] = dist;
+  if (a.events) a.events[i] = nev;
+  if (a.status) a.status[i] = st;
+  if (a.hash) a.hash[i] = hh;
+}
+
 __global__ void k_elem_boxes(const ElemRec* __restrict__ elems, int64_t n, double* box6)
 {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -607,6 +715,18 @@ void launch_find_element(const FindElementArgs& a, const LaunchCfg& c, hipStream
 {
   if (a.n <= 0) return;
   hipLaunchKernelGGL(k_find_element, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+}
+
+void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s)
+{
+  if (a.n <= 0) return;
+  const size_t sh = sizeof(int32_t) * ((size_t)c.stack + MW_PRIMS) * c.block;
+  if (face_order_kind == FO_MOAB)
+    hipLaunchKernelGGL(k_mesh_walk<FO_MOAB>, grid1(a.n, c.block), dim3(c.block), sh, s, a, c.stack);
+  else if (face_order_kind == FO_MOCK)
+    hipLaunchKernelGGL(k_mesh_walk<FO_MOCK>, grid1(a.n, c.block), dim3(c.block), sh, s, a, c.stack);
+  else
+    hipLaunchKernelGGL(k_mesh_walk<FO_TABLE>, grid1(a.n, c.block), dim3(c.block), sh, s, a, c.stack);
 }
 
 void launch_next_element(const NextElementArgs& a, hipStream_t s)

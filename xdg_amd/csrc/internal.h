@@ -321,6 +321,27 @@ struct SegmentsArgs {
   int32_t surf_tree;
 };
 void launch_segments(const SegmentsArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s);
+
+// Element-by-element particle walk (tools/walk_elements.h:53-117).
+struct MeshWalkArgs {
+  const xdgd::ElemTreeDev* etable;
+  int32_t nelem_trees;
+  const xdgd::SurfTreeDev* stable;
+  int32_t nsurf_trees;
+  int32_t tree;
+  uint64_t first;
+  int64_t n;
+  uint32_t seed, stream;
+  double lo[3], hi[3];  // start sampling box
+  double mfp;
+  int32_t max_events;
+  double* distance;               // nullable outputs, one per particle
+  int32_t* events;
+  uint8_t* status;
+  uint64_t* hash;
+  unsigned long long* tally_fx;   // nullable, per element, scale 2^40
+};
+void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s);
 // 0 = MOAB canonical, 1 = MeshMock, 2 = other (read from the tree)
 int face_order_kind(const int32_t* face_order);
 

@@ -33,25 +33,6 @@ constexpr int HIST = 8;  // exclusion history: last hit faces since the last col
 enum : uint8_t { TR_ALIVE = 0, TR_ABSORBED, TR_LEAKED, TR_LOST, TR_STUCK, TR_CAPPED, TR_ERROR, TR_VACUUM };
 enum : uint8_t { BC_TRANSMISSION = 0, BC_VACUUM = 1, BC_REFLECTING = 2 };
 
-// Marsaglia (1972) isotropic direction as tools/ray_benchmark.h:24-40: one
-// Philox block per attempt.
-__device__ __forceinline__ bool iso_dir(uint32_t seed, uint32_t stream, uint64_t item, uint32_t& dc, D3& u)
-{
-  for (int a = 0; a < 1000; ++a) {
-    double ua, ub;
-    uniform2(seed, stream, item, dc++, ua, ub);
-    const double x1 = ua * 2.0 - 1.0;
-    const double x2 = ub * 2.0 - 1.0;
-    const double s = x1 * x1 + x2 * x2;
-    if (s > 0.0 && s < 1.0) {
-      const double t = 2.0 * sqrt(1.0 - s);
-      u = d3(x1 * t, x2 * t, 1.0 - 2.0 * s);
-      return true;
-    }
-  }
-  return false;
-}
-
 __device__ __forceinline__ RayD make_ray_d3(const D3& o, const D3& d)
 {
   RayD r;

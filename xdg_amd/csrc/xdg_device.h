@@ -379,6 +379,25 @@ __device__ __forceinline__ void uniform2(uint32_t seed, uint32_t stream, uint64_
   ub = (double)(b >> 11) * 0x1.0p-53;
 }
 
+// Marsaglia (1972) isotropic direction as tools/ray_benchmark.h:24-40: one
+// Philox block per attempt.
+__device__ __forceinline__ bool iso_dir(uint32_t seed, uint32_t stream, uint64_t item, uint32_t& dc, D3& u)
+{
+  for (int a = 0; a < 1000; ++a) {
+    double ua, ub;
+    uniform2(seed, stream, item, dc++, ua, ub);
+    const double x1 = ua * 2.0 - 1.0;
+    const double x2 = ub * 2.0 - 1.0;
+    const double s = x1 * x1 + x2 * x2;
+    if (s > 0.0 && s < 1.0) {
+      const double t = 2.0 * sqrt(1.0 - s);
+      u = d3(x1 * t, x2 * t, 1.0 - 2.0 * s);
+      return true;
+    }
+  }
+  return false;
+}
+
 // ln(x) for positive x: x = m 2^e with m in (sqrt(1/2), sqrt(2)];
 // ln(m) = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716, atanh series to
 // s^23 (Horner, plain IEEE ops); ln 2 split into a head with 32 significant

@@ -414,6 +414,23 @@ def tet_adjacency(tets: np.ndarray, face_order: np.ndarray, nv: int):
     return adj.reshape(ne, 4), bfaces, belem
 
 
+def two_blocks(n: int = 8, gap: float = 0.5, jitter: float = 0.2) -> TetMesh:
+    """Two Kuhn unit cubes, the second shifted by 1 + gap along x, as one
+    mesh: a particle that leaves the first block can re-enter the mesh in
+    the second through the implicit complement."""
+    a = kuhn_cube(n, jitter, seed=4)
+    b = kuhn_cube(n, jitter, seed=5)
+    nv, ne = a.verts.shape[0], a.tets.shape[0]
+    bv = b.verts.copy()
+    bv[:, 0] += 1.0 + gap
+    badj = np.where(b.adj >= 0, b.adj + ne, -1).astype(np.int32)
+    return TetMesh(np.concatenate([a.verts, bv]),
+                   np.concatenate([a.tets, b.tets + nv]).astype(np.int32),
+                   np.concatenate([a.adj, badj]).astype(np.int32), a.face_order,
+                   np.concatenate([a.bfaces, b.bfaces + nv]).astype(np.int32),
+                   np.concatenate([a.bface_elem, b.bface_elem + ne]).astype(np.int32))
+
+
 def tet_signed_volumes(verts, tets):
     p = verts[tets]
     e1 = p[:, 1] - p[:, 0]
