@@ -255,6 +255,29 @@ def bench_tet(args, ctx, torch, stream, sh):
            "algorithmic_gbs": round(seg_bytes * ns_ / avg_w / 1e9, 2),
            "capped": bool(status & 1)}
     log(f"tet walk {tet['msegs_s']:.2f} Msegs/s, {steps_per_seg:.1f} elements/segment")
+
+    # element-by-element particle walk (tools/walk_elements.h) on the same
+    # mesh: 2^20 particles, mean free path 0.1, isotropic scattering
+    npart, mfp = 1 << 20, 0.1
+    mw_tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device=dev)
+    mw_out = {}
+
+    def mwalk():
+        mw_out["r"] = ctx.mesh_walk(et, 0, npart, 6, 0, [0.0] * 3, [1.0] * 3, mfp, tally=mw_tally,
+                                    stream=sh)
+
+    per_m, _ = time_launches(torch, stream, mwalk, max(2, args.steps // 5), 1)
+    avg_m = sum(per_m) / len(per_m)
+    dist, ev, st, _ = mw_out["r"]
+    events = ev.double().sum().item()
+    tet["mesh_walk"] = {"mparticles_s": round(npart / avg_m / 1e6, 3),
+                        "ms_per_launch": round(avg_m * 1e3, 4), "particles": npart,
+                        "mfp": mfp, "events_per_particle": round(events / npart, 3),
+                        "events_per_s": events / avg_m,
+                        "left_mesh": int((st == 2).sum().item()),
+                        "mean_distance": round(dist.mean().item(), 6)}
+    log(f"mesh walk {tet['mesh_walk']['mparticles_s']:.2f} Mparticles/s, "
+        f"{events / npart:.1f} events/particle")
     return tet
 
 

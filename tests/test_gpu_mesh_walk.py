@@ -45,7 +45,8 @@ CASES = {
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_mesh_walk_matches_oracle(ctx, torch_mod, name):
+@pytest.mark.parametrize("persist", [True, False])
+def test_mesh_walk_matches_oracle(ctx, torch_mod, name, persist):
     torch = torch_mod
     make, lo, hi, mfp = CASES[name]
     mesh = make()
@@ -54,7 +55,11 @@ def test_mesh_walk_matches_oracle(ctx, torch_mod, name):
     n, first = 20000, 321
     rd, re, rs, rh, rt = om.mesh_walk(first, n, SEED, STREAM, lo, hi, mfp, tally=True)
     tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device="cuda")
-    d, e, s, h = ctx.mesh_walk(et, first, n, SEED, STREAM, lo, hi, mfp, tally=tally)
+    ctx.set_traversal(persistent=persist)
+    try:
+        d, e, s, h = ctx.mesh_walk(et, first, n, SEED, STREAM, lo, hi, mfp, tally=tally)
+    finally:
+        ctx.set_traversal()
     assert np.array_equal(host(s), rs)
     assert np.array_equal(host(e), re)
     assert np.array_equal(bits(host(d)), bits(rd))
@@ -62,6 +67,24 @@ def test_mesh_walk_matches_oracle(ctx, torch_mod, name):
     assert np.array_equal(host(tally), rt)
     assert set(np.unique(rs)) == {O.TR_LEAKED}
     assert re.mean() > 3
+
+
+@pytest.mark.parametrize("refill", [1, 17, 64])
+def test_mesh_walk_refill_thresholds(ctx, torch_mod, refill):
+    """The resident-grid walk gives the same particles for any refill
+    threshold, including a batch smaller than one wave."""
+    mesh = G.kuhn_cube(8, 0.2, seed=4)
+    et = _setup(ctx, mesh)
+    om = O.TetMesh(mesh)
+    ctx.set_option("refill", refill)
+    try:
+        for n in (37, 5000):
+            rd, re, rs, rh = om.mesh_walk(9, n, SEED, STREAM, [0.0] * 3, [1.0] * 3, 0.05)
+            d, e, s, h = ctx.mesh_walk(et, 9, n, SEED, STREAM, [0.0] * 3, [1.0] * 3, 0.05)
+            assert np.array_equal(host(e), re) and np.array_equal(bits(host(d)), bits(rd))
+            assert np.array_equal(host(h).view(np.uint64), rh) and np.array_equal(host(s), rs)
+    finally:
+        ctx.set_option("refill", 40)
 
 
 def test_mesh_walk_event_cap(ctx, torch_mod):

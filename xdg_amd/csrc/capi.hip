@@ -92,7 +92,7 @@ struct xdg_hip_ctx {
   int32_t* d_fb_list = nullptr;  // mixed ray_fire: rays deferred to the FP64 pass
   size_t fb_cap = 0;
   unsigned int* d_fb_count = nullptr;
-  unsigned long long* d_next = nullptr;  // persistent kernels' work counters: [0] ray_fire, [1] find_volume
+  unsigned long long* d_next = nullptr;  // persistent kernels' work counters: [0] ray_fire, [1] find_volume, [2] mesh walk
   void* d_order = nullptr;  // ray ordering: keys, indices and sort storage
   int64_t order_cap = 0;
   RayOrderScratch order_w{};
@@ -250,7 +250,7 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->d_status, sizeof(uint32_t) * 4) != hipSuccess ||
       hipMalloc(&c->d_fb_count, sizeof(unsigned int) * 4) != hipSuccess ||
-      hipMalloc(&c->d_next, sizeof(unsigned long long) * 2) != hipSuccess ||
+      hipMalloc(&c->d_next, sizeof(unsigned long long) * 3) != hipSuccess ||
       hipMemset(c->d_status, 0, sizeof(uint32_t) * 4) != hipSuccess) {  // status flags start clear
     if (c->d_next) (void)hipFree(c->d_next);
     if (c->d_fb_count) (void)hipFree(c->d_fb_count);
@@ -265,7 +265,7 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   c->cfg.defer = env_int("XDG_HIP_DEFER", 1) != 0;
   c->cfg.wide = env_int("XDG_HIP_WIDE", 1) != 0;
   c->cfg.spec = env_int("XDG_HIP_SPEC", 1) != 0;
-  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 3) & 7;
+  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 3) & 15;
   c->cfg.order = env_int("XDG_HIP_ORDER", 0);
   c->cfg.refill = env_int("XDG_HIP_REFILL", 40);
   try {
@@ -325,7 +325,7 @@ int xdg_hip_set_traversal(xdg_hip_ctx* c, int wide, int defer, int persistent)
   return guarded(c, [&]() {
     c->cfg.wide = wide != 0;
     c->cfg.defer = defer != 0;
-    c->cfg.persistent = persistent & 7;
+    c->cfg.persistent = persistent & 15;
     return XDG_HIP_OK;
   });
 }
@@ -351,7 +351,7 @@ int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
 int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
 
 const OptionRef OPTIONS[] = {
-    {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 7, opt_persistent},
+    {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 15, opt_persistent},
     {"ray_order", 0, 2, opt_order},         {"speculative", 0, 1, opt_spec}, {"wide", 0, 1, opt_wide},
     {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
     {"refill", 1, 64, opt_refill},
@@ -1058,7 +1058,9 @@ int xdg_hip_mesh_walk(xdg_hip_ctx* c, int32_t element_tree, uint64_t first, int6
     a.status = status_out;
     a.hash = hash_out;
     a.tally_fx = (unsigned long long*)tally_fx;
-    launch_mesh_walk(a, face_order_kind(t.dev.face_order), cfg_for(c, depth), pick(c, stream));
+    const LaunchCfg cfg = cfg_for(c, depth);
+    launch_mesh_walk(a, face_order_kind(t.dev.face_order), cfg, (cfg.persistent & 8) ? c->d_next + 2 : nullptr,
+                     cfg.refill, pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
   });

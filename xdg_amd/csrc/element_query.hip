@@ -591,6 +591,121 @@ constexpr int MW_PRIMS = 8;        // re-entry exclusions kept per event
 constexpr int MW_STARTS = 1000;    // start samples tried per particle
 constexpr int MW_REENTRIES = 64;   // re-entry rays per event
 
+// One particle's state between events.
+struct MwState {
+  D3 r, u;
+  int32_t e;
+  double dist;
+  int32_t nev;
+  unsigned long long hh;
+  uint32_t dc;
+  uint8_t st;
+  uint64_t item;
+};
+
+// Particle::initialize of the tool: sample a location within the model
+// (walk_elements.h:64-71), an isotropic direction, normalized.
+__device__ __forceinline__ void mw_start(const MeshWalkArgs& a, const ElemTreeDev& T, int64_t i, MwState& S,
+                                         int32_t* stk, int cap)
+{
+  S.item = a.first + (uint64_t)i;
+  S.dc = 0;
+  S.st = MW_ALIVE;
+  S.e = ID_NONE;
+  S.r = d3(0.0, 0.0, 0.0);
+  S.u = d3(0.0, 0.0, 0.0);
+  S.dist = 0.0;
+  S.nev = 0;
+  S.hh = HASH_SEED;
+  for (int att = 0; att < MW_STARTS && S.e == ID_NONE; ++att) {
+    double ua, ub, uc, unused;
+    uniform2(a.seed, a.stream, S.item, S.dc++, ua, ub);
+    uniform2(a.seed, a.stream, S.item, S.dc++, uc, unused);
+    S.r = d3(a.lo[0] + (a.hi[0] - a.lo[0]) * ua, a.lo[1] + (a.hi[1] - a.lo[1]) * ub,
+             a.lo[2] + (a.hi[2] - a.lo[2]) * uc);
+    S.e = find_element_dev(T, S.r, stk, blockDim.x, cap);
+  }
+  if (S.e == ID_NONE) {
+    S.st = MW_ERROR;
+    return;
+  }
+  if (!iso_dir(a.seed, a.stream, S.item, S.dc, S.u))
+    S.st = MW_ERROR;
+  else
+    S.u = normalize(S.u);
+}
+
+// One event of the tool's loop (walk_elements.h:73-103); sets the final
+// status when the particle ends.
+template <int FO>
+__device__ __forceinline__ void mw_event(const MeshWalkArgs& a, const ElemTreeDev& T, bool has_ipc, MwState& S,
+                                         int32_t* stk, int32_t* prims, int cap)
+{
+  const int idx = major_axis(S.u);
+  // the element record is requested first and the collision distance drawn
+  // while it is in flight (the draw does not depend on the step)
+  ElemLoad L;
+  elem_load<FO>(T, S.e, L);
+  double ua, ub;
+  uniform2(a.seed, a.stream, S.item, S.dc++, ua, ub);
+  const double coll = -log_series(1.0 - ua) * a.mfp;
+  int32_t next;
+  double ex;
+  elem_step<FO>(T, L, S.r, S.u, idx, pick3(S.u.x, S.u.y, S.u.z, idx), next, ex);
+  const int32_t here = S.e;
+  const double s = coll < ex ? coll : ex;
+  S.r = d3(S.r.x + S.u.x * s, S.r.y + S.u.y * s, S.r.z + S.u.z * s);
+  S.dist += s;
+  if (coll < ex) {
+    if (!iso_dir(a.seed, a.stream, S.item, S.dc, S.u)) S.st = MW_ERROR;
+  } else {
+    S.e = next;
+  }
+  S.hh = mix64(S.hh ^ (unsigned long long)(uint32_t)here);
+  S.hh = mix64(S.hh ^ (unsigned long long)__double_as_longlong(s));
+  if (a.tally_fx) atomicAdd(a.tally_fx + here, (unsigned long long)__double2ll_rn(s * 1099511627776.0));
+  // re-entry through the implicit complement (walk_elements.h:92-102)
+  int tries = 0, np = 0;
+  while (S.st == MW_ALIVE && S.e == ID_NONE && has_ipc) {
+    if (++tries > MW_REENTRIES) {
+      S.st = MW_LOST;
+      break;
+    }
+    RayD ray;
+    ray.o = S.r;
+    ray.d = S.u;
+    ray.rb = cross(S.u, S.r);
+    ray.idx = major_axis(S.u);
+    ray.oi = pick3(S.r.x, S.r.y, S.r.z, ray.idx);
+    ray.di = pick3(S.u.x, S.u.y, S.u.z, ray.idx);
+    HitState h{INFTY, INT32_MAX, ID_NONE, false};
+    trace<RAYFIRE, false>(a.stable[T.ipc_tree], ray, ORIENT_EXITING, prims, np, stk, blockDim.x, cap, h);
+    if (h.surf == ID_NONE) break;
+    // the hit face joins the exclusions (ray_tracer.cpp:324)
+    if (np == MW_PRIMS) {
+      for (int j = 1; j < MW_PRIMS; ++j) prims[j - 1] = prims[j];
+      --np;
+    }
+    prims[np++] = h.face;
+    S.r = d3(S.r.x + S.u.x * h.t, S.r.y + S.u.y * h.t, S.r.z + S.u.z * h.t);
+    S.dist += h.t;
+    S.e = find_element_dev(
+        T, d3(S.r.x + S.u.x * TINY_BIT, S.r.y + S.u.y * TINY_BIT, S.r.z + S.u.z * TINY_BIT), stk, blockDim.x, cap);
+  }
+  ++S.nev;
+  if (S.st == MW_ALIVE && S.e != ID_NONE && S.nev >= a.max_events) S.st = MW_CAPPED;
+  if (S.st == MW_ALIVE && S.e == ID_NONE) S.st = MW_LEFT;
+}
+
+__device__ __forceinline__ void mw_finish(const MeshWalkArgs& a, int64_t i, const MwState& S)
+{
+  if (a.distance) a.distance[i] = S.dist;
+  if (a.events) a.events[i] = S.nev;
+  if (a.status) a.status[i] = S.st;
+  if (a.hash) a.hash[i] = S.hh;
+}
+
+// one particle per lane
 template <int FO>
 __global__ void k_mesh_walk(MeshWalkArgs a, int cap)
 {
@@ -601,90 +716,55 @@ __global__ void k_mesh_walk(MeshWalkArgs a, int cap)
   int32_t* prims = lds + (size_t)cap * blockDim.x + MW_PRIMS * threadIdx.x;
   const ElemTreeDev T = a.etable[a.tree];
   const bool has_ipc = T.ipc_tree >= 1 && T.ipc_tree <= a.nsurf_trees;
-  const uint64_t item = a.first + (uint64_t)i;
-  uint32_t dc = 0;
-  uint8_t st = MW_ALIVE;
-  D3 r = d3(0.0, 0.0, 0.0), u = d3(0.0, 0.0, 0.0);
-  int32_t e = ID_NONE;
-  // sample a location within the model (walk_elements.h:64-68)
-  for (int att = 0; att < MW_STARTS && e == ID_NONE; ++att) {
-    double ua, ub, uc, unused;
-    uniform2(a.seed, a.stream, item, dc++, ua, ub);
-    uniform2(a.seed, a.stream, item, dc++, uc, unused);
-    r = d3(a.lo[0] + (a.hi[0] - a.lo[0]) * ua, a.lo[1] + (a.hi[1] - a.lo[1]) * ub,
-           a.lo[2] + (a.hi[2] - a.lo[2]) * uc);
-    e = find_element_dev(T, r, stk, blockDim.x, cap);
-  }
-  if (e == ID_NONE) st = MW_ERROR;
-  // u = rand_dir(); u.normalize() (walk_elements.h:70-71)
-  if (st == MW_ALIVE) {
-    if (!iso_dir(a.seed, a.stream, item, dc, u))
-      st = MW_ERROR;
-    else
-      u = normalize(u);
-  }
-  double dist = 0.0;
-  int32_t nev = 0;
-  unsigned long long hh = HASH_SEED;
-  int np = 0;
-  while (st == MW_ALIVE && e != ID_NONE) {
-    const int idx = major_axis(u);
-    int32_t next;
-    double ex;
-    next_element_dev<FO>(T, e, r, u, idx, pick3(u.x, u.y, u.z, idx), next, ex);
-    double ua, ub;
-    uniform2(a.seed, a.stream, item, dc++, ua, ub);
-    const double coll = -log_series(1.0 - ua) * a.mfp;
-    const int32_t here = e;
-    const double s = coll < ex ? coll : ex;
-    r = d3(r.x + u.x * s, r.y + u.y * s, r.z + u.z * s);
-    dist += s;
-    if (coll < ex) {
-      if (!iso_dir(a.seed, a.stream, item, dc, u)) st = MW_ERROR;
-    } else {
-      e = next;
-    }
-    hh = mix64(hh ^ (unsigned long long)(uint32_t)here);
-    hh = mix64(hh ^ (unsigned long long)__double_as_longlong(s));
-    if (a.tally_fx) atomicAdd(a.tally_fx + here, (unsigned long long)__double2ll_rn(s * 1099511627776.0));
-    // re-entry through the implicit complement (walk_elements.h:92-102)
-    int tries = 0;
-    while (st == MW_ALIVE && e == ID_NONE && has_ipc) {
-      if (++tries > MW_REENTRIES) {
-        st = MW_LOST;
-        break;
-      }
-      RayD ray;
-      ray.o = r;
-      ray.d = u;
-      ray.rb = cross(u, r);
-      ray.idx = major_axis(u);
-      ray.oi = pick3(r.x, r.y, r.z, ray.idx);
-      ray.di = pick3(u.x, u.y, u.z, ray.idx);
-      HitState h{INFTY, INT32_MAX, ID_NONE, false};
-      trace<RAYFIRE, false>(a.stable[T.ipc_tree], ray, ORIENT_EXITING, prims, np, stk, blockDim.x, cap, h);
-      if (h.surf == ID_NONE) break;
-      // the hit face joins the exclusions (ray_tracer.cpp:324)
-      if (np == MW_PRIMS) {
-        for (int j = 1; j < MW_PRIMS; ++j) prims[j - 1] = prims[j];
-        --np;
-      }
-      prims[np++] = h.face;
-      r = d3(r.x + u.x * h.t, r.y + u.y * h.t, r.z + u.z * h.t);
-      dist += h.t;
-      e = find_element_dev(T, d3(r.x + u.x * TINY_BIT, r.y + u.y * TINY_BIT, r.z + u.z * TINY_BIT), stk,
-                           blockDim.x, cap);
-    }
-    np = 0;
-    ++nev;
-    if (st == MW_ALIVE && e != ID_NONE && nev >= a.max_events) st = MW_CAPPED;
-  }
-  if (st == MW_ALIVE) st = MW_LEFT;
-  if (a.distance) a.distance[i] = dist;
-  if (a.events) a.events[i] = nev;
-  if (a.status) a.status[i] = st;
-  if (a.hash) a.hash[i] = hh;
+  MwState S;
+  mw_start(a, T, i, S, stk, cap);
+  while (S.st == MW_ALIVE) mw_event<FO>(a, T, has_ipc, S, stk, prims, cap);
+  mw_finish(a, i, S);
 }
+
+// Resident grid: a lane whose particle ended waits until `refill` lanes of
+// its wave (or all that remain) are idle; they then claim as many new
+// particles with one atomic.  Particles have very different event counts,
+// so without this a wave runs as long as its longest particle.
+template <int FO>
+__global__ void k_mesh_walk_pt(MeshWalkArgs a, int cap, unsigned long long* next_item, int refill)
+{
+  extern __shared__ int32_t lds[];
+  int32_t* stk = lds + threadIdx.x;
+  int32_t* prims = lds + (size_t)cap * blockDim.x + MW_PRIMS * threadIdx.x;
+  const ElemTreeDev T = a.etable[a.tree];
+  const bool has_ipc = T.ipc_tree >= 1 && T.ipc_tree <= a.nsurf_trees;
+  const int lane = threadIdx.x & 63;
+  MwState S;
+  S.st = MW_ERROR;  // no particle yet
+  bool have = false;
+  int64_t i = 0;
+  while (true) {
+    const bool idle = S.st != MW_ALIVE;
+    if (idle && have) {
+      mw_finish(a, i, S);
+      have = false;
+    }
+    const int nact = __popcll(__ballot(1));
+    const int nidle = __popcll(__ballot(idle));
+    if (idle) {
+      if (nidle < (refill < nact ? refill : nact)) continue;
+      const unsigned long long mask = __ballot(1);  // the idle lanes here
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long b = 0;
+      if (lane == leader) b = atomicAdd(next_item, (unsigned long long)__popcll(mask));
+      b = __shfl(b, leader);
+      i = (int64_t)(b + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
+      if (i >= a.n) break;
+      have = true;
+      mw_start(a, T, i, S, stk, cap);
+      continue;
+    }
+    mw_event<FO>(a, T, has_ipc, S, stk, prims, cap);
+  }
+}
+
+__global__ void k_zero_counter(unsigned long long* p) { *p = 0ull; }
 
 __global__ void k_elem_boxes(const ElemRec* __restrict__ elems, int64_t n, double* box6)
 {
@@ -717,16 +797,41 @@ void launch_find_element(const FindElementArgs& a, const LaunchCfg& c, hipStream
   hipLaunchKernelGGL(k_find_element, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
 }
 
-void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s)
+template <class K>
+int64_t resident_blocks(K kernel, int block, size_t lds)
+{
+  int dev = 0, cu = 0, nb = 0;
+  XDG_HIP_CHECK(hipGetDevice(&dev));
+  XDG_HIP_CHECK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+  XDG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, lds));
+  return (int64_t)std::max(nb, 1) * std::max(cu, 1);
+}
+
+template <int FO>
+void launch_mesh_walk_t(const MeshWalkArgs& a, const LaunchCfg& c, unsigned long long* next_item, int refill,
+                        hipStream_t s)
+{
+  const size_t sh = sizeof(int32_t) * ((size_t)c.stack + MW_PRIMS) * c.block;
+  if (next_item) {
+    const int64_t need = (a.n + c.block - 1) / c.block;
+    const int64_t g = std::min(need, resident_blocks(k_mesh_walk_pt<FO>, c.block, sh));
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, next_item);
+    hipLaunchKernelGGL(k_mesh_walk_pt<FO>, dim3((unsigned)g), dim3(c.block), sh, s, a, c.stack, next_item, refill);
+  } else {
+    hipLaunchKernelGGL(k_mesh_walk<FO>, grid1(a.n, c.block), dim3(c.block), sh, s, a, c.stack);
+  }
+}
+
+void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, unsigned long long* next_item,
+                      int refill, hipStream_t s)
 {
   if (a.n <= 0) return;
-  const size_t sh = sizeof(int32_t) * ((size_t)c.stack + MW_PRIMS) * c.block;
   if (face_order_kind == FO_MOAB)
-    hipLaunchKernelGGL(k_mesh_walk<FO_MOAB>, grid1(a.n, c.block), dim3(c.block), sh, s, a, c.stack);
+    launch_mesh_walk_t<FO_MOAB>(a, c, next_item, refill, s);
   else if (face_order_kind == FO_MOCK)
-    hipLaunchKernelGGL(k_mesh_walk<FO_MOCK>, grid1(a.n, c.block), dim3(c.block), sh, s, a, c.stack);
+    launch_mesh_walk_t<FO_MOCK>(a, c, next_item, refill, s);
   else
-    hipLaunchKernelGGL(k_mesh_walk<FO_TABLE>, grid1(a.n, c.block), dim3(c.block), sh, s, a, c.stack);
+    launch_mesh_walk_t<FO_TABLE>(a, c, next_item, refill, s);
 }
 
 void launch_next_element(const NextElementArgs& a, hipStream_t s)

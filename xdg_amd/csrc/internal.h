@@ -113,7 +113,7 @@ struct LaunchCfg {
   int defer = 1;  // mixed ray_fire: FP64 pass for far-origin rays
   int wide = 1;   // surface queries traverse the BVH4
   // resident-grid kernels with per-lane work refill: bit 0 plain ray_fire,
-  // bit 1 mixed ray_fire, bit 2 find_volume
+  // bit 1 mixed ray_fire, bit 2 find_volume, bit 3 mesh walk
   int persistent = 3;
   int order = 0;  // ray_fire coherence ordering mode (0 = off)
   int spec = 1;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
@@ -341,7 +341,10 @@ struct MeshWalkArgs {
   uint64_t* hash;
   unsigned long long* tally_fx;   // nullable, per element, scale 2^40
 };
-void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s);
+// next_item (nullable): claim counter of the resident-grid kernel (zeroed by
+// the launcher); null runs one particle per lane
+void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, unsigned long long* next_item,
+                      int refill, hipStream_t s);
 // 0 = MOAB canonical, 1 = MeshMock, 2 = other (read from the tree)
 int face_order_kind(const int32_t* face_order);
 
