@@ -124,3 +124,65 @@ def test_released_tree_is_gone(ctx, problem):
     ctx.release_surface_tree(tree)
     with pytest.raises(capi.XdgHipError):
         ctx.tree_info(tree)
+
+
+def test_full_size_central_differences(ctx, torch_mod):
+    """The adjoint workload at its full size: icosphere subdivision 6 (81 920
+    triangles) radially perturbed, 4 M rays from points with r < 9.5,
+    L = sum w_i t_i with w_i ~ N(0, 1).  The full dL/dV from the autograd
+    harness equals the oracle's closed form (relative 1e-12).  On 64 random
+    vertices the adjoint of the rays that hit a triangle of the vertex
+    matches central differences (h = 1e-6, fourth order) of the same rays'
+    distances, relative 1e-6.  Rays whose hit lies within 1e-4 (barycentric)
+    of an edge are left out of that comparison: a perturbation of h moves
+    them across the edge onto another triangle, where t has another slope,
+    so no finite difference of width 4h is the derivative there."""
+    torch = torch_mod
+    from xdg_amd import adjoint as A
+    m = G.perturbed_sphere(6, 10.0, 0.05)
+    assert m.tris.shape[0] == 81920
+    n = 4_000_000
+    org = P.ball_points(5, 0, n, 9.5)
+    dirs = P.iso_dirs(5, 1, n)
+    w = P.normals(5, 2, n)
+    surf = A.DifferentiableSurface(ctx, m.tris)
+    verts = dev(torch, m.verts).requires_grad_(True)
+    o, d = dev(torch, org), dev(torch, dirs)
+    t = A.ray_distance(o, d, verts, surf)
+    (dev(torch, w) * t).sum().backward()
+    gv = host(verts.grad)
+    tree = surf.tree_for(dev(torch, m.verts))
+    tt, _, face = ctx.ray_fire(tree, o, d, precision=1)
+    tt, face = host(tt), host(face)
+    assert (face >= 0).all()
+    ov, _, _ = O.adjoint(m.verts, m.tris, org, dirs, tt, face, w)
+    assert np.allclose(gv, ov, rtol=1e-12, atol=1e-12 * np.abs(ov).max())
+    # barycentric coordinates of every hit
+    q = org + tt[:, None] * dirs
+    a, b, c = (m.verts[m.tris[face, j]] for j in range(3))
+    e1, e2, qa = b - a, c - a, q - a
+    d11, d12, d22 = (e1 * e1).sum(1), (e1 * e2).sum(1), (e2 * e2).sum(1)
+    q1, q2 = (qa * e1).sum(1), (qa * e2).sum(1)
+    den = d11 * d22 - d12 * d12
+    l1 = (d22 * q1 - d12 * q2) / den
+    l2 = (d11 * q2 - d12 * q1) / den
+    interior = np.minimum(np.minimum(l1, l2), 1.0 - l1 - l2) > 1e-4
+    rng = np.random.default_rng(11)
+    for vi in rng.choice(m.verts.shape[0], 64, replace=False):
+        star = np.nonzero((m.tris == vi).any(1))[0]
+        sel = np.nonzero(np.isin(face, star) & interior)[0]
+        assert sel.size > 0
+        so, sd, sw = dev(torch, org[sel]), dev(torch, dirs[sel]), w[sel]
+        g = torch.zeros_like(verts.detach())
+        ctx.ray_distance_adjoint(so, sd, dev(torch, tt[sel]), dev(torch, face[sel]), dev(torch, sw),
+                                 verts.detach(), surf.tris, g, None, None)
+        g = host(g)
+        for k in range(3):
+            def fire(step):
+                v = m.verts.copy()
+                v[vi, k] += step
+                tk, _, _ = ctx.ray_fire(surf.tree_for(dev(torch, v)), so, sd, precision=1)
+                return host(tk)
+
+            fd = (sw * (8.0 * (fire(H) - fire(-H)) - (fire(2 * H) - fire(-2 * H)))).sum() / (12 * H)
+            assert abs(fd - g[vi, k]) <= 1e-6 * max(1.0, abs(g[vi, k])), (vi, k, fd, g[vi, k])
