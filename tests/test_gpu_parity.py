@@ -427,7 +427,8 @@ def test_nested_find_volume_and_fire(ctx, torch_mod):
 def test_find_volume_far_points_and_variants(ctx, torch_mod, persist):
     """Points beyond the FP32 limit of the trees (deferred to the FP64 pass
     by the persistent mixed kernel) mixed with near ones; persistent and
-    per-point kernels, both precisions, equal the oracle."""
+    per-point kernels, both precisions, two volume orders, equal the
+    oracle."""
     torch = torch_mod
     m = G.nested_spheres(2)
     vols = list(m.volumes)
@@ -445,6 +446,13 @@ def test_find_volume_far_points_and_variants(ctx, torch_mod, persist):
         for prec in (0, 1):
             got = host(ctx.find_volume(trees, vols, m.ipc, dev(torch, pts), dev(torch, dirs), precision=prec))
             assert np.array_equal(got, ref)
+            # a volume order with the implicit complement first and the
+            # outermost volume before the inner ones
+            order = [vols[-1], vols[-2]] + vols[:-2]
+            tr = [trees[vols.index(v)] for v in order]
+            ref2 = O.find_volume([otrees[vols.index(v)] for v in order], order, m.ipc, pts, dirs)
+            got2 = host(ctx.find_volume(tr, order, m.ipc, dev(torch, pts), dev(torch, dirs), precision=prec))
+            assert np.array_equal(got2, ref2)
     finally:
         ctx.set_traversal()
 
