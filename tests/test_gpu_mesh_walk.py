@@ -45,8 +45,8 @@ CASES = {
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
-@pytest.mark.parametrize("persist", [True, False])
-def test_mesh_walk_matches_oracle(ctx, torch_mod, name, persist):
+@pytest.mark.parametrize("wave", [1, 0])
+def test_mesh_walk_matches_oracle(ctx, torch_mod, name, wave):
     torch = torch_mod
     make, lo, hi, mfp = CASES[name]
     mesh = make()
@@ -55,11 +55,11 @@ def test_mesh_walk_matches_oracle(ctx, torch_mod, name, persist):
     n, first = 20000, 321
     rd, re, rs, rh, rt = om.mesh_walk(first, n, SEED, STREAM, lo, hi, mfp, tally=True)
     tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device="cuda")
-    ctx.set_traversal(persistent=persist)
+    ctx.set_option("mw_wave", wave)
     try:
         d, e, s, h = ctx.mesh_walk(et, first, n, SEED, STREAM, lo, hi, mfp, tally=tally)
     finally:
-        ctx.set_traversal()
+        ctx.set_option("mw_wave", 1)
     assert np.array_equal(host(s), rs)
     assert np.array_equal(host(e), re)
     assert np.array_equal(bits(host(d)), bits(rd))
@@ -69,22 +69,19 @@ def test_mesh_walk_matches_oracle(ctx, torch_mod, name, persist):
     assert re.mean() > 3
 
 
-@pytest.mark.parametrize("refill", [1, 17, 64])
-def test_mesh_walk_refill_thresholds(ctx, torch_mod, refill):
-    """The resident-grid walk gives the same particles for any refill
-    threshold, including a batch smaller than one wave."""
-    mesh = G.kuhn_cube(8, 0.2, seed=4)
+def test_mesh_walk_many_reentries(ctx, torch_mod):
+    """Two blocks with a narrow gap and a long mean free path: particles
+    cross the gap several times, more often than the batched re-entry
+    rounds, so the last walk re-enters inline; small batches (below one
+    wave) too.  Equal to the oracle."""
+    mesh = G.two_blocks(6, 0.05)
     et = _setup(ctx, mesh)
     om = O.TetMesh(mesh)
-    ctx.set_option("refill", refill)
-    try:
-        for n in (37, 5000):
-            rd, re, rs, rh = om.mesh_walk(9, n, SEED, STREAM, [0.0] * 3, [1.0] * 3, 0.05)
-            d, e, s, h = ctx.mesh_walk(et, 9, n, SEED, STREAM, [0.0] * 3, [1.0] * 3, 0.05)
-            assert np.array_equal(host(e), re) and np.array_equal(bits(host(d)), bits(rd))
-            assert np.array_equal(host(h).view(np.uint64), rh) and np.array_equal(host(s), rs)
-    finally:
-        ctx.set_option("refill", 40)
+    for n in (37, 6000):
+        rd, re, rs, rh = om.mesh_walk(5, n, SEED, STREAM, [0.0] * 3, [2.05, 1.0, 1.0], 0.4)
+        d, e, s, h = ctx.mesh_walk(et, 5, n, SEED, STREAM, [0.0] * 3, [2.05, 1.0, 1.0], 0.4)
+        assert np.array_equal(host(e), re) and np.array_equal(bits(host(d)), bits(rd))
+        assert np.array_equal(host(h).view(np.uint64), rh) and np.array_equal(host(s), rs)
 
 
 def test_mesh_walk_event_cap(ctx, torch_mod):

@@ -231,13 +231,12 @@ class Context:
 
     def set_traversal(self, wide=True, defer=True, persistent=None):
         """persistent: None for the default (ray_fire, both precisions),
-        True / False for every kernel, or the raw mask (bit 0
-        plain ray_fire, bit 1 mixed ray_fire, bit 2 find_volume, bit 3 mesh
-        walk)."""
+        True / False for every kernel, or the raw mask (bit 0 plain
+        ray_fire, bit 1 mixed ray_fire, bit 2 find_volume)."""
         if persistent is None:
             mask = PERSIST_DEFAULT
         elif persistent is True or persistent is False:
-            mask = 15 if persistent else 0
+            mask = 7 if persistent else 0
         else:
             mask = int(persistent)
         self._check(lib().xdg_hip_set_traversal(self.h, int(wide), int(defer), mask))

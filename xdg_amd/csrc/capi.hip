@@ -92,7 +92,10 @@ struct xdg_hip_ctx {
   int32_t* d_fb_list = nullptr;  // mixed ray_fire: rays deferred to the FP64 pass
   size_t fb_cap = 0;
   unsigned int* d_fb_count = nullptr;
-  unsigned long long* d_next = nullptr;  // persistent kernels' work counters: [0] ray_fire, [1] find_volume, [2] mesh walk
+  unsigned long long* d_next = nullptr;  // persistent kernels' work counters: [0] ray_fire, [1] find_volume
+  char* d_mw = nullptr;  // wavefront mesh walk scratch
+  int64_t mw_cap = 0;
+  MeshWalkScratch mw_w{};
   void* d_order = nullptr;  // ray ordering: keys, indices and sort storage
   int64_t order_cap = 0;
   RayOrderScratch order_w{};
@@ -205,6 +208,26 @@ int32_t* fb_list_for(xdg_hip_ctx* c, int64_t n)
   return c->d_fb_list;
 }
 
+// scratch of the wavefront mesh walk for n particles (grown on demand)
+const MeshWalkScratch& mw_scratch_for(xdg_hip_ctx* c, int64_t n)
+{
+  if (n > c->mw_cap) {
+    XDG_HIP_CHECK(hipDeviceSynchronize());
+    if (c->d_mw) XDG_HIP_CHECK(hipFree(c->d_mw));
+    c->d_mw = nullptr;
+    c->mw_cap = 0;
+    const size_t sv = ((size_t)n * sizeof(MwSave) + 255) & ~(size_t)255;
+    const size_t sl = ((size_t)n * sizeof(int32_t) + 255) & ~(size_t)255;
+    XDG_HIP_CHECK(hipMalloc((void**)&c->d_mw, sv + 2 * sl + 256));
+    c->mw_cap = n;
+    c->mw_w.save = (MwSave*)c->d_mw;
+    c->mw_w.list[0] = (int32_t*)(c->d_mw + sv);
+    c->mw_w.list[1] = (int32_t*)(c->d_mw + sv + sl);
+    c->mw_w.count = (unsigned int*)(c->d_mw + sv + 2 * sl);
+  }
+  return c->mw_w;
+}
+
 // rays below this count are traced in the caller's order
 constexpr int64_t ORDER_MIN_RAYS = 1 << 16;
 
@@ -250,7 +273,7 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->d_status, sizeof(uint32_t) * 4) != hipSuccess ||
       hipMalloc(&c->d_fb_count, sizeof(unsigned int) * 4) != hipSuccess ||
-      hipMalloc(&c->d_next, sizeof(unsigned long long) * 3) != hipSuccess ||
+      hipMalloc(&c->d_next, sizeof(unsigned long long) * 2) != hipSuccess ||
       hipMemset(c->d_status, 0, sizeof(uint32_t) * 4) != hipSuccess) {  // status flags start clear
     if (c->d_next) (void)hipFree(c->d_next);
     if (c->d_fb_count) (void)hipFree(c->d_fb_count);
@@ -265,7 +288,7 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   c->cfg.defer = env_int("XDG_HIP_DEFER", 1) != 0;
   c->cfg.wide = env_int("XDG_HIP_WIDE", 1) != 0;
   c->cfg.spec = env_int("XDG_HIP_SPEC", 1) != 0;
-  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 3) & 15;
+  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 3) & 7;
   c->cfg.order = env_int("XDG_HIP_ORDER", 0);
   c->cfg.refill = env_int("XDG_HIP_REFILL", 40);
   try {
@@ -298,6 +321,7 @@ int xdg_hip_destroy(xdg_hip_ctx* c)
     if (c->d_small) (void)hipFree(c->d_small);
     if (c->d_status) (void)hipFree(c->d_status);
     if (c->d_fb_list) (void)hipFree(c->d_fb_list);
+    if (c->d_mw) (void)hipFree(c->d_mw);
     if (c->d_fb_count) (void)hipFree(c->d_fb_count);
     if (c->d_next) (void)hipFree(c->d_next);
     if (c->d_order) (void)hipFree(c->d_order);
@@ -325,7 +349,7 @@ int xdg_hip_set_traversal(xdg_hip_ctx* c, int wide, int defer, int persistent)
   return guarded(c, [&]() {
     c->cfg.wide = wide != 0;
     c->cfg.defer = defer != 0;
-    c->cfg.persistent = persistent & 15;
+    c->cfg.persistent = persistent & 7;
     return XDG_HIP_OK;
   });
 }
@@ -349,12 +373,13 @@ int* opt_wide(xdg_hip_ctx* c) { return &c->cfg.wide; }
 int* opt_defer(xdg_hip_ctx* c) { return &c->cfg.defer; }
 int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
 int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
+int* opt_mw_wave(xdg_hip_ctx* c) { return &c->cfg.mw_wave; }
 
 const OptionRef OPTIONS[] = {
-    {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 15, opt_persistent},
+    {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 7, opt_persistent},
     {"ray_order", 0, 2, opt_order},         {"speculative", 0, 1, opt_spec}, {"wide", 0, 1, opt_wide},
     {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
-    {"refill", 1, 64, opt_refill},
+    {"refill", 1, 64, opt_refill},          {"mw_wave", 0, 1, opt_mw_wave},
 };
 
 const OptionRef& find_option(const char* name)
@@ -1059,8 +1084,9 @@ int xdg_hip_mesh_walk(xdg_hip_ctx* c, int32_t element_tree, uint64_t first, int6
     a.hash = hash_out;
     a.tally_fx = (unsigned long long*)tally_fx;
     const LaunchCfg cfg = cfg_for(c, depth);
-    launch_mesh_walk(a, face_order_kind(t.dev.face_order), cfg, (cfg.persistent & 8) ? c->d_next + 2 : nullptr,
-                     cfg.refill, pick(c, stream));
+    const bool wave = cfg.mw_wave && t.dev.ipc_tree >= 1;
+    launch_mesh_walk(a, face_order_kind(t.dev.face_order), cfg, wave ? &mw_scratch_for(c, n) : nullptr,
+                     pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
   });

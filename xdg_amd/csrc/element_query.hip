@@ -635,17 +635,17 @@ __device__ __forceinline__ void mw_start(const MeshWalkArgs& a, const ElemTreeDe
     S.u = normalize(S.u);
 }
 
-// One event of the tool's loop (walk_elements.h:73-103); sets the final
-// status when the particle ends.
+// The part of an event before any re-entry (walk_elements.h:73-90):
+// next_element, the collision draw, the move, the tally.  L holds the
+// record of S.e, requested by the previous step (or before the loop); the
+// collision distance is drawn while it is in flight, and the record of the
+// element the particle is in afterwards is requested before the tally
+// atomic, so the next step waits for that record alone, not for the
+// atomic as well.
 template <int FO>
-__device__ __forceinline__ void mw_event(const MeshWalkArgs& a, const ElemTreeDev& T, bool has_ipc, MwState& S,
-                                         int32_t* stk, int32_t* prims, int cap)
+__device__ __forceinline__ void mw_step(const MeshWalkArgs& a, const ElemTreeDev& T, MwState& S, ElemLoad& L)
 {
   const int idx = major_axis(S.u);
-  // the element record is requested first and the collision distance drawn
-  // while it is in flight (the draw does not depend on the step)
-  ElemLoad L;
-  elem_load<FO>(T, S.e, L);
   double ua, ub;
   uniform2(a.seed, a.stream, S.item, S.dc++, ua, ub);
   const double coll = -log_series(1.0 - ua) * a.mfp;
@@ -661,12 +661,21 @@ __device__ __forceinline__ void mw_event(const MeshWalkArgs& a, const ElemTreeDe
   } else {
     S.e = next;
   }
+  // a particle that left the mesh requests its last record again (unused,
+  // always in range), so the load is not under a branch
+  elem_load<FO>(T, S.e != ID_NONE ? S.e : here, L);
   S.hh = mix64(S.hh ^ (unsigned long long)(uint32_t)here);
   S.hh = mix64(S.hh ^ (unsigned long long)__double_as_longlong(s));
   if (a.tally_fx) atomicAdd(a.tally_fx + here, (unsigned long long)__double2ll_rn(s * 1099511627776.0));
-  // re-entry through the implicit complement (walk_elements.h:92-102)
+}
+
+// Re-entry through the implicit complement (walk_elements.h:92-102) of a
+// particle that left the mesh; prims holds the event's hit faces.
+__device__ __forceinline__ void mw_reenter(const MeshWalkArgs& a, const ElemTreeDev& T, MwState& S, int32_t* stk,
+                                           int32_t* prims, int cap)
+{
   int tries = 0, np = 0;
-  while (S.st == MW_ALIVE && S.e == ID_NONE && has_ipc) {
+  while (S.st == MW_ALIVE && S.e == ID_NONE) {
     if (++tries > MW_REENTRIES) {
       S.st = MW_LOST;
       break;
@@ -692,6 +701,12 @@ __device__ __forceinline__ void mw_event(const MeshWalkArgs& a, const ElemTreeDe
     S.e = find_element_dev(
         T, d3(S.r.x + S.u.x * TINY_BIT, S.r.y + S.u.y * TINY_BIT, S.r.z + S.u.z * TINY_BIT), stk, blockDim.x, cap);
   }
+}
+
+// The end of an event (walk_elements.h:103-104): count it; the event cap;
+// a particle outside the mesh after its re-entry attempts has left.
+__device__ __forceinline__ void mw_close(const MeshWalkArgs& a, MwState& S)
+{
   ++S.nev;
   if (S.st == MW_ALIVE && S.e != ID_NONE && S.nev >= a.max_events) S.st = MW_CAPPED;
   if (S.st == MW_ALIVE && S.e == ID_NONE) S.st = MW_LEFT;
@@ -705,66 +720,108 @@ __device__ __forceinline__ void mw_finish(const MeshWalkArgs& a, int64_t i, cons
   if (a.hash) a.hash[i] = S.hh;
 }
 
-// one particle per lane
-template <int FO>
-__global__ void k_mesh_walk(MeshWalkArgs a, int cap)
+__device__ __forceinline__ void mw_store(MwSave& m, const MwState& S)
+{
+  m.r[0] = S.r.x;
+  m.r[1] = S.r.y;
+  m.r[2] = S.r.z;
+  m.u[0] = S.u.x;
+  m.u[1] = S.u.y;
+  m.u[2] = S.u.z;
+  m.dist = S.dist;
+  m.hh = S.hh;
+  m.e = S.e;
+  m.nev = S.nev;
+  m.dc = S.dc;
+}
+
+__device__ __forceinline__ void mw_load(const MwSave& m, uint64_t item, MwState& S)
+{
+  S.r = d3(m.r[0], m.r[1], m.r[2]);
+  S.u = d3(m.u[0], m.u[1], m.u[2]);
+  S.dist = m.dist;
+  S.hh = m.hh;
+  S.e = m.e;
+  S.nev = m.nev;
+  S.dc = m.dc;
+  S.st = MW_ALIVE;
+  S.item = item;
+}
+
+// Walk kernel of the wavefront: particles from in_list (every particle,
+// started here, when in_list is null) run events until they end.  A
+// particle that leaves the mesh is parked in out_list for the batched
+// re-entry kernel; with out_list null it re-enters inline.  Tracing the
+// re-entry rays of a whole list together keeps that long, rare traversal
+// from running with one active lane while the rest of the wave waits.
+// At most 256 lanes per workgroup, so the inline variant (which carries
+// the re-entry traversal too) may hold more than 128 VGPRs instead of
+// spilling.
+template <int FO, bool INLINE>
+__global__ __launch_bounds__(256) void k_mw_walk(MeshWalkArgs a, int cap, MwSave* save, const int32_t* in_list,
+                                                 const unsigned int* in_count, int32_t* out_list,
+                                                 unsigned int* out_count)
 {
   extern __shared__ int32_t lds[];
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t n_in = in_count ? (int64_t)*in_count : a.n;
+  if (j >= n_in) return;
+  const int64_t i = in_list ? (int64_t)in_list[j] : j;
   int32_t* stk = lds + threadIdx.x;
   int32_t* prims = lds + (size_t)cap * blockDim.x + MW_PRIMS * threadIdx.x;
   const ElemTreeDev T = a.etable[a.tree];
   const bool has_ipc = T.ipc_tree >= 1 && T.ipc_tree <= a.nsurf_trees;
   MwState S;
-  mw_start(a, T, i, S, stk, cap);
-  while (S.st == MW_ALIVE) mw_event<FO>(a, T, has_ipc, S, stk, prims, cap);
+  if (in_list)
+    mw_load(save[i], a.first + (uint64_t)i, S);
+  else
+    mw_start(a, T, i, S, stk, cap);
+  ElemLoad L;
+  if (S.st == MW_ALIVE) elem_load<FO>(T, S.e, L);
+  while (S.st == MW_ALIVE) {
+    mw_step<FO>(a, T, S, L);
+    if (S.st == MW_ALIVE && S.e == ID_NONE && has_ipc) {
+      if (!INLINE) {
+        mw_store(save[i], S);
+        out_list[atomicAdd(out_count, 1u)] = (int32_t)i;
+        return;
+      }
+      mw_reenter(a, T, S, stk, prims, cap);
+      if (S.e != ID_NONE) elem_load<FO>(T, S.e, L);
+    }
+    mw_close(a, S);
+  }
   mw_finish(a, i, S);
 }
 
-// Resident grid: a lane whose particle ended waits until `refill` lanes of
-// its wave (or all that remain) are idle; they then claim as many new
-// particles with one atomic.  Particles have very different event counts,
-// so without this a wave runs as long as its longest particle.
-template <int FO>
-__global__ void k_mesh_walk_pt(MeshWalkArgs a, int cap, unsigned long long* next_item, int refill)
+// Re-entry kernel of the wavefront: the parked particles' re-entry rays;
+// particles back in the mesh go on in out_list.
+__global__ void k_mw_reenter(MeshWalkArgs a, int cap, MwSave* save, const int32_t* in_list,
+                             const unsigned int* in_count, int32_t* out_list, unsigned int* out_count)
 {
   extern __shared__ int32_t lds[];
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= (int64_t)*in_count) return;
+  const int64_t i = in_list[j];
   int32_t* stk = lds + threadIdx.x;
   int32_t* prims = lds + (size_t)cap * blockDim.x + MW_PRIMS * threadIdx.x;
   const ElemTreeDev T = a.etable[a.tree];
-  const bool has_ipc = T.ipc_tree >= 1 && T.ipc_tree <= a.nsurf_trees;
-  const int lane = threadIdx.x & 63;
   MwState S;
-  S.st = MW_ERROR;  // no particle yet
-  bool have = false;
-  int64_t i = 0;
-  while (true) {
-    const bool idle = S.st != MW_ALIVE;
-    if (idle && have) {
-      mw_finish(a, i, S);
-      have = false;
-    }
-    const int nact = __popcll(__ballot(1));
-    const int nidle = __popcll(__ballot(idle));
-    if (idle) {
-      if (nidle < (refill < nact ? refill : nact)) continue;
-      const unsigned long long mask = __ballot(1);  // the idle lanes here
-      const int leader = __ffsll((long long)mask) - 1;
-      unsigned long long b = 0;
-      if (lane == leader) b = atomicAdd(next_item, (unsigned long long)__popcll(mask));
-      b = __shfl(b, leader);
-      i = (int64_t)(b + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
-      if (i >= a.n) break;
-      have = true;
-      mw_start(a, T, i, S, stk, cap);
-      continue;
-    }
-    mw_event<FO>(a, T, has_ipc, S, stk, prims, cap);
+  mw_load(save[i], a.first + (uint64_t)i, S);
+  mw_reenter(a, T, S, stk, prims, cap);
+  mw_close(a, S);
+  if (S.st == MW_ALIVE) {
+    mw_store(save[i], S);
+    out_list[atomicAdd(out_count, 1u)] = (int32_t)i;
+  } else {
+    mw_finish(a, i, S);
   }
 }
 
-__global__ void k_zero_counter(unsigned long long* p) { *p = 0ull; }
+__global__ void k_zero_counters(unsigned int* p, int n)
+{
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
 
 __global__ void k_elem_boxes(const ElemRec* __restrict__ elems, int64_t n, double* box6)
 {
@@ -797,41 +854,44 @@ void launch_find_element(const FindElementArgs& a, const LaunchCfg& c, hipStream
   hipLaunchKernelGGL(k_find_element, grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
 }
 
-template <class K>
-int64_t resident_blocks(K kernel, int block, size_t lds)
-{
-  int dev = 0, cu = 0, nb = 0;
-  XDG_HIP_CHECK(hipGetDevice(&dev));
-  XDG_HIP_CHECK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
-  XDG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, lds));
-  return (int64_t)std::max(nb, 1) * std::max(cu, 1);
-}
-
 template <int FO>
-void launch_mesh_walk_t(const MeshWalkArgs& a, const LaunchCfg& c, unsigned long long* next_item, int refill,
-                        hipStream_t s)
+void launch_mesh_walk_t(const MeshWalkArgs& a, const LaunchCfg& c, const MeshWalkScratch* w, hipStream_t s)
 {
-  const size_t sh = sizeof(int32_t) * ((size_t)c.stack + MW_PRIMS) * c.block;
-  if (next_item) {
-    const int64_t need = (a.n + c.block - 1) / c.block;
-    const int64_t g = std::min(need, resident_blocks(k_mesh_walk_pt<FO>, c.block, sh));
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, next_item);
-    hipLaunchKernelGGL(k_mesh_walk_pt<FO>, dim3((unsigned)g), dim3(c.block), sh, s, a, c.stack, next_item, refill);
-  } else {
-    hipLaunchKernelGGL(k_mesh_walk<FO>, grid1(a.n, c.block), dim3(c.block), sh, s, a, c.stack);
+  const int block = c.block > 256 ? 256 : c.block;  // k_mw_walk's launch bound
+  const size_t sh = sizeof(int32_t) * ((size_t)c.stack + MW_PRIMS) * block;
+  const dim3 g = grid1(a.n, block), b(block);
+  if (!w) {  // everything inline, one particle per lane
+    hipLaunchKernelGGL((k_mw_walk<FO, true>), g, b, sh, s, a, c.stack, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return;
   }
+  // rounds of walk -> batched re-entry; the last walk re-enters inline, so
+  // every particle ends whatever the number of re-entries
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, w->count, 2 * MW_ROUNDS);
+  const int32_t* in_list = nullptr;
+  const unsigned int* in_count = nullptr;
+  for (int r = 0; r < MW_ROUNDS; ++r) {
+    hipLaunchKernelGGL((k_mw_walk<FO, false>), g, b, sh, s, a, c.stack, w->save, in_list, in_count, w->list[0],
+                       w->count + 2 * r);
+    hipLaunchKernelGGL(k_mw_reenter, g, b, sh, s, a, c.stack, w->save, (const int32_t*)w->list[0],
+                       (const unsigned int*)(w->count + 2 * r), w->list[1], w->count + 2 * r + 1);
+    // the next walk reads list 1 and parks into list 0 again; the stream
+    // orders each list's reads before it is rewritten
+    in_list = w->list[1];
+    in_count = w->count + 2 * r + 1;
+  }
+  hipLaunchKernelGGL((k_mw_walk<FO, true>), g, b, sh, s, a, c.stack, w->save, in_list, in_count, nullptr, nullptr);
 }
 
-void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, unsigned long long* next_item,
-                      int refill, hipStream_t s)
+void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, const MeshWalkScratch* w,
+                      hipStream_t s)
 {
   if (a.n <= 0) return;
   if (face_order_kind == FO_MOAB)
-    launch_mesh_walk_t<FO_MOAB>(a, c, next_item, refill, s);
+    launch_mesh_walk_t<FO_MOAB>(a, c, w, s);
   else if (face_order_kind == FO_MOCK)
-    launch_mesh_walk_t<FO_MOCK>(a, c, next_item, refill, s);
+    launch_mesh_walk_t<FO_MOCK>(a, c, w, s);
   else
-    launch_mesh_walk_t<FO_TABLE>(a, c, next_item, refill, s);
+    launch_mesh_walk_t<FO_TABLE>(a, c, w, s);
 }
 
 void launch_next_element(const NextElementArgs& a, hipStream_t s)

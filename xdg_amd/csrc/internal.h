@@ -113,11 +113,12 @@ struct LaunchCfg {
   int defer = 1;  // mixed ray_fire: FP64 pass for far-origin rays
   int wide = 1;   // surface queries traverse the BVH4
   // resident-grid kernels with per-lane work refill: bit 0 plain ray_fire,
-  // bit 1 mixed ray_fire, bit 2 find_volume, bit 3 mesh walk
+  // bit 1 mixed ray_fire, bit 2 find_volume
   int persistent = 3;
   int order = 0;  // ray_fire coherence ordering mode (0 = off)
   int spec = 1;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
   int refill = 40; // persistent ray_fire / find_volume: idle lanes of a wave needed before a refill
+  int mw_wave = 1;  // mesh walk: batched re-entry rays (wavefront) instead of inline ones
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
@@ -341,10 +342,24 @@ struct MeshWalkArgs {
   uint64_t* hash;
   unsigned long long* tally_fx;   // nullable, per element, scale 2^40
 };
-// next_item (nullable): claim counter of the resident-grid kernel (zeroed by
-// the launcher); null runs one particle per lane
-void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, unsigned long long* next_item,
-                      int refill, hipStream_t s);
+// A particle's state between the kernels of the wavefront mesh walk.
+struct MwSave {
+  double r[3], u[3], dist;
+  unsigned long long hh;
+  int32_t e, nev;
+  uint32_t dc;
+  int32_t pad;
+};
+constexpr int MW_ROUNDS = 3;  // walk / batched re-entry rounds before the inline fallback
+struct MeshWalkScratch {
+  MwSave* save;               // n particles
+  int32_t* list[2];           // parked and re-entered particle lists, n each
+  unsigned int* count;        // 2 * MW_ROUNDS list counters
+};
+// w (nullable): scratch of the wavefront walk; null runs every particle
+// inline, one per lane
+void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, const MeshWalkScratch* w,
+                      hipStream_t s);
 // 0 = MOAB canonical, 1 = MeshMock, 2 = other (read from the tree)
 int face_order_kind(const int32_t* face_order);
 
