@@ -117,7 +117,9 @@ __device__ __forceinline__ void write_hit(const RayFireArgs& a, int64_t i, const
 // Persistent ray_fire over the BVH4.  A fixed grid of resident waves works
 // through the batch: a lane whose ray is finished writes its result and
 // takes the next unclaimed ray (one 64-bit atomic per wave per refill), so a
-// wave does not idle behind its slowest lane.  MIXED queues far-origin rays
+// wave does not idle behind its slowest lane.  A lane that reaches a leaf
+// while others still descend parks it and keeps descending until every
+// lane holds one (speculative traversal, Aila & Laine 2009).  MIXED queues far-origin rays
 // for the FP64 pass (k_ray_fire_deferred).  Every lane leaves the loop once
 // the counter passes n, so the grid always drains.
 template <bool MIXED>
@@ -140,8 +142,9 @@ __device__ __forceinline__ void ray_fire_pt_body(const RayFireArgs& a, int cap, 
   float tf = 0.0f;
   PlaneOfs po{0, 0, 0, 0, 0, 0};
   int32_t node = DONE;
+  int32_t leaf = DONE;  // a leaf parked while the wave keeps descending
   while (true) {
-    const bool done = node == DONE;
+    const bool done = node == DONE && leaf == DONE;
     if (done && have) {
       write_hit(a, ri, h);
       have = false;
@@ -228,13 +231,22 @@ __device__ __forceinline__ void ray_fire_pt_body(const RayFireArgs& a, int cap, 
       } else {
         node = S.empty() ? DONE : S.pop();
       }
+      if (node < 0 && node != DONE && leaf == DONE) {
+        leaf = node;
+        node = S.empty() ? DONE : S.pop();
+      }
+      if (__all(leaf != DONE)) break;
     }
-    if (node != DONE) {
-      const int32_t first = leaf_first(node);
-      const int32_t cnt = leaf_count(node);
+    if (leaf == DONE && node != DONE) {
+      leaf = node;
+      node = S.empty() ? DONE : S.pop();
+    }
+    if (leaf != DONE) {
+      const int32_t first = leaf_first(leaf);
+      const int32_t cnt = leaf_count(leaf);
+      leaf = DONE;
       for (int32_t j = 0; j < cnt; ++j) tri_closest_hit<RAYFIRE>(T, first + j, ray, a.orientation, ex, k, h);
       tf = f32_up(h.t);
-      node = S.empty() ? DONE : S.pop();
     }
   }
 }
