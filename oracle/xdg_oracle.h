@@ -33,6 +33,11 @@
  *   xo_find_element               analytic Kuhn-cell location + brute force
  *   xo_segments                   sum(len) == chord invariant + MeshMock
  *                                 walk golden sequences
+ *   xo_segments_volume            walk_elements(elem, start, end) sequences of
+ *                                 the compiled reference on MeshMock (starts
+ *                                 inside and outside the mesh); slab closed
+ *                                 forms of the entry on the unit cube (sum to
+ *                                 |end - entry|, to the chord, none on a miss)
  *   xo_philox4x32_10              Random123 known-answer vectors
  *   xo_log                        glibc log() within 2 ulp
  *   xo_transport                  no reference output exists
@@ -40,9 +45,26 @@
  *                                 drand48); pinned by the homogeneous-medium
  *                                 closed forms (mean track mfp/p, mean
  *                                 collisions 1/p), region-volume tally
- *                                 fractions and thread-count independence
+ *                                 fractions and thread-count independence;
+ *                                 boundary conditions: a reflecting cube
+ *                                 unfolds each history to a straight line
+ *                                 (events = integer planes crossed + 1), a
+ *                                 reflecting sphere keeps the mean track
+ *                                 mfp/p with no leak, vacuum spheres tally
+ *                                 nothing for outside starts
+ *   xo_mesh_walk                  no reference output exists (drand48,
+ *                                 rand_dir); pinned by straight flights
+ *                                 without collisions: slab exit distance of
+ *                                 the cube, and exit + gap + chord through a
+ *                                 second block (re-entry); tally conservation;
+ *                                 thread-count independence
  *   xo_adjoint                    central finite differences of the
  *                                 oracle's own ray_fire distances
+ *   pyoracle.OverlapQueries       the overlap check's queries composed of
+ *                                 xo_point_in_volume and xo_ray_fire; pinned
+ *                                 by a brute-force Moeller-Trumbore
+ *                                 classification of every vertex location
+ *                                 on four box models
  */
 #ifndef XDG_ORACLE_H
 #define XDG_ORACLE_H
