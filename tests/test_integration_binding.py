@@ -1,6 +1,7 @@
-"""The RayTracer subclass shown in INTEGRATION.md compiles against the
-reference's own headers, with the constants.h edit INTEGRATION.md lists
-applied to a scratch copy (CPU only; skipped where the reference tree is
+"""The RayTracer subclass of the integration guide (its C++ block that
+starts with `// file: include/xdg/hip/ray_tracer.h`) compiles against the
+reference's own headers, with the guide's constants.h edit (a HIP library
+id) applied to a scratch copy (CPU only; skipped where the reference tree is
 absent)."""
 import os
 import re
