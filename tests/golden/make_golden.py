@@ -1,7 +1,7 @@
 """Write golden vectors from the reference's own compiled sources.
 
 Runs only in the development container (it needs the reference tree):
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [walk_se|watertight]
 The driver oracle/_ref/ref_driver is built from the reference's
 include/xdg/geometry/plucker.h, face_common.h, src/geometry/closest.cpp and
 src/mesh_manager_interface.cpp (MeshMock from tests/mesh_mock.h).  Outputs
@@ -268,9 +268,68 @@ def walk_start_end():
           int(se_out[:, 0].max()))
 
 
+def sphere_targets(model):
+    """Vertices, then edge midpoints (edges as sorted vertex pairs, in
+    np.unique order), of a triangulated sphere, with the triangles that
+    share each target: (targets, per-target triangle lists)."""
+    v, t = model.verts, model.tris
+    e = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]])
+    e = np.unique(np.sort(e, axis=1), axis=0)
+    targets = np.concatenate([v, 0.5 * (v[e[:, 0]] + v[e[:, 1]])])
+    by_vert = [[] for _ in range(v.shape[0])]
+    for i, tri in enumerate(t):
+        for k in tri:
+            by_vert[int(k)].append(i)
+    adj = [sorted(x) for x in by_vert]
+    for a, b in e:
+        adj.append(sorted(set(by_vert[int(a)]) & set(by_vert[int(b)])))
+    return targets, adj
+
+
+def watertight():
+    """Rays from the centre of the subdivision-5 icosphere (r = 10) aimed at
+    every vertex and every edge midpoint, each against the triangles that
+    share its target, through the reference's plucker_ray_tri_intersect with
+    the arguments of TriangleIntersectionFunc (tMax = INFTY, tMin = 0, no
+    orientation; src/triangle_intersect.cpp:68-74).  Kept: every ray that
+    no sharing triangle accepts (a leak of the reference's test), and a
+    fixed sample of the others."""
+    model = G.single_sphere(5, 10.0)
+    targets, adj = sphere_targets(model)
+    d = targets / np.sqrt(targets[:, 0] * targets[:, 0] + targets[:, 1] * targets[:, 1]
+                          + targets[:, 2] * targets[:, 2])[:, None]
+    o = np.zeros(3)
+    rows, owner = [], []
+    for i, tl in enumerate(adj):
+        for f in tl:
+            tri = model.verts[model.tris[f]]
+            rows.append(np.concatenate([tri.ravel(), o, d[i], [INFTY, 0.0, 0.0, 0.0]]))
+            owner.append(i)
+    rows = np.asarray(rows)
+    owner = np.asarray(owner)
+    out = run("plucker", rows, 2)
+    hit_any = np.zeros(len(adj), bool)
+    np.logical_or.at(hit_any, owner, out[:, 0] != 0.0)
+    leaks = np.nonzero(~hit_any)[0]
+    nv = model.verts.shape[0]
+    sample = np.concatenate([np.arange(0, nv, 41), np.arange(nv, len(adj), 97)])
+    keep = np.union1d(leaks, sample)
+    sel = np.isin(owner, keep)
+    np.savez_compressed(os.path.join(HERE, "golden_watertight.npz"),
+                        ray=keep.astype(np.int32), dir=d[keep],
+                        leak=(~hit_any[keep]), n_vertices=np.int32(nv),
+                        case_ray=owner[sel].astype(np.int32),
+                        plucker_in=rows[sel], plucker_out=out[sel])
+    print("watertight: rays", len(adj), "leaks", leaks.tolist(),
+          "kept", keep.size, "cases", int(sel.sum()))
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["walk_se"]:
         walk_start_end()
+    elif sys.argv[1:] == ["watertight"]:
+        watertight()
     else:
         main()
         walk_start_end()
+        watertight()

@@ -231,6 +231,39 @@ def test_sphere_ray_fire_bitexact(ctx, torch_mod, sphere5, prec, orientation, wi
     assert np.array_equal(bits(host(t)), bits(rt))
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("wide", [True, False])
+def test_watertight_vertex_and_edge_rays(ctx, torch_mod, sphere5, golden_dir, prec, wide):
+    """Rays through every vertex and every edge midpoint of the icosphere,
+    from its centre and three other interior points: hits, misses and
+    distance bits equal to the oracle's.  From the centre, the two rays the
+    reference's own Plücker test leaks (golden_watertight.npz) miss."""
+    torch = torch_mod
+    m, tree, ot = sphere5
+    v, t3 = m.verts, m.tris
+    e = np.concatenate([t3[:, [0, 1]], t3[:, [1, 2]], t3[:, [2, 0]]])
+    e = np.unique(np.sort(e, axis=1), axis=0)
+    targets = np.concatenate([v, 0.5 * (v[e[:, 0]] + v[e[:, 1]])])
+    origins = np.concatenate([np.zeros((1, 3)), P.ball_points(7, P.STREAM_POS, 3, 4.0)])
+    org = np.repeat(origins, targets.shape[0], axis=0)
+    d = np.tile(targets, (origins.shape[0], 1)) - org
+    d = d / np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])[:, None]
+    g = np.load(os.path.join(golden_dir, "golden_watertight.npz"))
+    assert np.array_equal(bits(d[g["ray"]]), bits(g["dir"]))
+    ctx.set_traversal(wide=wide, defer=True)
+    try:
+        t, s, f = ctx.ray_fire(tree, dev(torch, org), dev(torch, d), precision=prec)
+    finally:
+        ctx.set_traversal()
+    rt, rs, rf = ot.ray_fire(org, d)
+    assert np.array_equal(host(s), rs)
+    assert np.array_equal(host(f), rf)
+    assert np.array_equal(bits(host(t)), bits(rt))
+    leaks = g["ray"][g["leak"]]
+    assert leaks.size and (host(s)[leaks] == -1).all()
+    assert (rs >= 0).mean() > 0.999
+
+
 @pytest.mark.parametrize("refill", [1, 17, 64])
 def test_persistent_refill_thresholds(ctx, torch_mod, sphere5, refill):
     """Persistent ray_fire with lanes refilled one at a time, in groups, or

@@ -202,3 +202,34 @@ def test_segments_volume_matches_global_inside():
         a = e1[o1[i]:o1[i] + c1[i]]
         b = e2[o2[i]:o2[i] + c2[i]]
         assert c1[i] < c2[i] and np.array_equal(a, b[:c1[i]])
+
+
+def test_watertight_sphere_golden(golden_dir):
+    """Rays from the centre of the subdivision-5 icosphere through vertices
+    and edge midpoints.  The reference's own plucker_ray_tri_intersect
+    leaks two of the 40 962 (no sharing triangle accepts: the shared edge's
+    coordinate is zeroed by the 20-epsilon tolerance and the barycentric
+    guard u + v > 1 then fails by rounding in both triangles).  The oracle
+    must give the same per-triangle bits, and its ray_fire over the whole
+    sphere must leak exactly those rays and hit every other one at the
+    nearest accepting triangle's distance."""
+    g = np.load(os.path.join(golden_dir, "golden_watertight.npz"))
+    cin, cout = g["plucker_in"], g["plucker_out"]
+    for c, want in zip(cin, cout):
+        h, t = O.plucker(c[:9], c[9:12], c[12:15], c[15], c[16], c[17] != 0, int(c[18]))
+        assert (float(h), bits(t)) == (want[0], bits(want[1]))
+    leak = g["leak"]
+    assert 0 < leak.sum() < leak.size
+    m = G.single_sphere(5, 10.0)
+    tree = O.Tree.from_model(m, 1)
+    d = g["dir"]
+    t, s, _ = tree.ray_fire(np.zeros_like(d), d)
+    assert np.array_equal(s < 0, leak)
+    case_ray = g["case_ray"]
+    for j, r in enumerate(g["ray"]):
+        acc = (case_ray == r) & (cout[:, 0] != 0.0)
+        if leak[j]:
+            assert not acc.any()
+            continue
+        assert s[j] == 1
+        assert t[j] == cout[acc, 1].min()
