@@ -155,3 +155,30 @@ def test_exclusion_bound(ctx, torch_mod):
     with pytest.raises(Exception):
         ctx.ray_fire(tree, dev(torch, org), dev(torch, dirs),
                      excl=dev(torch, np.zeros((n, 1025), np.int32)))
+
+
+def test_lattice_degenerate_segments(ctx, torch_mod):
+    """Segments along the edges and faces of an unjittered Kuhn mesh and
+    through its vertices (exit-face ties, zero-length entries): counts,
+    sequence hashes and tallies equal to the oracle's, in the global and
+    the volume form."""
+    from lattice_cases import lattice_segments
+    torch = torch_mod
+    mesh = G.kuhn_cube(8, 0.0, seed=3)
+    et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
+    bm = mesh.boundary_model()
+    ipc = build_tree(ctx, bm, bm.ipc)
+    vol = build_tree(ctx, bm, 1)
+    ctx.attach_boundary(et, ipc, mesh.bface_elem)
+    om = O.TetMesh(mesh)
+    ovol = O.Tree.from_model(bm, 1)
+    p0, p1 = lattice_segments(8)
+    tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device="cuda")
+    c, h = ctx.tally_segments(et, dev(torch, p0), dev(torch, p1), tally=tally)
+    rc, rh, rt = om.segments(p0, p1, tally=True)
+    assert np.array_equal(host(c), rc) and np.array_equal(host(h).view(np.uint64), rh)
+    assert np.array_equal(host(tally), rt)
+    rc, rh, rf = om.segments_volume(ovol, p0, p1)
+    c, h = ctx.tally_segments_volume(et, vol, dev(torch, p0), dev(torch, p1))
+    assert np.array_equal(host(c), rc) and np.array_equal(host(h).view(np.uint64), rh)
+    assert ctx.status() == (2 if (rf & 2).any() else 0)

@@ -133,3 +133,22 @@ def test_library_exports_header():
     assert len(declared) > 30
     missing = [f for f in declared if f not in exported]
     assert not missing, missing
+
+
+def test_lattice_segments_conserve_chord():
+    """On the unjittered Kuhn mesh, segments along lattice edges, faces and
+    through vertices: the oracle's per-segment lengths (zero-length entries
+    kept) still sum to the chord, and such ties do occur."""
+    from lattice_cases import lattice_segments
+    mesh = G.kuhn_cube(8, 0.0, seed=3)
+    om = O.TetMesh(mesh)
+    p0, p1 = lattice_segments(8)
+    cnt, _, (off, _, ln) = om.segments(p0, p1, sequences=np.full(len(p0), 64))
+    assert cnt.max() < 64
+    chord = np.sqrt(((p1 - p0) ** 2).sum(1))
+    zero = 0
+    for i in range(len(p0)):
+        seg = ln[off[i]:off[i] + cnt[i]]
+        assert abs(seg.sum() - chord[i]) <= 1e-14 * chord[i]
+        zero += int((seg == 0.0).sum())
+    assert zero > 100
