@@ -12,7 +12,10 @@
  *
  * Parity status per function (what pins it; see tests/test_oracle_*.py):
  *   xo_plucker_ray_tri_intersect  golden vectors from the compiled reference
- *                                 header + MeshMock known answers + closed form
+ *                                 header (random cases, and the vertex and
+ *                                 edge-midpoint rays of an icosphere, where
+ *                                 the reference leaks two rays) + MeshMock
+ *                                 known answers + closed form
  *   xo_face_normal                golden vectors (compiled face_common.h)
  *   xo_closest_location_on_triangle / xo_determine_region
  *                                 golden vectors (compiled closest.cpp) +
@@ -31,8 +34,10 @@
  *                                 test_point_in_volume.cpp, test_occluded.cpp,
  *                                 test_closest.cpp) + brute force == BVH
  *   xo_find_element               analytic Kuhn-cell location + brute force
- *   xo_segments                   sum(len) == chord invariant + MeshMock
- *                                 walk golden sequences
+ *   xo_segments                   sum(len) == chord invariant (random, and
+ *                                 along lattice edges, planes and vertices of
+ *                                 an unjittered Kuhn mesh) + MeshMock walk
+ *                                 golden sequences
  *   xo_segments_volume            walk_elements(elem, start, end) sequences of
  *                                 the compiled reference on MeshMock (starts
  *                                 inside and outside the mesh); slab closed
