@@ -5,7 +5,7 @@
  * entry point replaces one method of the reference's abstract RayTracer
  * (include/xdg/ray_tracing_interface.h:16-147) or the facade callers on the
  * hot path (src/xdg.cpp), batched: one call processes n queries.  A C++
- * RayTracer-shaped class over this ABI is in include/xdg_hip/hip_ray_tracer.h.
+ * RayTracer-shaped class over this ABI is in include/xdg_hip.hpp.
  *
  * Conventions
  *   - Plain pointers and sizes only.  "host" arrays are read during the call;
@@ -17,8 +17,25 @@
  *     code; xdg_hip_last_error(ctx) describes the last failure.  No C++
  *     exception crosses the ABI.
  *   - Calls on one context from several host threads are serialized.
+ *   - Streams: queries on different streams may overlap, except that the
+ *     context's shared scratch (the persistent kernels' work counters, the
+ *     mixed-precision deferral list, the ray-ordering buffers, the mesh
+ *     walk's particle store) serves one launch at a time.  The library
+ *     orders those launches itself: a ray_fire, find_volume or wavefront
+ *     mesh walk on stream B waits (hipStreamWaitEvent) for the last such
+ *     launch on stream A.  Under graph capture that ordering is the
+ *     caller's.  A growth of the scratch (a larger batch than before)
+ *     synchronizes the device.
  *   - Types follow include/xdg/constants.h: ids are int32, ID_NONE = -1,
  *     INFTY = DBL_MAX.  Tree ids start at 1 (ray_tracing_interface.cpp:10-18).
+ *   - Fixed-point tallies (order independent, so bit-exact for any launch
+ *     shape): tet-walk and mesh-walk tallies are int64 sum(llrint(len *
+ *     2^40)) per element, exact to 2^-40 and safe up to 2^62 / 2^40 =
+ *     4.19e6 length units per element; transport tallies are int64
+ *     sum(llrint(len * 2^32)) per volume slot, safe up to 2^30 = 1.07e9.
+ *     After each tallying call a check kernel sets status bit
+ *     XDG_HIP_STATUS_TALLY_RANGE when any entry of the tally array has
+ *     reached 2^62 in magnitude (or wrapped negative), before 2^63 wraps.
  *   - Tie rule (the only intended difference from Embree's visit-order
  *     dependence, triangle_intersect.cpp:78): among bitwise-equal distances
  *     the smallest face id wins; find_element returns the lowest containing
@@ -154,7 +171,8 @@ int xdg_hip_release_surface_tree(xdg_hip_ctx* ctx, int32_t tree);
 /* RayTracer::ray_fire (ray_tracing_interface.h:81-86; Embree
  * ray_tracer.cpp:290-326).  tree_per_ray (device, nullable) overrides
  * `tree` per ray.  tmax (device, nullable): dist_limit per ray (INFTY when
- * NULL).  excl (device, nullable): k <= 8 face ids per ray (-1 = empty).
+ * NULL).  excl (device, nullable): k <= XDG_HIP_MAX_EXCLUDE (1024) face
+ * ids per ray, read in place (-1 = empty).
  * Miss: t = INFTY, surf = face = -1.  The reference appends the hit face to
  * the caller's exclusion vector: face_out carries it. */
 int xdg_hip_ray_fire(xdg_hip_ctx* ctx, int32_t tree, const int32_t* tree_per_ray, int64_t n, const double* org,
@@ -280,9 +298,10 @@ int xdg_hip_mesh_walk(xdg_hip_ctx* ctx, int32_t element_tree, uint64_t first, in
                       double* distance_out, int32_t* events_out, uint8_t* status_out, uint64_t* hash_out,
                       int64_t* tally_fx, void* stream);
 
-/* Synchronize the device, return and clear the status word (bit 0: a walk of
- * the segment calls hit its step bound since the last call; bit 1: a
- * volume segment found no element behind its entry hit). */
+/* Synchronize the device, return and clear the status word (bits below). */
+#define XDG_HIP_STATUS_STEP_CAP 1u      /* a segment walk hit its step bound */
+#define XDG_HIP_STATUS_NO_ENTRY_ELEM 2u /* a volume segment found no element behind its entry hit */
+#define XDG_HIP_STATUS_TALLY_RANGE 4u   /* a fixed-point tally reached 2^62 in magnitude */
 int xdg_hip_status(xdg_hip_ctx* ctx, uint32_t* flags);
 
 /* Evaluate one device geometry primitive per item, for parity checks
@@ -367,7 +386,14 @@ int xdg_hip_transport_get_config(xdg_hip_ctx* ctx, int32_t handle, int* block, i
 int xdg_hip_transport_run(xdg_hip_ctx* ctx, int32_t handle, uint64_t first, int64_t n, int precision,
                           uint64_t* tally_fx, int32_t* events, uint8_t* status, uint64_t* hash, void* stream);
 /* The same launch captured into an instantiated HIP graph (opaque handle):
- * replay with xdg_hip_graph_launch, free with xdg_hip_graph_destroy. */
+ * replay with xdg_hip_graph_launch, free with xdg_hip_graph_destroy (before
+ * xdg_hip_destroy).  The graph holds device pointers of the capture time:
+ * the output arrays (kept alive by the caller), the tree table and trees,
+ * and the transport's buffers.  Building, releasing or attaching any tree
+ * on the context, destroying the transport, or a launch shape that needs a
+ * larger stack spill area (transport_set_config / autotune followed by a
+ * run) invalidates it: xdg_hip_graph_launch then returns
+ * XDG_HIP_ERR_INVALID without launching; capture again. */
 int xdg_hip_transport_capture(xdg_hip_ctx* ctx, int32_t handle, uint64_t first, int64_t n, int precision,
                               uint64_t* tally_fx, int32_t* events, uint8_t* status, uint64_t* hash, void** graph);
 int xdg_hip_graph_launch(xdg_hip_ctx* ctx, void* graph, void* stream);

@@ -65,6 +65,8 @@ def lib():
         L.xo_find_element.argtypes = [_vp, C.c_int64, _d, C.c_int, C.c_int,
                                       _i32]
         L.xo_next_element.argtypes = [_vp, C.c_int32, _d, _d, _i32, _d]
+        L.xo_set_fault.restype = None
+        L.xo_set_fault.argtypes = [C.c_int]
         L.xo_walk_elements.restype = C.c_int64
         L.xo_walk_elements.argtypes = [_vp, C.c_int32, _d, _d, C.c_double,
                                        C.c_int64, _i32, _d]
@@ -156,6 +158,29 @@ def closest_point(v9, p):
 
 def region(v9, p):
     return lib().xo_determine_region(_p(_f64(v9), _d), _p(_f64(p), _d))
+
+
+FAULT_NONE = 0
+FAULT_FIRST_VISITED = 1
+FAULT_NO_DILATION = 2
+FAULT_TET_TRANSPOSE = 3
+
+
+class planted_fault:
+    """Context manager that plants a mistake in the oracle (tests that show
+    a check fails on it).  Element trees built inside the block keep the
+    boxes they were built with."""
+
+    def __init__(self, code):
+        self.code = int(code)
+
+    def __enter__(self):
+        lib().xo_set_fault(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().xo_set_fault(FAULT_NONE)
+        return False
 
 
 def tet_contains(p, v12):

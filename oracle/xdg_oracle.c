@@ -34,6 +34,11 @@ static const float MIN_RCP_INPUT = FLT_MIN * 1E5;
 #define MAX_WALK_STEPS (1 << 16)
 #define MAX_SEG_LOOPS 64
 
+/* planted mistake for tests (xo_set_fault); XO_FAULT_NONE in normal use */
+static int g_fault = XO_FAULT_NONE;
+
+void xo_set_fault(int code) { g_fault = code; }
+
 /* ------------------------------------------------------------------------ */
 /* vector helpers                                                            */
 /* ------------------------------------------------------------------------ */
@@ -402,6 +407,10 @@ int xo_tet_contains(const double p[3], const double v[12], double* lam_out)
   double l[3];
   for (int k = 0; k < 3; ++k)
     l[k] = i0[k] * rhs[0] + i1[k] * rhs[1] + i2[k] * rhs[2];
+  if (g_fault == XO_FAULT_TET_TRANSPOSE) {
+    const double* ic[3] = {i0, i1, i2};
+    for (int k = 0; k < 3; ++k) l[k] = ic[k][0] * rhs[0] + ic[k][1] * rhs[1] + ic[k][2] * rhs[2];
+  }
   /* tetrahedron_contain.cpp:32: 1.0f - (l1 + l2 + l3) */
   double l0 = 1.0 - (l[0] + l[1] + l[2]);
   double b[4] = {l0, l[0], l[1], l[2]};
@@ -943,7 +952,7 @@ xo_tetmesh* xo_tetmesh_create(int64_t nv, const double* verts, int64_t ne,
         }
       double w[3];
       v_sub(hi, lo, w);
-      double bump = sqrt(v_dot(w, w)) * 1e-6;
+      double bump = g_fault == XO_FAULT_NO_DILATION ? 0.0 : sqrt(v_dot(w, w)) * 1e-6;
       for (int k = 0; k < 3; ++k) {
         blo[3 * e + k] = lo[k] - bump;
         bhi[3 * e + k] = hi[k] + bump;
@@ -993,7 +1002,10 @@ static int32_t find_element_one(const xo_tetmesh* m, const double* p,
         int32_t e = b->order[first + i];
         if (e >= best) continue;
         tet_coords(m, e, v);
-        if (xo_tet_contains(p, v, NULL)) best = e;
+        if (xo_tet_contains(p, v, NULL)) {
+          if (g_fault == XO_FAULT_FIRST_VISITED) return e;
+          best = e;
+        }
       }
     } else {
       stack[sp++] = nd->right;

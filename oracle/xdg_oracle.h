@@ -21,11 +21,13 @@
  *                                 golden vectors (compiled closest.cpp) +
  *                                 test_closest.cpp known answers
  *   xo_tet_contains               reference booleans (test_tet_containment.cpp,
- *                                 test_tet_intersection.cpp) + exact
- *                                 barycentric identities.  The lambda bits at
- *                                 face ties depend on the absent `linalg`
- *                                 library's inverse(): PARITY UNPINNED at
- *                                 exact-tie bits.
+ *                                 test_tet_intersection.cpp) + barycentric
+ *                                 identities (lambda(sum w_i v_i) = w to 1e-9,
+ *                                 vertices -> unit vectors, vertex permutations
+ *                                 permute lambda; a planted transposed inverse
+ *                                 fails).  The lambda bits at face ties depend
+ *                                 on the absent `linalg` library's inverse():
+ *                                 PARITY UNPINNED at exact-tie bits.
  *   xo_next_element / xo_walk_elements
  *                                 golden sequences from the compiled
  *                                 mesh_manager_interface.cpp on MeshMock
@@ -33,7 +35,15 @@
  *                                 MeshMock known answers (test_ray_fire.cpp,
  *                                 test_point_in_volume.cpp, test_occluded.cpp,
  *                                 test_closest.cpp) + brute force == BVH
- *   xo_find_element               analytic Kuhn-cell location + brute force
+ *   xo_find_element               analytic Kuhn-cell location on the
+ *                                 unjittered cube (unique element away from
+ *                                 faces; lowest index among the exact ties of
+ *                                 dyadic points on shared faces, edges,
+ *                                 vertices) + tree == brute force on a
+ *                                 jittered mesh, points a few ulps outside
+ *                                 the boundary included; planted mistakes
+ *                                 (first-visited element, undilated boxes)
+ *                                 fail (tests/test_oracle_find_element.py)
  *   xo_segments                   sum(len) == chord invariant (random, and
  *                                 along lattice edges, planes and vertices of
  *                                 an unjittered Kuhn mesh) + MeshMock walk
@@ -183,6 +193,19 @@ void xo_tetmesh_destroy(xo_tetmesh* m);
  * test passes, else -1. */
 void xo_find_element(const xo_tetmesh* m, int64_t n, const double* pts,
                      int brute, int nthreads, int32_t* elem_out);
+
+/* Planted mistakes, for tests that show a check can fail (0 = none, the
+ * default).  XO_FAULT_FIRST_VISITED: the element-tree find_element returns
+ * the first containing element in tree-visit order instead of the lowest
+ * index.  XO_FAULT_NO_DILATION: element trees created while it is set use
+ * undilated element boxes.  XO_FAULT_TET_TRANSPOSE: tet_contains
+ * multiplies by the transpose of the inverse.  Process-wide; not thread-safe against queries
+ * running at the same time. */
+#define XO_FAULT_NONE 0
+#define XO_FAULT_FIRST_VISITED 1
+#define XO_FAULT_NO_DILATION 2
+#define XO_FAULT_TET_TRANSPOSE 3 /* tet_contains applies the transposed inverse */
+void xo_set_fault(int code);
 
 /* MeshManager::next_element (mesh_manager_interface.cpp:150-200).
  * No exit face -> next = -1, dist = DBL_MAX (reference: UB). */

@@ -1,0 +1,158 @@
+"""GPU parity where the error bounds are tight.
+
+* Mixed precision: origins at 0.25, 0.5, 0.9 and 0.999 of the tree's
+  o_limit (the largest |origin|_inf the FP32 box test is proven for), aimed
+  back at the single sphere and at the nested spheres, with near-axis
+  directions (components 1e-20 and 1e-35, through the clamped FP32
+  reciprocal) and all three orientations, on the BVH4 and the binary tree:
+  the mixed kernels must trace every one of these rays in FP32 (no deferral)
+  and still equal the FP64 oracle bit for bit.
+* find_element on the unjittered Kuhn cube: the element follows from the
+  point alone (analytic location; the lowest index among exact ties on
+  shared faces, edges and vertices), and points just outside the boundary
+  are found through the per-element box dilation.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from xdg_amd import geometry as G
+from xdg_amd import philox as P
+
+import kuhn_cases as K
+from gpu_util import bits, build_tree, dev, host
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(torch_mod):
+    from xdg_amd import capi
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def limit_rays(seed, n, lim, scale, radius):
+    """n rays with |origin|_inf = scale * lim aimed at points inside radius;
+    every third ray starts on an axis with direction (-+1, +-1e-20, +-1e-35)."""
+    rng = np.random.default_rng(seed)
+    tgt = P.ball_points(seed, P.STREAM_POS, n, radius)
+    o = rng.normal(size=(n, 3))
+    o = o / np.abs(o).max(axis=1)[:, None] * (scale * lim)
+    d = tgt - o
+    d /= np.sqrt((d * d).sum(axis=1))[:, None]
+    ax_rays = np.arange(n) % 3 == 0
+    k = int(ax_rays.sum())
+    ax = rng.integers(0, 3, k)
+    sgn = rng.choice([-1.0, 1.0], k)
+    oa = tgt[ax_rays].copy()
+    oa[np.arange(k), ax] = sgn * (scale * lim)
+    da = np.zeros((k, 3))
+    da[np.arange(k), ax] = -sgn
+    da[np.arange(k), (ax + 1) % 3] = rng.choice([-1.0, 1.0], k) * 1e-20
+    da[np.arange(k), (ax + 2) % 3] = rng.choice([-1.0, 1.0], k) * 1e-35
+    o[ax_rays] = oa
+    d[ax_rays] = da
+    assert np.abs(o).max() <= lim
+    return o, d
+
+
+@pytest.mark.parametrize("wide", [True, False])
+def test_mixed_origins_up_to_o_limit_single_sphere(ctx, torch_mod, wide):
+    torch = torch_mod
+    m = G.single_sphere(5, 10.0)
+    tree = build_tree(ctx, m, 1)
+    ot = O.Tree.from_model(m, 1)
+    lim = ctx.tree_info(tree)["o_limit"]
+    assert 1000.0 < lim < 1100.0
+    ctx.set_traversal(wide=wide, defer=True)
+    try:
+        for i, scale in enumerate((0.25, 0.5, 0.9, 0.999)):
+            org, dirs = limit_rays(40 + i, 6000, lim, scale, 9.5)
+            d_org, d_dir = dev(torch, org), dev(torch, dirs)
+            for orientation in (-1, 0, 1):
+                rt, rs, rf = ot.ray_fire(org, dirs, orientation=orientation)
+                assert (rs >= 0).mean() > 0.95
+                for prec in (1, 0):
+                    t, s, f = ctx.ray_fire(tree, d_org, d_dir, orientation=orientation, precision=prec)
+                    assert np.array_equal(host(s), rs), (scale, orientation, prec)
+                    assert np.array_equal(host(f), rf), (scale, orientation, prec)
+                    assert np.array_equal(bits(host(t)), bits(rt)), (scale, orientation, prec)
+                st = ctx.ray_fire_stats(tree, d_org, d_dir, orientation=orientation, precision=1)
+                assert st["fallbacks"] == 0  # all traced with FP32 boxes
+            inside = ot.point_in_volume(org, dirs)
+            for prec in (1, 0):
+                got = host(ctx.point_in_volume(tree, d_org, d_dir, precision=prec)).astype(bool)
+                assert np.array_equal(got, inside)
+            hit_ref, _ = ot.occluded(org, dirs)
+            hit, _ = ctx.occluded(tree, d_org, d_dir, precision=1)
+            assert np.array_equal(host(hit).astype(bool), hit_ref)
+    finally:
+        ctx.set_traversal()
+
+
+@pytest.mark.parametrize("wide", [True, False])
+def test_mixed_origins_up_to_o_limit_nested(ctx, torch_mod, wide):
+    torch = torch_mod
+    m = G.nested_spheres(4)
+    vols = list(m.volumes)
+    trees = [build_tree(ctx, m, v) for v in vols]
+    otrees = [O.Tree.from_model(m, v) for v in vols]
+    ctx.set_traversal(wide=wide, defer=True)
+    try:
+        for v, tree, ot in zip(vols, trees, otrees):
+            if ot.n < 1000:
+                continue  # the 12-triangle implicit complement
+            lim = ctx.tree_info(tree)["o_limit"]
+            assert 1000.0 < lim < 1100.0
+            for i, scale in enumerate((0.25, 0.5, 0.9, 0.999)):
+                org, dirs = limit_rays(60 + 7 * v + i, 4000, lim, scale, 14.5)
+                d_org, d_dir = dev(torch, org), dev(torch, dirs)
+                for orientation in (-1, 0, 1):
+                    rt, rs, rf = ot.ray_fire(org, dirs, orientation=orientation)
+                    t, s, f = ctx.ray_fire(tree, d_org, d_dir, orientation=orientation, precision=1)
+                    assert np.array_equal(host(s), rs), (v, scale, orientation)
+                    assert np.array_equal(host(f), rf), (v, scale, orientation)
+                    assert np.array_equal(bits(host(t)), bits(rt)), (v, scale, orientation)
+                st = ctx.ray_fire_stats(tree, d_org, d_dir, orientation=0, precision=1)
+                assert st["fallbacks"] == 0
+        # find_volume at the limit of the outermost trees
+        lim = min(ctx.tree_info(t)["o_limit"] for t in trees)
+        org, dirs = limit_rays(99, 20000, lim, 0.999, 14.0)
+        ref = O.find_volume(otrees, vols, m.ipc, org, dirs)
+        for prec in (1, 0):
+            got = host(ctx.find_volume(trees, vols, m.ipc, dev(torch, org), dev(torch, dirs), precision=prec))
+            assert np.array_equal(got, ref)
+    finally:
+        ctx.set_traversal()
+
+
+def test_find_element_analytic_kuhn(ctx, torch_mod):
+    torch = torch_mod
+    n = 16
+    mesh = G.kuhn_cube(n, 0.0)
+    et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
+    pts, exp = K.interior_points(n, 20000, seed=11)
+    assert np.array_equal(host(ctx.find_element(et, dev(torch, pts))), exp)
+    dy = K.dyadic_points(n, 3, 20000, seed=12)
+    dexp = np.array([K.lowest_containing(n, p) for p in dy], dtype=np.int32)
+    assert np.array_equal(host(ctx.find_element(et, dev(torch, dy))), dexp)
+    # just outside the boundary, within the containment tolerance
+    rng = np.random.default_rng(13)
+    q = rng.random((4000, 3)) * 0.98 + 0.01
+    ax = rng.integers(0, 3, 4000)
+    side = rng.integers(0, 2, 4000)
+    q[np.arange(4000), ax] = np.where(side == 1, np.nextafter(1.0, 2.0), -(2.0 ** -56))
+    om = O.TetMesh(mesh, with_ipc=False)
+    brute = om.find_element(q, brute=True)
+    assert (brute >= 0).mean() > 0.99
+    assert np.array_equal(host(ctx.find_element(et, dev(torch, q))), brute)

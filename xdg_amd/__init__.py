@@ -4,7 +4,7 @@ Submodules:
   philox    counter-based RNG and sampling protocols (host side)
   geometry  synthetic surface models and tetrahedral meshes
   capi      ctypes binding of the C-ABI library libxdg_hip.so (HIP kernels)
-  raytracer batched Python front-end over the C-ABI (HipRayTracer)
-  autograd  torch.autograd.Function around the ray-distance adjoint
+  adjoint   torch.autograd.Function around the ray-distance adjoint
+  overlap   the overlap check's map built from the batched GPU queries
 """
 __all__ = ["philox", "geometry"]

@@ -62,6 +62,9 @@ class RayDistance(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, org, dirs, verts, surf: DifferentiableSurface):
+        for name, x in (("org", org), ("dirs", dirs), ("verts", verts)):
+            if x.dtype != torch.float64:
+                raise TypeError(f"ray_distance: {name} must be float64, got {x.dtype}")
         tree = surf.tree_for(verts)
         o = org.detach().contiguous()
         d = dirs.detach().contiguous()

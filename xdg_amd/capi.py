@@ -159,17 +159,24 @@ def lib():
     return _LIB
 
 
+F64 = (torch.float64,)
+I32 = (torch.int32,)
+I64 = (torch.int64, torch.uint64)  # 64-bit tallies and hashes (two's complement)
+U8 = (torch.uint8,)
+
+
 def _ptr(x):
-    """Device pointer of a torch tensor (or None)."""
+    """Device pointer of an already checked torch tensor (or None)."""
     if x is None:
         return None
-    if isinstance(x, torch.Tensor):
-        if not x.is_cuda:
-            raise ValueError("query arrays must be device tensors")
-        if not x.is_contiguous():
-            raise ValueError("query arrays must be contiguous")
-        return _vp(x.data_ptr())
-    raise TypeError(f"expected a torch tensor, got {type(x)}")
+    return _vp(x.data_ptr())
+
+
+def _count(x):
+    """Rows of a query array (its leading dimension)."""
+    if not isinstance(x, torch.Tensor):
+        raise TypeError(f"expected a torch tensor, got {type(x)}")
+    return x.shape[0] if x.dim() > 0 else 0
 
 
 def _ordered(fn):
@@ -211,6 +218,36 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def _arg(self, x, name, dtypes, numel=None, required=True, min_numel=None):
+        """Validate one device array before any launch: a contiguous torch
+        tensor on this context's device, of one of `dtypes`, holding exactly
+        `numel` (or at least `min_numel`) elements.  The C-ABI takes plain
+        pointers and cannot check sizes itself, so a wrong array here would
+        be read or written out of bounds."""
+        if x is None:
+            if required:
+                raise ValueError(f"{name} is required")
+            return None
+        if not isinstance(x, torch.Tensor):
+            raise TypeError(f"{name}: expected a torch tensor, got {type(x)}")
+        if not x.is_cuda:
+            raise ValueError(f"{name} must be a device tensor")
+        if x.device.index != self.device:
+            raise ValueError(f"{name} is on {x.device}, the context on cuda:{self.device}")
+        if x.dtype not in dtypes:
+            raise TypeError(f"{name} must be {' or '.join(str(d) for d in dtypes)}, got {x.dtype}")
+        if not x.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+        if numel is not None and x.numel() != numel:
+            raise ValueError(f"{name} must hold {numel} elements, has {x.numel()}")
+        if min_numel is not None and x.numel() < min_numel:
+            raise ValueError(f"{name} must hold at least {min_numel} elements, has {x.numel()}")
+        return x
+
+    def _prims(self, tree, element=False):
+        """Triangles / elements of a tree (sizes the per-element tallies)."""
+        return self.tree_info(tree, element)["n_prims"]
 
     def _check(self, rc):
         if rc != OK:
@@ -304,17 +341,37 @@ class Context:
         return s.value, e.value
 
     # ---- queries (device tensors) ------------------------------------------
+    def _rays(self, org, dirs, dirs_required=True, names=("org", "dirs")):
+        n = _count(org)
+        self._arg(org, names[0], F64, 3 * n)
+        self._arg(dirs, names[1], F64, 3 * n, required=dirs_required)
+        return n
+
+    def _excl(self, excl, n):
+        if excl is None:
+            return 0
+        k = excl.shape[1] if excl.dim() == 2 else -1
+        if k < 0 or k > MAX_EXCLUDE:
+            raise ValueError(f"excl must be (n, k) with k <= {MAX_EXCLUDE}")
+        self._arg(excl, "excl", I32, k * n)
+        return k
+
     @_ordered
     def ray_fire(self, tree, org, dirs, tmax=None, orientation=EXITING,
                  excl=None, precision=PLAIN, tree_per_ray=None, out=None,
                  stream=None):
-        n = org.shape[0]
+        n = self._rays(org, dirs)
+        self._arg(tmax, "tmax", F64, n, required=False)
+        self._arg(tree_per_ray, "tree_per_ray", I32, n, required=False)
+        k = self._excl(excl, n)
         if out is None:
             out = (torch.empty(n, dtype=torch.float64, device=org.device),
                    torch.empty(n, dtype=torch.int32, device=org.device),
                    torch.empty(n, dtype=torch.int32, device=org.device))
         t, s, f = out
-        k = 0 if excl is None else excl.shape[1]
+        self._arg(t, "t_out", F64, n)
+        self._arg(s, "surf_out", I32, n)
+        self._arg(f, "face_out", I32, n, required=False)
         self._done(lib().xdg_hip_ray_fire(
             self.h, tree, _ptr(tree_per_ray), n, _ptr(org), _ptr(dirs),
             _ptr(tmax), orientation, _ptr(excl), k, precision, _ptr(t),
@@ -326,9 +383,11 @@ class Context:
                        excl=None, precision=PLAIN, tree_per_ray=None,
                        stream=None):
         """Work counters of the instrumented traversal (synchronous)."""
-        n = org.shape[0]
+        n = self._rays(org, dirs)
+        self._arg(tmax, "tmax", F64, n, required=False)
+        self._arg(tree_per_ray, "tree_per_ray", I32, n, required=False)
+        k = self._excl(excl, n)
         cnt = (C.c_uint64 * 12)()
-        k = 0 if excl is None else excl.shape[1]
         self._check(lib().xdg_hip_ray_fire_stats(
             self.h, tree, _ptr(tree_per_ray), n, _ptr(org), _ptr(dirs),
             _ptr(tmax), orientation, _ptr(excl), k, precision, cnt))
@@ -340,9 +399,9 @@ class Context:
     @_ordered
     def point_in_volume(self, tree, pts, dirs=None, excl=None, precision=PLAIN,
                         stream=None):
-        n = pts.shape[0]
+        n = self._rays(pts, dirs, dirs_required=False, names=("pts", "dirs"))
+        k = self._excl(excl, n)
         out = torch.empty(n, dtype=torch.uint8, device=pts.device)
-        k = 0 if excl is None else excl.shape[1]
         self._done(lib().xdg_hip_point_in_volume(
             self.h, tree, n, _ptr(pts), _ptr(dirs), _ptr(excl), k, precision,
             _ptr(out), stream), stream)
@@ -351,11 +410,14 @@ class Context:
     @_ordered
     def find_volume(self, vol_trees, vol_ids, ipc, pts, dirs=None,
                     precision=PLAIN, out=None, stream=None):
-        n = pts.shape[0]
+        n = self._rays(pts, dirs, dirs_required=False, names=("pts", "dirs"))
         if out is None:
             out = torch.empty(n, dtype=torch.int32, device=pts.device)
+        self._arg(out, "vol_out", I32, n)
         vt, vtp = _host(vol_trees, np.int32)
         vi, vip = _host(vol_ids, np.int32)
+        if vt.size != vi.size:
+            raise ValueError("vol_trees and vol_ids differ in length")
         self._done(lib().xdg_hip_find_volume(
             self.h, vt.size, vtp, vip, ipc, n, _ptr(pts), _ptr(dirs), precision,
             _ptr(out), stream), stream)
@@ -366,11 +428,13 @@ class Context:
                          precision=PLAIN, stream=None):
         """point_in_volume of every point against every listed volume
         (tools/overlap.cpp:9-61).  Returns (count (n,), ids (n, max_k))."""
-        n = pts.shape[0]
+        n = self._rays(pts, dirs, names=("pts", "dirs"))
         cnt = torch.empty(n, dtype=torch.int32, device=pts.device)
         ids = torch.full((n, max_k), -1, dtype=torch.int32, device=pts.device)
         vt, vtp = _host(vol_trees, np.int32)
         vi, vip = _host(vol_ids, np.int32)
+        if vt.size != vi.size:
+            raise ValueError("vol_trees and vol_ids differ in length")
         self._done(lib().xdg_hip_point_in_volumes(
             self.h, vt.size, vtp, vip, n, _ptr(pts), _ptr(dirs), max_k,
             precision, _ptr(cnt), _ptr(ids), stream), stream)
@@ -382,12 +446,16 @@ class Context:
         """First EXITING hit within length over the listed volumes, skipping
         two volume ids per ray (tools/overlap.cpp:229-265).  Returns
         (t, surface, volume)."""
-        n = org.shape[0]
+        n = self._rays(org, dirs)
+        self._arg(length, "length", F64, n)
+        self._arg(skip, "skip", I32, 2 * n)
         t = torch.empty(n, dtype=torch.float64, device=org.device)
         surf = torch.empty(n, dtype=torch.int32, device=org.device)
         vol = torch.empty(n, dtype=torch.int32, device=org.device)
         vt, vtp = _host(vol_trees, np.int32)
         vi, vip = _host(vol_ids, np.int32)
+        if vt.size != vi.size:
+            raise ValueError("vol_trees and vol_ids differ in length")
         self._done(lib().xdg_hip_edge_rays(
             self.h, vt.size, vtp, vip, n, _ptr(org), _ptr(dirs), _ptr(length),
             _ptr(skip), precision, _ptr(t), _ptr(surf), _ptr(vol), stream),
@@ -396,7 +464,7 @@ class Context:
 
     @_ordered
     def occluded(self, tree, org, dirs, precision=PLAIN, stream=None):
-        n = org.shape[0]
+        n = self._rays(org, dirs)
         hit = torch.empty(n, dtype=torch.uint8, device=org.device)
         dist = torch.empty(n, dtype=torch.float64, device=org.device)
         self._done(lib().xdg_hip_occluded(self.h, tree, n, _ptr(org),
@@ -406,7 +474,8 @@ class Context:
 
     @_ordered
     def closest(self, tree, pts, stream=None):
-        n = pts.shape[0]
+        n = _count(pts)
+        self._arg(pts, "pts", F64, 3 * n)
         dist = torch.empty(n, dtype=torch.float64, device=pts.device)
         face = torch.empty(n, dtype=torch.int32, device=pts.device)
         self._done(lib().xdg_hip_closest(self.h, tree, n, _ptr(pts),
@@ -415,7 +484,8 @@ class Context:
 
     @_ordered
     def find_element(self, etree, pts, stream=None):
-        n = pts.shape[0]
+        n = _count(pts)
+        self._arg(pts, "pts", F64, 3 * n)
         out = torch.empty(n, dtype=torch.int32, device=pts.device)
         self._done(lib().xdg_hip_find_element(self.h, etree, n, _ptr(pts),
                                                _ptr(out), stream), stream)
@@ -423,7 +493,10 @@ class Context:
 
     @_ordered
     def next_element(self, etree, elem, r, u, stream=None):
-        n = elem.shape[0]
+        n = _count(elem)
+        self._arg(elem, "elem", I32, n)
+        self._arg(r, "r", F64, 3 * n)
+        self._arg(u, "u", F64, 3 * n)
         nx = torch.empty(n, dtype=torch.int32, device=r.device)
         d = torch.empty(n, dtype=torch.float64, device=r.device)
         self._done(lib().xdg_hip_next_element(self.h, etree, n, _ptr(elem),
@@ -431,14 +504,25 @@ class Context:
                                                _ptr(d), stream), stream)
         return nx, d
 
-    @_ordered
-    def tally_segments(self, etree, p0, p1, tally=None, seq_off=None,
-                       seq_elem=None, seq_len=None, out=None, stream=None):
-        n = p0.shape[0]
+    def _segment_args(self, etree, p0, p1, tally, seq_off, seq_elem, seq_len, out):
+        n = self._rays(p0, p1, names=("p0", "p1"))
+        self._arg(tally, "tally", I64, self._prims(etree, element=True), required=False)
+        if seq_off is not None:
+            self._arg(seq_off, "seq_off", (torch.int64,), n + 1)
+            total = int(seq_off[-1].item()) if n > 0 else 0
+            self._arg(seq_elem, "seq_elem", I32, min_numel=total)
+            self._arg(seq_len, "seq_len", F64, min_numel=total)
         if out is None:
             out = (torch.empty(n, dtype=torch.int32, device=p0.device),
                    torch.empty(n, dtype=torch.int64, device=p0.device))
-        cnt, hsh = out
+        self._arg(out[0], "count_out", I32, n)
+        self._arg(out[1], "hash_out", I64, n)
+        return n, out
+
+    @_ordered
+    def tally_segments(self, etree, p0, p1, tally=None, seq_off=None,
+                       seq_elem=None, seq_len=None, out=None, stream=None):
+        n, (cnt, hsh) = self._segment_args(etree, p0, p1, tally, seq_off, seq_elem, seq_len, out)
         self._done(lib().xdg_hip_tally_segments(
             self.h, etree, n, _ptr(p0), _ptr(p1), _ptr(tally), _ptr(cnt),
             _ptr(hsh), _ptr(seq_off), _ptr(seq_elem), _ptr(seq_len), stream), stream)
@@ -450,11 +534,7 @@ class Context:
                               out=None, stream=None):
         """XDG::segments(volume, start, end) (src/xdg.cpp:200-227): etree and
         stree are the volume's element and surface trees."""
-        n = p0.shape[0]
-        if out is None:
-            out = (torch.empty(n, dtype=torch.int32, device=p0.device),
-                   torch.empty(n, dtype=torch.int64, device=p0.device))
-        cnt, hsh = out
+        n, (cnt, hsh) = self._segment_args(etree, p0, p1, tally, seq_off, seq_elem, seq_len, out)
         self._done(lib().xdg_hip_tally_segments_volume(
             self.h, etree, stree, n, _ptr(p0), _ptr(p1), _ptr(tally),
             _ptr(cnt), _ptr(hsh), _ptr(seq_off), _ptr(seq_elem),
@@ -466,6 +546,7 @@ class Context:
                   max_events=100000, tally=None, stream=None):
         """Element-by-element particle walk (tools/walk_elements.h:53-117).
         Returns (distance, events, status, hash) device tensors."""
+        self._arg(tally, "tally", I64, self._prims(etree, element=True), required=False)
         dev = self.dev
         dist = torch.empty(n, dtype=torch.float64, device=dev)
         ev = torch.empty(n, dtype=torch.int32, device=dev)
@@ -543,11 +624,18 @@ class Context:
                        hash=z(n, dtype=torch.int64))
         return out
 
+    def _transport_out(self, out, nvol, n):
+        self._arg(out["tally_fx"], "tally_fx", I64, nvol)
+        self._arg(out.get("events"), "events", I32, n, required=False)
+        self._arg(out.get("status"), "status", U8, n, required=False)
+        self._arg(out.get("hash"), "hash", I64, n, required=False)
+
     @_ordered
     def transport_run(self, handle, first, n, nvol, precision=MIXED,
                       out=None, per_history=True, stream=None):
         if out is None:
             out = self.transport_outputs(nvol, n, per_history)
+        self._transport_out(out, nvol, n)
         self._done(lib().xdg_hip_transport_run(
             self.h, handle, int(first), int(n), precision,
             _ptr(out["tally_fx"]), _ptr(out.get("events")),
@@ -556,6 +644,7 @@ class Context:
 
     def transport_capture(self, handle, first, n, out, precision=MIXED):
         """Instantiated graph of one transport launch writing into out."""
+        self._transport_out(out, out["tally_fx"].numel(), n)
         g = _vp()
         self._check(lib().xdg_hip_transport_capture(
             self.h, handle, int(first), int(n), precision,
@@ -581,10 +670,19 @@ class Context:
                              dldv=None, dldo=None, dldd=None, stream=None):
         """dL/d(verts) accumulated into dldv; dL/d(org), dL/d(dir) written
         into dldo / dldd (all device tensors; None skips)."""
-        n = org.shape[0]
+        n = self._rays(org, dirs)
+        nv, ntri = _count(verts), _count(tris)
+        self._arg(t, "t", F64, n)
+        self._arg(face, "face", I32, n)
+        self._arg(dldt, "dldt", F64, n)
+        self._arg(verts, "verts", F64, 3 * nv)
+        self._arg(tris, "tris", I32, 3 * ntri)
+        self._arg(dldv, "dldv", F64, 3 * nv, required=False)
+        self._arg(dldo, "dldo", F64, 3 * n, required=False)
+        self._arg(dldd, "dldd", F64, 3 * n, required=False)
         self._done(lib().xdg_hip_ray_distance_adjoint(
             self.h, n, _ptr(org), _ptr(dirs), _ptr(t), _ptr(face), _ptr(dldt),
-            verts.shape[0], _ptr(verts), tris.shape[0], _ptr(tris),
+            nv, _ptr(verts), ntri, _ptr(tris),
             _ptr(dldv), _ptr(dldo), _ptr(dldd), stream), stream)
 
     def status(self):

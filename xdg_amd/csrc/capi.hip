@@ -41,6 +41,18 @@ struct TransportHost {
   size_t spill_entries = 0;
   unsigned long long* d_next = nullptr;
   std::vector<int32_t> host_trees;
+  uint64_t generation = 1;  // bumped when the spill area moves
+};
+
+// An instantiated transport graph and what it was captured against: a
+// replay after the context's or the transport's buffers were freed or moved
+// is refused instead of reading freed memory.
+struct GraphHandle {
+  hipGraphExec_t exec = nullptr;
+  const void* ctx = nullptr;
+  uint64_t ctx_generation = 0;
+  int32_t transport = 0;
+  uint64_t transport_generation = 0;
 };
 
 struct ElemTreeHost {
@@ -100,6 +112,14 @@ struct xdg_hip_ctx {
   int64_t order_cap = 0;
   RayOrderScratch order_w{};
   std::vector<TransportHost*> transports;  // handle = index + 1 (nullptr once destroyed)
+  // bumped whenever device memory a captured graph may point at is freed or
+  // moved (tree tables regrown, trees released); graphs remember the value
+  uint64_t generation = 1;
+  // the shared scratch above (d_next, d_fb_list/d_fb_count, d_order, d_mw)
+  // serves one launch at a time: the last stream that used it and an event
+  // after that use, so that a call on another stream waits for it
+  hipStream_t scratch_stream = nullptr;
+  hipEvent_t scratch_ev = nullptr;
   LaunchCfg cfg;
   int leaf_max = 2;
   int max_surf_depth = 1;
@@ -112,6 +132,7 @@ namespace {
 void sync_tables(xdg_hip_ctx* c)
 {
   XDG_HIP_CHECK(hipDeviceSynchronize());
+  ++c->generation;
   size_t ns = c->surf.size() + 1;
   if (ns > c->d_surf_cap) {
     if (c->d_surf) XDG_HIP_CHECK(hipFree(c->d_surf));
@@ -155,6 +176,51 @@ const ElemTreeHost& elem_tree(xdg_hip_ctx* c, int32_t tree)
 }
 
 hipStream_t pick(xdg_hip_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+
+bool capturing(hipStream_t s)
+{
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  XDG_HIP_CHECK(hipStreamIsCapturing(s, &cs));
+  return cs != hipStreamCaptureStatusNone;
+}
+
+// Before a launch that uses the context's shared scratch on stream s: wait
+// for the last such launch if it went to another stream.  (Not under graph
+// capture: a captured launch is ordered by the caller.)
+void scratch_acquire(xdg_hip_ctx* c, hipStream_t s)
+{
+  if (c->scratch_stream && c->scratch_stream != s && !capturing(s))
+    XDG_HIP_CHECK(hipStreamWaitEvent(s, c->scratch_ev, 0));
+}
+
+// After that launch: later users on other streams wait for this point.
+void scratch_release(xdg_hip_ctx* c, hipStream_t s)
+{
+  if (capturing(s)) return;
+  XDG_HIP_CHECK(hipEventRecord(c->scratch_ev, s));
+  c->scratch_stream = s;
+}
+
+__global__ void k_tally_guard(const long long* __restrict__ tally, int64_t n, uint32_t* status, uint32_t bit)
+{
+  const long long lim = 1LL << 62;
+  bool near = false;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const long long v = tally[i];
+    near |= v >= lim || v < -lim;
+  }
+  if (near) atomicOr(status, bit);
+}
+
+// Sets `bit` of the status word when an int64 fixed-point tally has reached
+// 2^62 in magnitude (half its range; the sums are of non-negative lengths,
+// so a wrapped sum reads negative and is caught as well).
+void tally_guard(xdg_hip_ctx* c, const void* tally, int64_t n, uint32_t bit, hipStream_t s)
+{
+  if (!tally || n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+  k_tally_guard<<<dim3((unsigned)blocks), dim3(256), 0, s>>>((const long long*)tally, n, c->d_status, bit);
+}
 
 // traversal stack entries a surface query needs: one tree, or every tree
 // (per-ray trees)
@@ -282,6 +348,14 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
     delete c;
     return XDG_HIP_ERR_HIP;
   }
+  if (hipEventCreateWithFlags(&c->scratch_ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipFree(c->d_next);
+    (void)hipFree(c->d_fb_count);
+    (void)hipFree(c->d_status);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return XDG_HIP_ERR_HIP;
+  }
   c->cfg.block = env_int("XDG_HIP_BLOCK", 64);
   c->cfg.stack = env_int("XDG_HIP_STACK", 1);
   c->leaf_max = env_int("XDG_HIP_LEAF", 2);
@@ -325,6 +399,7 @@ int xdg_hip_destroy(xdg_hip_ctx* c)
     if (c->d_fb_count) (void)hipFree(c->d_fb_count);
     if (c->d_next) (void)hipFree(c->d_next);
     if (c->d_order) (void)hipFree(c->d_order);
+    if (c->scratch_ev) (void)hipEventDestroy(c->scratch_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
   }
   delete c;
@@ -507,7 +582,12 @@ int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
         t.dev.box_lo[k] = (float)lo;
         t.dev.box_hi[k] = (float)hi;
       }
-      double lim = bump * 1048576.0 - lb.max_abs;  // bump * 2^20 - max |coordinate|
+      // FP32 box test error for origin o: at most (3u max|x| + 5u |o|) / |d_i|
+      // in ray-parameter units (u = 2^-24) against a margin of bump / |d_i|,
+      // so bump * 2^20 - max|x| keeps it below 5/16 of the margin.  The 2^26
+      // cap keeps fl32(o) times the clamped reciprocal 1e30 of a near-axis
+      // direction finite.
+      double lim = std::min(bump * 1048576.0 - lb.max_abs, 67108864.0);
       t.dev.o_limit = lim > 0.0 ? (float)lim : 0.0f;
       if ((double)t.dev.o_limit > lim) t.dev.o_limit = nextafterf(t.dev.o_limit, 0.0f);
       t.n_nodes = lb.n_nodes;
@@ -709,12 +789,15 @@ int xdg_hip_ray_fire(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per_ray, 
       a.fb_count = c->d_fb_count;
     }
     a.next_ray = c->d_next;
+    const hipStream_t s = pick(c, stream);
+    scratch_acquire(c, s);
     if (c->cfg.order > 0 && n >= ORDER_MIN_RAYS) {
       RayOrderArgs o{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir};
-      a.order = launch_ray_order(o, c->cfg.order, order_scratch(c, n), pick(c, stream));
+      a.order = launch_ray_order(o, c->cfg.order, order_scratch(c, n), s);
     }
-    launch_ray_fire(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), pick(c, stream));
+    launch_ray_fire(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), s);
     XDG_HIP_CHECK(hipGetLastError());
+    scratch_release(c, s);
     return XDG_HIP_OK;
   });
 }
@@ -739,12 +822,14 @@ int xdg_hip_ray_fire_stats(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per
       XDG_HIP_CHECK(hipMemcpy(d_cnt, counters, XDG_HIP_NUM_COUNTERS * sizeof(uint64_t), hipMemcpyHostToDevice));
       RayFireArgs a{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir, tmax, orientation,
                     excl, excl ? k : 0, nullptr, nullptr, nullptr};
+      scratch_acquire(c, c->stream);
       if (c->cfg.order > 0 && n >= ORDER_MIN_RAYS) {
         RayOrderArgs o{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir};
         a.order = launch_ray_order(o, c->cfg.order, order_scratch(c, n), c->stream);
       }
       launch_ray_fire_stats(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), d_cnt, c->stream);
       XDG_HIP_CHECK(hipGetLastError());
+      scratch_release(c, c->stream);
       XDG_HIP_CHECK(hipStreamSynchronize(c->stream));
       XDG_HIP_CHECK(hipMemcpy(counters, d_cnt, XDG_HIP_NUM_COUNTERS * sizeof(uint64_t), hipMemcpyDeviceToHost));
     } catch (...) {
@@ -844,8 +929,10 @@ int xdg_hip_find_volume(xdg_hip_ctx* c, int nvol, const int32_t* vol_trees, cons
       a.fb_list = fb_list_for(c, n);
       a.fb_count = c->d_fb_count;
     }
+    scratch_acquire(c, s);
     launch_find_volume(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), s);
     XDG_HIP_CHECK(hipGetLastError());
+    scratch_release(c, s);
     return XDG_HIP_OK;
   });
 }
@@ -1005,6 +1092,7 @@ int xdg_hip_tally_segments(xdg_hip_ctx* c, int32_t element_tree, int64_t n, cons
     a.seq_len = seq_len;
     a.status = c->d_status;
     launch_segments(a, face_order_kind(t.dev.face_order), cfg_for(c, depth), pick(c, stream));
+    tally_guard(c, tally_fx, t.dev.n_prims, XDG_HIP_STATUS_TALLY_RANGE, pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
   });
@@ -1042,6 +1130,7 @@ int xdg_hip_tally_segments_volume(xdg_hip_ctx* c, int32_t element_tree, int32_t 
     a.volume_mode = 1;
     a.surf_tree = surface_tree;
     launch_segments(a, face_order_kind(t.dev.face_order), cfg_for(c, depth), pick(c, stream));
+    tally_guard(c, tally_fx, t.dev.n_prims, XDG_HIP_STATUS_TALLY_RANGE, pick(c, stream));
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
   });
@@ -1085,9 +1174,12 @@ int xdg_hip_mesh_walk(xdg_hip_ctx* c, int32_t element_tree, uint64_t first, int6
     a.tally_fx = (unsigned long long*)tally_fx;
     const LaunchCfg cfg = cfg_for(c, depth);
     const bool wave = cfg.mw_wave && t.dev.ipc_tree >= 1;
-    launch_mesh_walk(a, face_order_kind(t.dev.face_order), cfg, wave ? &mw_scratch_for(c, n) : nullptr,
-                     pick(c, stream));
+    const hipStream_t s = pick(c, stream);
+    if (wave) scratch_acquire(c, s);
+    launch_mesh_walk(a, face_order_kind(t.dev.face_order), cfg, wave ? &mw_scratch_for(c, n) : nullptr, s);
+    tally_guard(c, tally_fx, t.dev.n_prims, XDG_HIP_STATUS_TALLY_RANGE, s);
     XDG_HIP_CHECK(hipGetLastError());
+    if (wave) scratch_release(c, s);
     return XDG_HIP_OK;
   });
 }
@@ -1208,6 +1300,7 @@ TransportLaunch transport_prepare(xdg_hip_ctx* c, TransportHost& t, bool mixed, 
     t.spill_entries = 0;
     XDG_HIP_CHECK(hipMalloc(&t.spill, need * sizeof(int32_t)));
     t.spill_entries = need;
+    ++t.generation;
   }
   return L;
 }
@@ -1370,6 +1463,7 @@ int xdg_hip_transport_run(xdg_hip_ctx* c, int32_t handle, uint64_t first, int64_
     const bool mixed = precision == XDG_HIP_MIXED;
     const TransportLaunch L = transport_prepare(c, t, mixed, cs == hipStreamCaptureStatusNone);
     transport_launch(c, t, L, mixed, first, n, tally_fx, events, status, hash, s);
+    tally_guard(c, tally_fx, t.dev.nvol, XDG_HIP_STATUS_TALLY_RANGE, s);
     XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
   });
@@ -1390,6 +1484,7 @@ int xdg_hip_transport_capture(xdg_hip_ctx* c, int32_t handle, uint64_t first, in
     XDG_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     try {
       transport_launch(c, t, L, mixed, first, n, tally_fx, events, status, hash, c->stream);
+      tally_guard(c, tally_fx, t.dev.nvol, XDG_HIP_STATUS_TALLY_RANGE, c->stream);
     } catch (...) {
       (void)hipStreamEndCapture(c->stream, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -1400,7 +1495,8 @@ int xdg_hip_transport_capture(xdg_hip_ctx* c, int32_t handle, uint64_t first, in
     const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     XDG_HIP_CHECK(e);
-    *graph = (void*)ex;
+    auto* gh = new GraphHandle{ex, c, c->generation, handle, t.generation};
+    *graph = (void*)gh;
     return XDG_HIP_OK;
   });
 }
@@ -1409,7 +1505,15 @@ int xdg_hip_graph_launch(xdg_hip_ctx* c, void* graph, void* stream)
 {
   return guarded(c, [&]() {
     require(graph != nullptr, "null graph");
-    XDG_HIP_CHECK(hipGraphLaunch((hipGraphExec_t)graph, pick(c, stream)));
+    const GraphHandle* g = (const GraphHandle*)graph;
+    require(g->ctx == c, "graph was captured on another context");
+    require(g->ctx_generation == c->generation,
+            "stale graph: trees were built, released or attached since the capture; capture it again");
+    require(g->transport >= 1 && (size_t)g->transport <= c->transports.size() && c->transports[g->transport - 1],
+            "stale graph: its transport problem was destroyed");
+    require(c->transports[g->transport - 1]->generation == g->transport_generation,
+            "stale graph: the transport's stack spill area moved since the capture; capture it again");
+    XDG_HIP_CHECK(hipGraphLaunch(g->exec, pick(c, stream)));
     return XDG_HIP_OK;
   });
 }
@@ -1417,7 +1521,12 @@ int xdg_hip_graph_launch(xdg_hip_ctx* c, void* graph, void* stream)
 int xdg_hip_graph_destroy(xdg_hip_ctx* c, void* graph)
 {
   return guarded(c, [&]() {
-    if (graph) XDG_HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph));
+    if (!graph) return XDG_HIP_OK;
+    GraphHandle* g = (GraphHandle*)graph;
+    require(g->ctx == c, "graph was captured on another context");
+    const hipError_t e = hipGraphExecDestroy(g->exec);
+    delete g;
+    XDG_HIP_CHECK(e);
     return XDG_HIP_OK;
   });
 }
