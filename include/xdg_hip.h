@@ -179,6 +179,24 @@ int xdg_hip_ray_fire(xdg_hip_ctx* ctx, int32_t tree, const int32_t* tree_per_ray
                      const double* dir, const double* tmax, int orientation, const int32_t* excl, int k,
                      int precision, double* t_out, int32_t* surf_out, int32_t* face_out, void* stream);
 
+/* xdg_hip_ray_fire without any device allocation, for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  No counterpart in the reference.  Same arguments,
+ * results and kernels as xdg_hip_ray_fire; its scratch (work counter and
+ * the mixed kernel's deferral list) is one array per GPU in the library,
+ * 16 MiB, held until the process ends, instead of context buffers grown on
+ * demand.  n must not exceed XDG_HIP_NOALLOC_MAX_RAYS (XDG_HIP_ERR_INVALID,
+ * nothing launched); rays are traced in the caller's order (the
+ * "ray_order" option is ignored).  All calls on one GPU share the array:
+ * calls that overlap on one GPU (other streams, other contexts) race, so
+ * order them.  Counters are cleared by kernels, not memsets, so the call
+ * may be captured. */
+#define XDG_HIP_NOALLOC_MAX_RAYS 4194304
+int xdg_hip_ray_fire_noalloc(xdg_hip_ctx* ctx, int32_t tree, const int32_t* tree_per_ray, int64_t n,
+                             const double* org, const double* dir, const double* tmax, int orientation,
+                             const int32_t* excl, int k, int precision, double* t_out, int32_t* surf_out,
+                             int32_t* face_out, void* stream);
+
 /* Instrumented xdg_hip_ray_fire (same traversal, no outputs) for roofline
  * accounting; synchronous.  counters (host, XDG_HIP_NUM_COUNTERS): rays, BVH
  * nodes visited, triangle tests, Pluecker edge tests, tests reaching the

@@ -31,6 +31,7 @@ ANY = -1
 EXITING = 0
 ENTERING = 1
 MAX_EXCLUDE = 1024
+NOALLOC_MAX_RAYS = 4194304  # XDG_HIP_NOALLOC_MAX_RAYS
 
 INFTY = np.finfo(np.float64).max
 
@@ -99,6 +100,9 @@ def lib():
             "xdg_hip_release_surface_tree": [_vp, _i32],
             "xdg_hip_ray_fire": [_vp, _i32, _vp, _i64, _vp, _vp, _vp, C.c_int,
                                  _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp],
+            "xdg_hip_ray_fire_noalloc": [_vp, _i32, _vp, _i64, _vp, _vp, _vp,
+                                         C.c_int, _vp, C.c_int, C.c_int, _vp,
+                                         _vp, _vp, _vp],
             "xdg_hip_ray_fire_stats": [_vp, _i32, _vp, _i64, _vp, _vp, _vp,
                                        C.c_int, _vp, C.c_int, C.c_int, _vp],
             "xdg_hip_point_in_volume": [_vp, _i32, _i64, _vp, _vp, _vp, C.c_int,
@@ -373,6 +377,26 @@ class Context:
         self._arg(s, "surf_out", I32, n)
         self._arg(f, "face_out", I32, n, required=False)
         self._done(lib().xdg_hip_ray_fire(
+            self.h, tree, _ptr(tree_per_ray), n, _ptr(org), _ptr(dirs),
+            _ptr(tmax), orientation, _ptr(excl), k, precision, _ptr(t),
+            _ptr(s), _ptr(f), stream), stream)
+        return t, s, f
+
+    @_ordered
+    def ray_fire_noalloc(self, tree, org, dirs, out, tmax=None, orientation=EXITING,
+                         excl=None, precision=PLAIN, tree_per_ray=None, stream=None):
+        """xdg_hip_ray_fire_noalloc: ray_fire into caller-provided outputs
+        (t, surf, face) with no device allocation by the library; at most
+        NOALLOC_MAX_RAYS rays; calls that overlap on one GPU race."""
+        n = self._rays(org, dirs)
+        self._arg(tmax, "tmax", F64, n, required=False)
+        self._arg(tree_per_ray, "tree_per_ray", I32, n, required=False)
+        k = self._excl(excl, n)
+        t, s, f = out
+        self._arg(t, "t_out", F64, n)
+        self._arg(s, "surf_out", I32, n)
+        self._arg(f, "face_out", I32, n, required=False)
+        self._done(lib().xdg_hip_ray_fire_noalloc(
             self.h, tree, _ptr(tree_per_ray), n, _ptr(org), _ptr(dirs),
             _ptr(tmax), orientation, _ptr(excl), k, precision, _ptr(t),
             _ptr(s), _ptr(f), stream), stream)

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -297,6 +298,48 @@ const MeshWalkScratch& mw_scratch_for(xdg_hip_ctx* c, int64_t n)
 // rays below this count are traced in the caller's order
 constexpr int64_t ORDER_MIN_RAYS = 1 << 16;
 
+// ---- scratch of the allocation-free ray_fire --------------------------------
+// One array per GPU, in the code object, held until the process ends: the
+// persistent kernels' work counter and the mixed kernel's deferral list of
+// up to XDG_HIP_NOALLOC_MAX_RAYS rays.  Every allocation-free call on a GPU
+// shares it, so two such calls that overlap on one GPU race.
+struct NoAllocScratch {
+  unsigned long long next[4];  // [0]: next unclaimed ray of the persistent kernels
+  unsigned int fb_count[4];    // [0]: rays deferred to the FP64 pass
+  int32_t fb_list[XDG_HIP_NOALLOC_MAX_RAYS];
+};
+
+}  // namespace
+
+static __device__ __align__(256) NoAllocScratch scratch;
+
+namespace {
+
+// Device address of `scratch` on each GPU (host code never uses the array's
+// name as an address: it is a host-side stand-in).  Filled once per device
+// by xdg_hip_create with that device current.
+std::mutex g_scratch_mu;
+std::vector<NoAllocScratch*> g_scratch_ptr;
+
+hipError_t scratch_setup(int device)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  g_scratch_ptr[device] = (NoAllocScratch*)p;
+  return hipSuccess;
+}
+
+NoAllocScratch* scratch_for(int device)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (size_t)device < g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+
 template <class F>
 int guarded(xdg_hip_ctx* c, F&& f)
 {
@@ -348,7 +391,9 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
     delete c;
     return XDG_HIP_ERR_HIP;
   }
-  if (hipEventCreateWithFlags(&c->scratch_ev, hipEventDisableTiming) != hipSuccess) {
+  if (hipEventCreateWithFlags(&c->scratch_ev, hipEventDisableTiming) != hipSuccess ||
+      scratch_setup(device) != hipSuccess) {
+    if (c->scratch_ev) (void)hipEventDestroy(c->scratch_ev);
     (void)hipFree(c->d_next);
     (void)hipFree(c->d_fb_count);
     (void)hipFree(c->d_status);
@@ -798,6 +843,38 @@ int xdg_hip_ray_fire(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per_ray, 
     launch_ray_fire(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), s);
     XDG_HIP_CHECK(hipGetLastError());
     scratch_release(c, s);
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_ray_fire_noalloc(xdg_hip_ctx* c, int32_t tree, const int32_t* tree_per_ray, int64_t n, const double* org,
+                             const double* dir, const double* tmax, int orientation, const int32_t* excl, int k,
+                             int precision, double* t_out, int32_t* surf_out, int32_t* face_out, void* stream)
+{
+  return guarded(c, [&]() {
+    require(n >= 0, "negative n");
+    require(n <= XDG_HIP_NOALLOC_MAX_RAYS,
+            "xdg_hip_ray_fire_noalloc: n exceeds XDG_HIP_NOALLOC_MAX_RAYS (4194304 rays); split the batch");
+    require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
+    require(orientation >= -1 && orientation <= 1, "unknown orientation");
+    require(k >= 0 && k <= XDG_HIP_MAX_EXCLUDE, "k must be in [0, 1024]");
+    int depth = tree_per_ray ? surf_need(c, nullptr) : surf_need(c, &surf_tree(c, tree));
+    if (n == 0) return XDG_HIP_OK;
+    require(org && dir && t_out && surf_out, "null array");
+    NoAllocScratch* sp = scratch_for(c->device);
+    if (!sp) throw HipError{"allocation-free scratch is not set up on this device"};
+    char* base = (char*)sp;  // device address: offsets only, never dereferenced here
+    RayFireArgs a{c->d_surf, (int32_t)c->surf.size(), tree, tree_per_ray, n, org, dir, tmax, orientation,
+                  excl, excl ? k : 0, t_out, surf_out, face_out, nullptr, nullptr};
+    if (precision == XDG_HIP_MIXED && c->cfg.defer) {
+      a.fb_list = (int32_t*)(base + offsetof(NoAllocScratch, fb_list));
+      a.fb_count = (unsigned int*)(base + offsetof(NoAllocScratch, fb_count));
+    }
+    a.next_ray = (unsigned long long*)(base + offsetof(NoAllocScratch, next));
+    // traced in the caller's order: the ray-ordering option needs buffers
+    // sized by the batch, which this entry point does not allocate
+    launch_ray_fire(a, precision == XDG_HIP_MIXED, cfg_for(c, depth), pick(c, stream));
+    XDG_HIP_CHECK(hipGetLastError());
     return XDG_HIP_OK;
   });
 }
