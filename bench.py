@@ -34,6 +34,22 @@ sys.path.insert(0, ROOT)
 METRIC = "ray_fire Mrays/s: mixed vs FP64 kernel, identical hits; tet-walk Msegs/s"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# VALU issue roof: 256 CUs x 4 SIMD16 = 1024 SIMDs at the 2.4 GHz peak
+# engine clock; a wave64 VALU instruction holds its SIMD for 4 cycles, so
+# the chip issues at most 1024 x 2.4e9 / 4 = 614.4 G wave-instructions/s
+SIMDS = 1024
+CLOCK_HZ = 2.4e9
+VALU_PEAK_GINST = SIMDS * CLOCK_HZ / 4 / 1e9
+
+
+def load_profile(name):
+    """Per-launch counters committed under profiles/ (rocprofv3 runs of this
+    bench; each summary next to the JSON gives its command)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def log(msg):
@@ -51,6 +67,9 @@ def parse():
     p.add_argument("--histories", type=int, default=1 << 20)
     p.add_argument("--adjoint-rays", type=int, default=4_000_000)
     p.add_argument("--cpu-sample", type=int, default=10_000_000)
+    p.add_argument("--cpu-tet-sample", type=int, default=500_000)
+    p.add_argument("--sphere-rays", type=int, default=1_000_000)
+    p.add_argument("--no-sphere", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-tet", action="store_true")
     p.add_argument("--no-transport", action="store_true")
@@ -183,31 +202,39 @@ def bench_ray_fire(args, ctx, torch, stream, sh):
         f"N {N:.2f} -> {bytes_ray:.0f} B/ray, {achieved:.0f} GB/s, FP64 "
         f"{fp64_tflops:.2f} TFLOP/s; BVH4 nodes/ray {V4:.2f}, SIMD efficiency "
         f"node loop {simd_node:.3f}, triangle loop {simd_tri:.3f}")
-    traffic = None
-    pmc = os.path.join(ROOT, "profiles", "ray_fire_mixed_nested_spheres_pmc.json")
-    if os.path.exists(pmc):
-        with open(pmc) as f:
-            rec = json.load(f)
-        if rec.get("queries") == n:
-            traffic = rec.get("hbm_bytes_per_launch")
+    roof = {"bound": "valu", "unit": "Ginst/s", "peak": round(VALU_PEAK_GINST, 1),
+            "achieved": None, "frac": None, "traffic": None,
+            "kernel": "k_ray_fire_pt_mixed (persistent BVH4, FP32 boxes; + deferred FP64 pass)"}
+    prof = load_profile("ray_fire_mixed_nested_spheres_pmc.json")
+    if prof and prof.get("queries") == n:
+        # issue roof: wave-level VALU instructions per launch (SQ_INSTS_VALU,
+        # committed profile of this workload) over this run's launch time
+        ginst = prof["sq_insts_valu_per_launch"] / avg_m / 1e9
+        traffic = prof["hbm_bytes_per_launch"]
+        roof.update(achieved=round(ginst, 2), frac=round(ginst / VALU_PEAK_GINST, 4),
+                    traffic=traffic, traffic_source="profiles/ray_fire_mixed_nested_spheres_pmc.json "
+                    "(rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-probe scaled)",
+                    formula="frac = SQ_INSTS_VALU x 4 / (1024 SIMDs x 2.4 GHz x launch time)",
+                    hbm={"achieved": round(traffic / avg_m / 1e9, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(traffic / avg_m / 1e9 / HBM_PEAK_GBS, 4),
+                         "note": "measured L2-to-fabric bytes per launch over this run's launch time"})
+    roof.update(effective_gbs=round(achieved, 2),
+                effective_note="algorithmic bytes per ray (48 + 16 + 128 V + 80 F, binary FP64 "
+                               "layout) x rays / launch time; node reads are served by L2 and the "
+                               "Infinity Cache, so this is not a fraction of the HBM roof",
+                bytes_per_ray=round(bytes_ray, 2),
+                work_per_ray={"V": round(V, 3), "F": round(F, 3), "E": round(E, 3),
+                              "C": round(Cf, 3), "N": round(N, 3)},
+                fp64={"achieved_tflops": round(fp64_tflops, 4), "peak_tflops": FP64_PEAK_TFLOPS,
+                      "frac": round(fp64_tflops / FP64_PEAK_TFLOPS, 4)},
+                bvh4_nodes_per_ray=round(V4, 3),
+                simd_efficiency_node_loop=round(simd_node, 4),
+                simd_efficiency_triangle_loop=round(simd_tri, 4))
     res = {
         "n": n, "avg_m": avg_m, "avg_p": avg_p, "mixed_mrays": mixed_mrays,
         "plain_mrays": plain_mrays, "identical": identical, "rel": rel,
         "fv_mqs": fv_mqs, "vol_identical": vol_identical,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     "kernel": "k_ray_fire<mixed> (+ deferred FP64 pass)",
-                     "bytes_per_ray": round(bytes_ray, 2),
-                     "work_per_ray": {"V": round(V, 3), "F": round(F, 3), "E": round(E, 3),
-                                      "C": round(Cf, 3), "N": round(N, 3)},
-                     "fp64_tflops": round(fp64_tflops, 4),
-                     "fp64_peak_tflops": FP64_PEAK_TFLOPS,
-                     "fp64_frac": round(fp64_tflops / FP64_PEAK_TFLOPS, 4),
-                     "bvh4_nodes_per_ray": round(V4, 3),
-                     "simd_efficiency_node_loop": round(simd_node, 4),
-                     "simd_efficiency_triangle_loop": round(simd_tri, 4)},
+        "roofline": roof,
         "model": model, "vols": vols, "pts": pts, "dirs": dirs,
         "d_vol": d_vol, "t_mixed": tm,
     }
@@ -248,12 +275,25 @@ def bench_tet(args, ctx, torch, stream, sh):
     status = ctx.status()
     # bytes per segment = 48 + 16 + S (16 conn + 16 adj + 96 verts + 16 tally RMW)
     seg_bytes = 64 + steps_per_seg * 144
+    alg_gbs = seg_bytes * ns_ / avg_w / 1e9
     tet = {"msegs_s": round(ns_ / avg_w / 1e6, 3), "ms_per_launch": round(avg_w * 1e3, 4),
            "segments": ns_, "tets": int(mesh.tets.shape[0]),
            "elements_per_segment": round(steps_per_seg, 3),
            "element_steps_per_s": ns_ * steps_per_seg / avg_w,
-           "algorithmic_gbs": round(seg_bytes * ns_ / avg_w / 1e9, 2),
            "capped": bool(status & 1)}
+    roof = {"bound": "hbm", "achieved": round(alg_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(alg_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_segments (tallies)",
+            "bytes_per_segment": round(seg_bytes, 2),
+            "model": "64 + S x (128 element record + 16 tally read-modify-write) bytes per segment"}
+    prof = load_profile("tet_walk_kuhn64_counters.json")
+    if prof and prof.get("segments") == ns_:
+        tr = prof["fetch_bytes_per_launch"] + prof["write_bytes_per_launch"]
+        roof.update(traffic=tr, traffic_source="profiles/tet_walk_kuhn64_counters.json "
+                    "(rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-probe scaled)",
+                    traffic_over_algorithmic=round(tr / (seg_bytes * ns_), 3))
+    tet["roofline"] = roof
+    tet["_p"] = (p0, p1, cnt, hsh, mesh)
     log(f"tet walk {tet['msegs_s']:.2f} Msegs/s, {steps_per_seg:.1f} elements/segment")
 
     # element-by-element particle walk (tools/walk_elements.h) on the same
@@ -279,6 +319,50 @@ def bench_tet(args, ctx, torch, stream, sh):
     log(f"mesh walk {tet['mesh_walk']['mparticles_s']:.2f} Mparticles/s, "
         f"{events / npart:.1f} events/particle")
     return tet
+
+
+# ---------------------------------------------------------------------------
+# single sphere: the reference's own CPU-runnable ray_fire case
+# ---------------------------------------------------------------------------
+
+def bench_sphere(args, ctx, torch, stream, sh):
+    """Icosphere r = 10, subdivision 5 (20 480 triangles); 1 M rays with
+    origins uniform in the ball r = 9 and isotropic directions (Philox seed
+    1), EXITING: ray_fire mixed and plain, as tools/ray_benchmark.cpp:205-225
+    times its trace phase."""
+    from xdg_amd import capi
+    from xdg_amd import geometry as G
+    from xdg_amd import philox as P
+    dev = torch.device("cuda", 0)
+    m = G.single_sphere(5, 10.0)
+    tree = build_trees(ctx, m, [1])[0]
+    n = args.sphere_rays
+    org = P.ball_points(1, P.STREAM_POS, n, 9.0)
+    dirs = P.iso_dirs(1, P.STREAM_DIR, n)
+    d_org = torch.from_numpy(org).to(dev)
+    d_dir = torch.from_numpy(dirs).to(dev)
+    outs = {p: (torch.empty(n, dtype=torch.float64, device=dev),
+                torch.empty(n, dtype=torch.int32, device=dev),
+                torch.empty(n, dtype=torch.int32, device=dev)) for p in (capi.MIXED, capi.PLAIN)}
+    per = {}
+    for prec in (capi.MIXED, capi.PLAIN):
+        fn = lambda: ctx.ray_fire(tree, d_org, d_dir, precision=prec, out=outs[prec], stream=sh)  # noqa: E731
+        per[prec], _ = time_launches(torch, stream, fn, args.steps, args.warmup)
+    avg = {p: sum(v) / len(v) for p, v in per.items()}
+    tm, sm, fm = outs[capi.MIXED]
+    tp, sp_, fp = outs[capi.PLAIN]
+    identical = bool(torch.equal(sm, sp_) and torch.equal(fm, fp)
+                     and torch.equal(tm.view(torch.int64), tp.view(torch.int64)))
+    res = {"workload": "single_sphere_ray_fire", "geometry": "icosphere r=10 subdiv 5 (20480 tris)",
+           "rays": n, "seed": 1, "mixed_mrays_s": round(n / avg[capi.MIXED] / 1e6, 3),
+           "plain_fp64_mrays_s": round(n / avg[capi.PLAIN] / 1e6, 3),
+           "mixed_ms": round(avg[capi.MIXED] * 1e3, 4), "plain_ms": round(avg[capi.PLAIN] * 1e3, 4),
+           "mixed_over_plain": round(avg[capi.PLAIN] / avg[capi.MIXED], 4),
+           "identical_hits": identical, "hits": int((sp_ >= 0).sum().item())}
+    log(f"single sphere: mixed {res['mixed_mrays_s']:.1f} Mrays/s, plain "
+        f"{res['plain_fp64_mrays_s']:.1f} Mrays/s, identical {identical}")
+    res["_p"] = (m, org, dirs, tm)
+    return res
 
 
 # ---------------------------------------------------------------------------
@@ -408,6 +492,73 @@ def cpu_baseline(args, hl):
     return cpu
 
 
+def _host_info():
+    ncpu = len(os.sched_getaffinity(0))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return ncpu, model
+
+
+def _median_passes(fn, passes=5):
+    fn()  # warm-up
+    times = []
+    for _ in range(passes):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), sum(times)
+
+
+def cpu_baseline_sphere(sp):
+    """The oracle's ray_fire over the same 1 M single-sphere rays."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    m, org, dirs, t_gpu = sp["_p"]
+    ncpu, cpu_model = _host_info()
+    ot = O.Tree.from_model(m, 1)
+    res = {}
+    dt, tot = _median_passes(lambda: res.update(r=ot.ray_fire(org, dirs, nthreads=ncpu)))
+    agree = bool(np.array_equal(res["r"][0].view(np.uint64), t_gpu.cpu().numpy().view(np.uint64)))
+    n = org.shape[0]
+    return {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": ncpu, "cpu": cpu_model,
+            "kind": "port",
+            "sample": f"all {n} single-sphere rays, C oracle (FP64 host BVH, OpenMP dynamic "
+                      f"schedule); warm-up then median of 5 passes ({dt:.3f} s each); "
+                      f"distances bit-identical to the GPU: {agree}"}
+
+
+def cpu_baseline_tet(args, tet):
+    """The oracle's XDG::segments walk with per-element tallies (the loop
+    tools/tally_segments.h:49-88 times) over the first segments of the same
+    batch, on every host core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    p0, p1, cnt, hsh, mesh = tet["_p"]
+    ncpu, cpu_model = _host_info()
+    m = min(args.cpu_tet_sample, p0.shape[0])
+    a = p0[:m].cpu().numpy()
+    b = p1[:m].cpu().numpy()
+    om = O.TetMesh(mesh)
+    res = {}
+    dt, tot = _median_passes(lambda: res.update(r=om.segments(a, b, tally=True, nthreads=ncpu)))
+    rc, rh, _ = res["r"]
+    agree = bool(np.array_equal(rc, cnt[:m].cpu().numpy())
+                 and np.array_equal(rh, hsh[:m].cpu().numpy().view(np.uint64)))
+    return {"value": round(m / dt / 1e6, 4), "unit": "Msegs/s", "cores": ncpu, "cpu": cpu_model,
+            "kind": "port",
+            "sample": f"first {m} of the {p0.shape[0]} segments on the 64^3 x 6 Kuhn mesh, C oracle "
+                      f"XDG::segments + tallies (host element BVH, OpenMP dynamic schedule); "
+                      f"warm-up then median of 5 passes ({dt:.2f} s each, {tot + dt:.1f} s in all); "
+                      f"element sequences (counts and hashes) identical to the GPU: {agree}"}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -439,10 +590,23 @@ def main():
         del src, dst
 
     hl = bench_ray_fire(args, ctx, torch, stream, sh)
+    sph = None if args.no_sphere else bench_sphere(args, ctx, torch, stream, sh)
     tet = None if args.no_tet else bench_tet(args, ctx, torch, stream, sh)
     tr = None if args.no_transport else bench_transport(args, ctx, torch, stream, sh)
     adj = None if args.no_adjoint else bench_adjoint(args, ctx, torch, stream, sh)
     cpu = None if args.no_cpu else cpu_baseline(args, hl)
+    if cpu is not None:
+        ncpu, cpu_model = _host_info()
+        cpu["cpu"] = cpu_model
+        if sph is not None:
+            cpu["single_sphere"] = cpu_baseline_sphere(sph)
+            log(f"cpu baseline single sphere {cpu['single_sphere']['value']:.2f} Mrays/s")
+        if tet is not None:
+            cpu["tet_walk"] = cpu_baseline_tet(args, tet)
+            log(f"cpu baseline tet walk {cpu['tet_walk']['value']:.4f} Msegs/s")
+    for leg in (sph, tet):
+        if leg is not None:
+            leg.pop("_p", None)
 
     n = hl["n"]
     line = {
@@ -469,6 +633,7 @@ def main():
         "max_rel_distance_error": hl["rel"],
         "find_volume_mqueries_s": round(hl["fv_mqs"], 3),
         "find_volume_mixed_equals_plain": hl["vol_identical"],
+        "single_sphere": sph,
         "tet_walk": tet,
         "transport": tr,
         "adjoint": adj,

@@ -20,6 +20,7 @@
 #ifndef XDG_HIP_HPP
 #define XDG_HIP_HPP
 
+#include <algorithm>
 #include <array>
 #include <cfloat>
 #include <cstdint>
@@ -242,6 +243,72 @@ class HipRayTracer {
     int32_t r = ipc;
     s.get(&r, v, 4);
     return r;
+  }
+
+  // Boundary faces of an element tree for the implicit-complement re-entry
+  // of segments (src/xdg.cpp:164-185): the complement's surface tree and,
+  // indexed by its face ids, the element index behind each face (-1 none).
+  void attach_boundary(TreeID element_tree, TreeID surface_tree, const std::vector<int32_t>& bface_elem)
+  {
+    tree_check(xdg_hip_attach_boundary(ctx_, element_tree, surface_tree, (int64_t)bface_elem.size(),
+                                       bface_elem.data()),
+               element_tree);
+  }
+
+  // MeshManager::next_element (mesh_manager_interface.cpp:150-200) on
+  // element indices of the tree: {next element or ID_NONE, exit distance}.
+  std::pair<MeshID, double> next_element(TreeID element_tree, MeshID element, const Position& r,
+                                         const Direction& u) const
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    Stage s(*this);
+    int32_t* e = (int32_t*)s.take(8);
+    check(xdg_hip_memcpy_htod(ctx_, e, &element, 4));
+    double* rp = s.put_d3(r);
+    double* up = s.put_d3(u);
+    int32_t* nx = (int32_t*)s.take(8);
+    double* d = (double*)s.take(8);
+    tree_check(xdg_hip_next_element(ctx_, element_tree, 1, e, rp, up, nx, d, nullptr), element_tree);
+    int32_t nh = ID_NONE;
+    double dh = INFTY;
+    s.get(&nh, nx, 4);
+    s.get(&dh, d, 8);
+    return {nh, dh};
+  }
+
+  // XDG::segments(start, end) (src/xdg.cpp:142-198) over an element tree:
+  // the (element index, length) pieces in walk order.  At most `cap` pieces
+  // are returned (the walk's own count is in *count when given).
+  std::vector<std::pair<MeshID, double>> segments(TreeID element_tree, const Position& start, const Position& end,
+                                                  int cap = 2048, int32_t* count = nullptr) const
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    Stage s(*this);
+    double* p0 = s.put_d3(start);
+    double* p1 = s.put_d3(end);
+    int32_t* cnt = (int32_t*)s.take(8);
+    uint64_t* hsh = (uint64_t*)s.take(8);
+    const int64_t off[2] = {0, cap};
+    int64_t* offd = (int64_t*)s.take(sizeof off);
+    check(xdg_hip_memcpy_htod(ctx_, offd, off, sizeof off));
+    int32_t* se = (int32_t*)s.take((size_t)cap * 4);
+    double* sl = (double*)s.take((size_t)cap * 8);
+    tree_check(xdg_hip_tally_segments(ctx_, element_tree, 1, p0, p1, nullptr, cnt, hsh, offd, se, sl, nullptr),
+               element_tree);
+    int32_t n = 0;
+    s.get(&n, cnt, 4);
+    if (count) *count = n;
+    const int m = std::min<int>(n, cap);
+    std::vector<int32_t> eh((size_t)m);
+    std::vector<double> lh((size_t)m);
+    if (m > 0) {
+      s.get(eh.data(), se, (size_t)m * 4);
+      s.get(lh.data(), sl, (size_t)m * 8);
+    }
+    std::vector<std::pair<MeshID, double>> out;
+    out.reserve((size_t)m);
+    for (int i = 0; i < m; ++i) out.emplace_back(eh[(size_t)i], lh[(size_t)i]);
+    return out;
   }
 
   // ---- batched queries on device arrays (the performance path) ----------
