@@ -122,6 +122,8 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *   "mw_wave"     mesh walk: particles that leave the mesh are parked and
  *                 their re-entry rays traced together in a separate kernel
  *                 (1), or each lane traces its own at once (0) (default 1)
+ *   "refit"       surface trees built while set keep their topology, so
+ *                 xdg_hip_refit_surface_tree can move their vertices (0)
  * Unknown names give XDG_HIP_ERR_NOT_FOUND, out-of-range values
  * XDG_HIP_ERR_INVALID. */
 int xdg_hip_set_option(xdg_hip_ctx* ctx, const char* name, int value);
@@ -157,6 +159,17 @@ int xdg_hip_build_element_tree(xdg_hip_ctx* ctx, int64_t n_verts, const double* 
  * indexed by the face ids of the surface tree. */
 int xdg_hip_attach_boundary(xdg_hip_ctx* ctx, int32_t element_tree, int32_t surface_tree, int64_t n_faces,
                             const int32_t* bface_elem);
+
+/* New coordinates for the vertices of a surface tree built while the
+ * "refit" option was set (same n_verts, triangles and ids as the build;
+ * verts is a DEVICE array of 3 n_verts doubles): triangle records, normals,
+ * box dilation (bump >= 0 replaces it, bump < 0 keeps it), both node layouts
+ * and the mixed-precision limit are recomputed on the device with the
+ * topology kept.  Queries stay exact for any vertex motion (boxes are
+ * recomputed, only their tightness changes).  Synchronous.  No counterpart
+ * in the reference (it rebuilds); for callers that move vertices with fixed
+ * connectivity, such as the differentiable-surface harness. */
+int xdg_hip_refit_surface_tree(xdg_hip_ctx* ctx, int32_t tree, int64_t n_verts, const double* verts, double bump);
 
 int xdg_hip_get_tree_info(xdg_hip_ctx* ctx, int32_t tree, int element, xdg_hip_tree_info* info);
 int xdg_hip_num_trees(xdg_hip_ctx* ctx, int32_t* surface_trees, int32_t* element_trees);

@@ -52,6 +52,20 @@ int main()
     }
   CHECK(adj_ok);
   CHECK(ea.tets[0] == 0 && ea.tets[3] == 8);  // tet 0 = mock triangle 0 + centre
+  // tets 8..11 are the y = -3 pair, then the y = 6 pair (the mock's order)
+  CHECK(ea.tets[4 * 8] == 0 && ea.tets[4 * 8 + 1] == 7 && ea.tets[4 * 10] == 1 && ea.tets[4 * 10 + 1] == 5);
+  // every neighbour pair shares a face: three common vertices
+  bool shared = true;
+  for (int e = 0; e < 12; ++e)
+    for (int f = 0; f < 4; ++f) {
+      const int n = ea.adj[4 * e + f];
+      if (n < 0) continue;
+      int common = 0;
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) common += ea.tets[4 * e + i] == ea.tets[4 * n + j];
+      shared &= common == 3;
+    }
+  CHECK(shared);
   // two volumes: duplicated vertex blocks, connectivity offset by the first
   BoxManager split(true);
   const auto ga = flatten_elements(split, {BoxManager::VOL, BoxManager::VOL_B});
@@ -80,6 +94,7 @@ int main()
   for (int f = 1000; f < 1012; ++f) faces.push_back(f);
   const auto bm = boundary_face_elements(box, faces, ea);
   CHECK(bm.size() == 1012 && bm[0] == -1 && bm[1000] == 0 && bm[1007] == 7);
+  CHECK(bm[1008] == 10 && bm[1009] == 11 && bm[1010] == 8 && bm[1011] == 9);
   const auto bh = boundary_face_elements(split, faces, ha);
   CHECK(bh[1005] == 5 && bh[1006] == -1);  // face 1006's owner is in the other volume
 

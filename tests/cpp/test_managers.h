@@ -40,8 +40,8 @@ inline bool contains(const std::array<P3, 4>& t, const P3& p, double tol)
 
 // ---------------------------------------------------------------------------
 // The mock box: [-2,5] x [-3,6] x [-4,7], 8 corners + centre; 12 faces (two
-// per surface); tet i = face i + centre, so its face 0 (mock order) is the
-// boundary face i.
+// per surface); 12 tets, each a boundary face and the centre, so face 0 of
+// a tet (mock order) is its boundary face.
 // ---------------------------------------------------------------------------
 class BoxManager {
  public:
@@ -52,6 +52,10 @@ class BoxManager {
           {x1, y1, z0}, {x0, y1, z0}, {x0, y0, z0}, {(x0 + x1) / 2, (y0 + y1) / 2, (z0 + z1) / 2}};
     t_ = {{0, 1, 2}, {0, 2, 3}, {4, 6, 5}, {4, 7, 6}, {0, 5, 1}, {0, 4, 5},
           {2, 6, 7}, {2, 7, 3}, {1, 5, 6}, {1, 6, 2}, {0, 7, 4}, {0, 3, 7}};
+    // the mock's tets: a boundary triangle and the centre, listed in the
+    // mock's own element order (the y = -3 pair before the y = 6 pair)
+    k_ = {{0, 1, 2, 8}, {0, 2, 3, 8}, {4, 6, 5, 8}, {4, 7, 6, 8}, {0, 5, 1, 8}, {0, 4, 5, 8},
+          {2, 6, 7, 8}, {2, 7, 3, 8}, {0, 7, 4, 8}, {0, 3, 7, 8}, {1, 5, 6, 8}, {1, 6, 2, 8}};
     const int a[12][4] = {{-1, 1, 4, 11}, {-1, 9, 0, 7}, {-1, 5, 3, 10}, {-1, 2, 8, 6},
                           {-1, 0, 5, 10}, {-1, 4, 8, 2}, {-1, 7, 11, 3}, {-1, 1, 6, 9},
                           {-1, 5, 9, 3}, {-1, 8, 1, 7}, {-1, 11, 4, 2}, {-1, 0, 10, 6}};
@@ -93,10 +97,8 @@ class BoxManager {
     std::map<int, int> loc;
     for (size_t i = 0; i < lv.size(); ++i) loc[lv[i]] = (int)i;
     std::vector<int> c;
-    for (MeshID e : get_volume_elements(vol)) {
-      const auto& t = t_[e - ELEM0];
-      for (int k : {t[0], t[1], t[2], 8}) c.push_back(loc[k]);
-    }
+    for (MeshID e : get_volume_elements(vol))
+      for (int k : k_[e - ELEM0]) c.push_back(loc[k]);
     return c;
   }
   MeshID adjacent_element(MeshID e, int f) const
@@ -104,26 +106,32 @@ class BoxManager {
     const int n = adj_[e - ELEM0][f];
     return n < 0 ? ID_NONE : ELEM0 + n;
   }
-  MeshID get_boundary_face_element(MeshID face) const { return ELEM0 + (face - FACE0); }
+  // the tet whose first three vertices are the face's
+  MeshID get_boundary_face_element(MeshID face) const
+  {
+    const auto& t = t_[face - FACE0];
+    for (int e = 0; e < 12; ++e)
+      if (k_[e][0] == t[0] && k_[e][1] == t[1] && k_[e][2] == t[2]) return ELEM0 + e;
+    return ID_NONE;
+  }
   std::array<P3, 4> element_vertices(MeshID e) const
   {
-    const auto& t = t_[e - ELEM0];
-    return {v_[t[0]], v_[t[1]], v_[t[2]], v_[8]};
+    const auto& t = k_[e - ELEM0];
+    return {v_[t[0]], v_[t[1]], v_[t[2]], v_[t[3]]};
   }
 
  private:
   std::vector<int> local_verts(MeshID vol) const
   {
     std::set<int> s;
-    for (MeshID e : get_volume_elements(vol)) {
-      for (int k : t_[e - ELEM0]) s.insert(k);
-      s.insert(8);
-    }
+    for (MeshID e : get_volume_elements(vol))
+      for (int k : k_[e - ELEM0]) s.insert(k);
     return std::vector<int>(s.begin(), s.end());
   }
   bool split_;
   std::vector<P3> v_;
   std::vector<std::array<int, 3>> t_;
+  std::vector<std::array<int, 4>> k_;
   int adj_[12][4];
 };
 

@@ -114,7 +114,7 @@ int main()
   for (MeshID s : all)
     for (MeshID f : box.get_surface_faces(s)) ipc_faces.push_back(f);
   const auto bmap = boundary_face_elements(box, ipc_faces, ea);
-  CHECK(bmap.size() == 1012 && bmap[1000] == 0 && bmap[1011] == 11 && bmap[5] == -1);
+  CHECK(bmap.size() == 1012 && bmap[1000] == 0 && bmap[1011] == 9 && bmap[5] == -1);
   rt.attach_boundary(et, it, bmap);
   // find_element: the lowest containing element index, by brute force
   const P3 probes[] = {{0.1, 0.2, 0.3}, {4.5, 1.0, 1.5}, {-1.5, -2.0, 6.0}, {1.5, 5.5, -3.5}, {10, 10, 10}};

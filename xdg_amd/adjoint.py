@@ -5,9 +5,11 @@ ray_fire (forward) and the hand-written adjoint kernel (backward).
     t = ray_distance(org, dirs, verts, surf)      # verts: (nv, 3) float64
     (w * t).sum().backward()                      # -> verts.grad, org.grad ...
 
-The surface tree is rebuilt whenever the vertex values differ from those
-of the last build (compared on the device), so an optimiser stepping
-`verts` sees fresh geometry.  Rays that miss give t = DBL_MAX and zero gradients.
+The surface tree is built once (with the library's "refit" option) and
+refitted on the device whenever the vertex values differ from those of the
+last refit (compared on the device), so an optimiser stepping `verts` sees
+fresh geometry without a rebuild; the connectivity is fixed.  Rays that miss
+give t = DBL_MAX and zero gradients.
 No counterpart in the reference (its queries are not differentiable).
 """
 from __future__ import annotations
@@ -40,19 +42,32 @@ class DifferentiableSurface:
         self._key = None
         self.tree = None
 
+    def _bump(self, vd):
+        if self.bump is not None:
+            return self.bump
+        lo = torch.clamp(vd.min(dim=0).values, max=0.0).cpu().numpy()
+        hi = torch.clamp(vd.max(dim=0).values, min=0.0).cpu().numpy()
+        return G.box_bump(lo, hi)
+
     def tree_for(self, verts: torch.Tensor) -> int:
-        vd = verts.detach()
-        same = (self.tree is not None and self._key is not None
-                and self._key.shape == vd.shape and torch.equal(self._key, vd))
-        if not same:
-            v = vd.cpu().numpy().astype(np.float64)
-            bump = self.bump if self.bump is not None else G.vertex_bump(v)
-            if self.tree is not None:
-                self.ctx.release_surface_tree(self.tree)
-                self.tree = None
+        vd = verts.detach().contiguous()
+        if self.tree is not None and self._key is not None and self._key.shape == vd.shape:
+            if not torch.equal(self._key, vd):
+                self.ctx.refit_surface_tree(self.tree, vd, self._bump(vd))
+                self._key = vd.clone()
+            return self.tree
+        if self.tree is not None:
+            self.ctx.release_surface_tree(self.tree)
+            self.tree = None
+        v = vd.cpu().numpy().astype(np.float64)
+        keep = self.ctx.get_option("refit")
+        self.ctx.set_option("refit", 1)
+        try:
             self.tree = self.ctx.build_surface_tree(
-                v, self.tris_host, self.face_ids, self.surf_ids, self.flip, bump)
-            self._key = vd.clone()
+                v, self.tris_host, self.face_ids, self.surf_ids, self.flip, self._bump(vd))
+        finally:
+            self.ctx.set_option("refit", keep)
+        self._key = vd.clone()
         return self.tree
 
 

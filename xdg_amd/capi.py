@@ -98,6 +98,7 @@ def lib():
             "xdg_hip_get_tree_info": [_vp, _i32, C.c_int, C.POINTER(TreeInfo)],
             "xdg_hip_num_trees": [_vp, C.POINTER(_i32), C.POINTER(_i32)],
             "xdg_hip_release_surface_tree": [_vp, _i32],
+            "xdg_hip_refit_surface_tree": [_vp, _i32, _i64, _vp, C.c_double],
             "xdg_hip_ray_fire": [_vp, _i32, _vp, _i64, _vp, _vp, _vp, C.c_int,
                                  _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp],
             "xdg_hip_ray_fire_noalloc": [_vp, _i32, _vp, _i64, _vp, _vp, _vp,
@@ -317,6 +318,14 @@ class Context:
 
     def release_surface_tree(self, tree):
         self._check(lib().xdg_hip_release_surface_tree(self.h, tree))
+
+    def refit_surface_tree(self, tree, verts, bump=-1.0):
+        """New vertex coordinates (device float64 (nv, 3), the build's nv)
+        for a tree built with the "refit" option; bump < 0 keeps the old."""
+        nv = _count(verts)
+        self._arg(verts, "verts", F64, 3 * nv)
+        torch.cuda.current_stream(self.dev).synchronize()
+        self._check(lib().xdg_hip_refit_surface_tree(self.h, tree, nv, _ptr(verts), float(bump)))
 
     def build_element_tree(self, verts, tets, adj, face_order):
         v, vp = _host(verts, np.float64)

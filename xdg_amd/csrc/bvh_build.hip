@@ -8,13 +8,16 @@
 //   * a BVH2 with both child boxes inline per node, FP64 (plain kernels) and
 //     outward-rounded FP32 (mixed kernels), identical topology;
 //   * the leaf-ordered primitive index array.
-// Pipeline: centroid bounds (wave reduction + ordered-int atomics) ->
-// 30-bit Morton codes -> rocPRIM radix sort of (code, index) pairs (stable,
-// hence deterministic) -> Karras (HPG 2012) radix-tree hierarchy over the
+// Pipeline, all on the device: centroid bounds (wave reduction +
+// ordered-int atomics) -> 63-bit Morton codes -> stable LSD radix sort of
+// (code, index) pairs (4-bit digits, hand-written below; stable, hence
+// deterministic) -> Karras (HPG 2012) radix-tree hierarchy over the
 // (code, index) keys -> bottom-up refit with atomic arrival flags -> node
-// emission, collapsing subtrees of <= leaf_max primitives into leaves.
+// emission, collapsing subtrees of <= leaf_max primitives into leaves ->
+// structural check by breadth-first levels.  The BVH4 the traversal kernels
+// walk is collapsed from that tree level by level on the device as well.
+// Only counts and the max |coordinate| travel to the host.
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <climits>
@@ -90,17 +93,22 @@ __global__ void k_centroid_bounds(const double* __restrict__ box6, int64_t n, un
   }
 }
 
-__device__ __forceinline__ uint32_t expand10(uint32_t v)
+
+// spread the low 21 bits of v to every third bit
+__device__ __forceinline__ uint64_t expand21(uint64_t v)
 {
-  v = (v * 0x00010001u) & 0xFF0000FFu;
-  v = (v * 0x00000101u) & 0x0F00F00Fu;
-  v = (v * 0x00000011u) & 0xC30C30C3u;
-  v = (v * 0x00000005u) & 0x49249249u;
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
   return v;
 }
 
-__global__ void k_morton(const double* __restrict__ box6, int64_t n, const unsigned long long* cb,
-                         uint32_t* keys, int32_t* vals)
+// 63-bit Morton code of each box centroid over the centroid bounds
+__global__ void k_morton(const double* __restrict__ box6, int64_t n, const unsigned long long* cb, uint64_t* keys,
+                         int32_t* vals)
 {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -110,28 +118,185 @@ __global__ void k_morton(const double* __restrict__ box6, int64_t n, const unsig
     ext[k] = ord_dec(cb[3 + k]) - lo[k];
   }
   const double* b = box6 + 6 * i;
-  uint32_t q[3];
+  uint64_t q[3];
   for (int k = 0; k < 3; ++k) {
     double c = 0.5 * (b[k] + b[3 + k]);
     double f = ext[k] > 0.0 ? (c - lo[k]) / ext[k] : 0.0;
-    double s = fmin(fmax(f * 1024.0, 0.0), 1023.0);
-    q[k] = (uint32_t)s;
+    double s = fmin(fmax(f * 2097152.0, 0.0), 2097151.0);
+    q[k] = (uint64_t)s;
   }
-  keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+  keys[i] = (expand21(q[0]) << 2) | (expand21(q[1]) << 1) | expand21(q[2]);
   vals[i] = (int32_t)i;
 }
 
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (uint64 key, int32 value) pairs, 4-bit digits.
+// Per pass: per-tile digit counts, one exclusive scan over (digit, tile),
+// then a stable scatter: a tile is staged in LDS, each thread ranks its 8
+// consecutive items against the counts of all lower threads (16-bit fields
+// packed 4 to a word, scanned across the workgroup with wave shuffles).
+// ---------------------------------------------------------------------------
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_BINS = 16;
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                       uint32_t* hist, int64_t ntiles)
+{
+  __shared__ uint32_t c[RS_BINS];
+  if (threadIdx.x < RS_BINS) c[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = blockIdx.x * (int64_t)RS_TILE;
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    const int64_t i = base + k * RS_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&c[(keys[i] >> shift) & 15u], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < RS_BINS) hist[threadIdx.x * ntiles + blockIdx.x] = c[threadIdx.x];
+}
+
+// exclusive scan of a[0..m) in place (one workgroup of 1024), total in *total
+__global__ void __launch_bounds__(1024) k_scan_excl(uint32_t* a, int64_t m, uint32_t* total)
+{
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int64_t base = 0; base < m; base += 1024 * 4) {
+    uint32_t v[4], sum = 0;
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = base + threadIdx.x * 4 + k;
+      v[k] = i < m ? a[i] : 0u;
+      sum += v[k];
+    }
+    uint32_t x = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (wv == 0) {
+      uint32_t w = lane < 16 ? wsum[lane] : 0u;
+      for (int off = 1; off < 16; off <<= 1) {
+        const uint32_t y = __shfl_up(w, off);
+        if (lane >= off) w += y;
+      }
+      if (lane < 16) wsum[lane] = w;
+    }
+    __syncthreads();
+    uint32_t e = carry + (wv ? wsum[wv - 1] : 0u) + x - sum;
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = base + threadIdx.x * 4 + k;
+      if (i < m) a[i] = e;
+      e += v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry += wsum[15];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__device__ __forceinline__ uint32_t field16(const uint64_t* w, uint32_t d)
+{
+  return (uint32_t)((w[d >> 2] >> (16 * (d & 3))) & 0xffffull);
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t* __restrict__ kin,
+                                                          const int32_t* __restrict__ vin, uint64_t* kout,
+                                                          int32_t* vout, int64_t n, int shift,
+                                                          const uint32_t* __restrict__ off, int64_t ntiles)
+{
+  __shared__ uint64_t sk[RS_TILE];
+  __shared__ int32_t sv[RS_TILE];
+  __shared__ uint64_t wtot[RS_THREADS / 64][4];
+  const int64_t base = blockIdx.x * (int64_t)RS_TILE;
+  const int64_t cnt = n - base < RS_TILE ? n - base : RS_TILE;
+  for (int k = 0; k < RS_ITEMS; ++k) {  // coalesced loads, staged in LDS
+    const int j = k * RS_THREADS + threadIdx.x;
+    if (j < cnt) {
+      sk[j] = kin[base + j];
+      sv[j] = vin[base + j];
+    }
+  }
+  __syncthreads();
+  // this thread's items: tile positions t*8 .. t*8+7, in order
+  const int first = threadIdx.x * RS_ITEMS;
+  uint64_t c[4] = {0, 0, 0, 0};
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    const int j = first + k;
+    if (j < cnt) {
+      const uint32_t d = (uint32_t)((sk[j] >> shift) & 15u);
+      c[d >> 2] += 1ull << (16 * (d & 3));
+    }
+  }
+  // exclusive scan of the packed counts over the workgroup's threads
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint64_t x[4] = {c[0], c[1], c[2], c[3]};
+  for (int off2 = 1; off2 < 64; off2 <<= 1)
+    for (int w = 0; w < 4; ++w) {
+      const uint64_t y = __shfl_up(x[w], off2);
+      if (lane >= off2) x[w] += y;
+    }
+  if (lane == 63)
+    for (int w = 0; w < 4; ++w) wtot[wv][w] = x[w];
+  __syncthreads();
+  uint64_t ex[4];
+  for (int w = 0; w < 4; ++w) {
+    uint64_t before = 0;
+    for (int q = 0; q < wv; ++q) before += wtot[q][w];
+    ex[w] = before + x[w] - c[w];
+  }
+  uint64_t run[4] = {0, 0, 0, 0};
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    const int j = first + k;
+    if (j >= cnt) break;
+    const uint64_t key = sk[j];
+    const uint32_t d = (uint32_t)((key >> shift) & 15u);
+    const int64_t pos = (int64_t)off[d * ntiles + blockIdx.x] + field16(ex, d) + field16(run, d);
+    run[d >> 2] += 1ull << (16 * (d & 3));
+    kout[pos] = key;
+    vout[pos] = sv[j];
+  }
+}
+
+// (keys, vals) sorted in place by the low `bits` bits of the keys; k2/v2
+// are scratch of the same size
+void radix_sort_pairs(uint64_t* keys, int32_t* vals, uint64_t* k2, int32_t* v2, int64_t n, int bits, hipStream_t s,
+                      std::vector<DevBuf>& tmp)
+{
+  if (n <= 1) return;
+  const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+  auto* hist = (uint32_t*)dev_alloc(sizeof(uint32_t) * RS_BINS * ntiles, tmp);
+  int passes = (bits + 3) / 4;
+  if (passes & 1) ++passes;  // even: the result lands back in keys/vals
+  uint64_t *ka = keys, *kb = k2;
+  int32_t *va = vals, *vb = v2;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 4 * p;
+    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, n, shift, hist, ntiles);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, hist, (int64_t)RS_BINS * ntiles, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n, shift, hist,
+                       ntiles);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+}
+
 // Karras 2012 prefix length over the augmented keys (code, position)
-__device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int64_t n, int64_t i, int64_t j)
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int64_t n, int64_t i, int64_t j)
 {
   if (j < 0 || j >= n) return -1;
-  uint32_t a = k[i], b = k[j];
-  if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
-  return __clz((int)(a ^ b));
+  const uint64_t a = k[i], b = k[j];
+  if (a == b) return 64 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+  return __clzll((long long)(a ^ b));
 }
 
 // children: internal node index, or ~leaf_position
-__global__ void k_karras(const uint32_t* __restrict__ keys, int64_t n, int32_t* left, int32_t* right,
+__global__ void k_karras(const uint64_t* __restrict__ keys, int64_t n, int32_t* left, int32_t* right,
                          int32_t* rfirst, int32_t* rlast, int32_t* parent_int, int32_t* parent_leaf)
 {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -274,23 +439,6 @@ __global__ void k_emit(const double* __restrict__ box6, const int32_t* __restric
   nf[i] = f;
 }
 
-// depth of every emitted node = length of its parent chain
-__global__ void k_depth(int64_t n, int leaf_max, const int32_t* __restrict__ rfirst,
-                        const int32_t* __restrict__ rlast, const int32_t* __restrict__ parent_int, int* depth)
-{
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
-  int64_t size = (int64_t)rlast[i] - rfirst[i] + 1;
-  if (i != 0 && size <= leaf_max) return;
-  int dd = 1;
-  int32_t p = parent_int[i];
-  while (p >= 0 && p < n - 1 && dd < 256) {
-    ++dd;
-    p = parent_int[p];
-  }
-  atomicMax(depth, dd);
-}
-
 __global__ void k_maxabs(const double* __restrict__ box6, int64_t n, unsigned long long* out)
 {
   double m = 0.0;
@@ -312,58 +460,122 @@ inline unsigned grid_for(int64_t n, int blk = BLK)
   return (unsigned)std::max<int64_t>(1, g);
 }
 
-// Host-side structural check of a freshly built tree, run before any
-// traversal kernel can read it: every internal index in range and reached
-// once, every leaf range inside [0, n), every primitive covered exactly once,
-// the leaf order a permutation, depth within the 64-entry traversal stack.
-void validate_tree(const LbvhOut& out, int64_t n)
+
+// ---------------------------------------------------------------------------
+// Structural check of the emitted binary tree, breadth-first from the root:
+// every child index in range, every primitive covered exactly once by a
+// reachable leaf, depth (root = 1) within the traversal stack, the leaf
+// order a permutation.  err bits: 1 index out of range, 2 too deep.
+// ---------------------------------------------------------------------------
+struct BItem {
+  int32_t node;
+  int32_t depth;
+};
+
+__global__ void k_b2_level(const NodeD* __restrict__ nd, int64_t n_nodes, int64_t n, const BItem* __restrict__ in,
+                           unsigned int nin, BItem* out, unsigned int* nout, unsigned int* seen, int* depth,
+                           unsigned int* err)  // out holds n_nodes items
 {
-  std::vector<int32_t> order((size_t)n);
-  XDG_HIP_CHECK(hipMemcpy(order.data(), out.order, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-  std::vector<uint8_t> seen((size_t)n, 0);
-  for (int64_t i = 0; i < n; ++i) {
-    if (order[i] < 0 || order[i] >= n || seen[order[i]]) throw HipError{"lbvh: leaf order is not a permutation"};
-    seen[order[i]] = 1;
+  const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nin) return;
+  const BItem it = in[i];
+  if (it.node < 0 || it.node >= n_nodes) {
+    atomicOr(err, 1u);
+    return;
   }
-  std::fill(seen.begin(), seen.end(), 0);
-  std::vector<NodeF> nodes((size_t)std::max<int64_t>(out.n_nodes, 0));
-  if (out.n_nodes > 0)
-    XDG_HIP_CHECK(hipMemcpy(nodes.data(), out.nodes_f, sizeof(NodeF) * out.n_nodes, hipMemcpyDeviceToHost));
-  std::vector<uint8_t> visited(nodes.size(), 0);
-  std::vector<std::pair<int32_t, int>> stack{{out.root, 1}};
-  int64_t covered = 0;
-  int maxd = 0;
-  while (!stack.empty()) {
-    auto [c, d] = stack.back();
-    stack.pop_back();
-    if (c < 0) {
-      int64_t f = leaf_first(c), k = leaf_count(c);
-      if (f < 0 || f + k > n) throw HipError{"lbvh: leaf range out of bounds"};
-      for (int64_t j = f; j < f + k; ++j) {
-        if (seen[j]) throw HipError{"lbvh: primitive covered twice"};
-        seen[j] = 1;
+  atomicMax(depth, it.depth);
+  if (it.depth > 64) {
+    atomicOr(err, 2u);
+    return;
+  }
+  const int32_t ch[2] = {nd[it.node].c0, nd[it.node].c1};
+  for (int c = 0; c < 2; ++c) {
+    const int32_t x = ch[c];
+    if (x >= 0) {
+      const unsigned int slot = atomicAdd(nout, 1u);
+      if ((int64_t)slot < n_nodes)
+        out[slot] = BItem{x, it.depth + 1};
+      else
+        atomicOr(err, 1u);  // more items than internal nodes: a node is reached twice
+    } else {
+      const int64_t f = leaf_first(x), k = leaf_count(x);
+      if (f < 0 || f + k > n) {
+        atomicOr(err, 1u);
+        continue;
       }
-      covered += k;
-      continue;
+      for (int64_t p = f; p < f + k; ++p) atomicAdd(seen + p, 1u);
     }
-    if (c >= out.n_nodes || visited[c]) throw HipError{"lbvh: bad or repeated internal node"};
-    visited[c] = 1;
-    maxd = std::max(maxd, d);
-    if (d > 64) throw HipError{"lbvh: tree deeper than 64"};
-    stack.push_back({nodes[c].c0, d + 1});
-    stack.push_back({nodes[c].c1, d + 1});
   }
-  if (covered != n) throw HipError{"lbvh: not every primitive is reachable"};
-  if (out.root >= 0 && maxd != out.depth) throw HipError{"lbvh: depth mismatch"};
 }
+
+// every entry of seen must be 1; order must be a permutation of [0, n)
+__global__ void k_cover_check(const unsigned int* __restrict__ seen, const int32_t* __restrict__ order,
+                              unsigned int* perm, int64_t n, unsigned int* err)
+{
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (seen[i] != 1u) atomicOr(err, 4u);
+  if (order) {
+    const int32_t o = order[i];
+    if (o < 0 || o >= n) atomicOr(err, 8u);
+    else if (atomicAdd(perm + o, 1u) != 0u) atomicOr(err, 8u);
+  }
+}
+
+// returns the depth (root = 1)
+int validate_tree(const LbvhOut& out, int64_t n, hipStream_t s)
+{
+  if (out.root < 0 || out.n_nodes == 0) return 0;  // a single leaf
+  std::vector<DevBuf> tmp;
+  try {
+    auto* lv0 = (BItem*)dev_alloc(sizeof(BItem) * (size_t)out.n_nodes, tmp);
+    auto* lv1 = (BItem*)dev_alloc(sizeof(BItem) * (size_t)out.n_nodes, tmp);
+    auto* seen = (unsigned int*)dev_alloc(sizeof(unsigned int) * (size_t)n, tmp);
+    auto* perm = (unsigned int*)dev_alloc(sizeof(unsigned int) * (size_t)n, tmp);
+    auto* misc = (unsigned int*)dev_alloc(sizeof(unsigned int) * 4, tmp);  // nout, err, depth
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(BLK), 0, s, seen, n, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(BLK), 0, s, perm, n, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(BLK), 0, s, misc, 4, 0u);
+    const BItem root{out.root, 1};
+    XDG_HIP_CHECK(hipMemcpyAsync(lv0, &root, sizeof root, hipMemcpyHostToDevice, s));
+    unsigned int nin = 1;
+    for (int level = 0; nin > 0; ++level) {
+      if (level > 64) throw HipError{"lbvh: tree deeper than 64"};
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, 1, 0u);
+      hipLaunchKernelGGL(k_b2_level, dim3(grid_for(nin)), dim3(BLK), 0, s, out.nodes_d, out.n_nodes, n, lv0, nin,
+                         lv1, misc, seen, (int*)(misc + 2), misc + 1);
+      unsigned int h2[2] = {0, 0};
+      XDG_HIP_CHECK(hipMemcpyAsync(h2, misc, sizeof h2, hipMemcpyDeviceToHost, s));
+      XDG_HIP_CHECK(hipStreamSynchronize(s));
+      nin = h2[0];
+      if (h2[1] != 0u || nin > (unsigned int)out.n_nodes) throw HipError{"lbvh: malformed tree (bad child index, node reached twice or too deep)"};
+      std::swap(lv0, lv1);
+    }
+    hipLaunchKernelGGL(k_cover_check, dim3(grid_for(n)), dim3(BLK), 0, s, seen, out.order, perm, n, misc + 1);
+    unsigned int h[3] = {0, 0, 0};
+    XDG_HIP_CHECK(hipMemcpyAsync(h, misc, sizeof h, hipMemcpyDeviceToHost, s));
+    XDG_HIP_CHECK(hipStreamSynchronize(s));
+    if (h[1] & 1u) throw HipError{"lbvh: child index out of range"};
+    if (h[1] & 2u) throw HipError{"lbvh: tree deeper than 64"};
+    if (h[1] & 4u) throw HipError{"lbvh: a primitive is not covered exactly once"};
+    if (h[1] & 8u) throw HipError{"lbvh: leaf order is not a permutation"};
+    dev_free_all(tmp);
+    return (int)h[2];
+  } catch (...) {
+    dev_free_all(tmp);
+    throw;
+  }
+}
+
 
 }  // namespace
 
 void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std::vector<DevBuf>& owned,
-                LbvhOut& out)
+                LbvhOut& out, bool keep_topology)
 {
   if (leaf_max < 1) leaf_max = 1;
   if (leaf_max > 8) leaf_max = 8;
+  out.leaf_max = leaf_max;
   std::vector<DevBuf> tmp;
   try {
     out.order = (int32_t*)dev_alloc(sizeof(int32_t) * std::max<int64_t>(n, 1), owned);
@@ -378,21 +590,18 @@ void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std:
     }
     // max |coordinate| (mixed-traversal safety limit)
     auto* d_max = (unsigned long long*)dev_alloc(sizeof(unsigned long long), tmp);
-    XDG_HIP_CHECK(hipMemsetAsync(d_max, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (unsigned int*)d_max, 2, 0u);
     hipLaunchKernelGGL(k_maxabs, dim3(std::min<unsigned>(grid_for(6 * n), 4096)), dim3(BLK), 0, s, box6, n, d_max);
 
     auto* cb = (unsigned long long*)dev_alloc(6 * sizeof(unsigned long long), tmp);
     hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, cb);
     hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min<unsigned>(grid_for(n), 4096)), dim3(BLK), 0, s, box6, n,
                        cb);
-    auto* keys = (uint32_t*)dev_alloc(sizeof(uint32_t) * n, tmp);
-    auto* vals = (int32_t*)dev_alloc(sizeof(int32_t) * n, tmp);
-    auto* keys2 = (uint32_t*)dev_alloc(sizeof(uint32_t) * n, tmp);
-    hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(BLK), 0, s, box6, n, cb, keys, vals);
-    size_t tbytes = 0;
-    XDG_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tbytes, keys, keys2, vals, out.order, (unsigned)n, 0, 30, s));
-    void* tstore = dev_alloc(tbytes, tmp);
-    XDG_HIP_CHECK(rocprim::radix_sort_pairs(tstore, tbytes, keys, keys2, vals, out.order, (unsigned)n, 0, 30, s));
+    auto* keys = (uint64_t*)dev_alloc(sizeof(uint64_t) * n, tmp);
+    auto* keys2 = (uint64_t*)dev_alloc(sizeof(uint64_t) * n, tmp);
+    auto* vals2 = (int32_t*)dev_alloc(sizeof(int32_t) * n, tmp);
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(BLK), 0, s, box6, n, cb, keys, out.order);
+    radix_sort_pairs(keys, out.order, keys2, vals2, n, 63, s, tmp);
 
     unsigned long long hmax = 0;
     if (n == 1) {
@@ -400,45 +609,37 @@ void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std:
       out.depth = 0;
     } else {
       int64_t ni = n - 1;
-      auto* left = (int32_t*)dev_alloc(sizeof(int32_t) * ni, tmp);
-      auto* right = (int32_t*)dev_alloc(sizeof(int32_t) * ni, tmp);
-      auto* rfirst = (int32_t*)dev_alloc(sizeof(int32_t) * ni, tmp);
-      auto* rlast = (int32_t*)dev_alloc(sizeof(int32_t) * ni, tmp);
-      auto* pint = (int32_t*)dev_alloc(sizeof(int32_t) * ni, tmp);
-      auto* pleaf = (int32_t*)dev_alloc(sizeof(int32_t) * n, tmp);
-      auto* flags = (unsigned int*)dev_alloc(sizeof(unsigned int) * ni, tmp);
-      auto* ibox = (double*)dev_alloc(sizeof(double) * 6 * ni, tmp);
-      auto* d_depth = (int*)dev_alloc(sizeof(int), tmp);
-      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ni)), dim3(BLK), 0, s, flags, ni, 0u);
-      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ni)), dim3(BLK), 0, s, (unsigned int*)pint, ni, 0xffffffffu);
-      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(BLK), 0, s, (unsigned int*)pleaf, n, 0xffffffffu);
-      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(BLK), 0, s, (unsigned int*)d_depth, 1, 0u);
-      hipLaunchKernelGGL(k_karras, dim3(grid_for(ni)), dim3(BLK), 0, s, keys2, n, left, right, rfirst, rlast, pint,
-                         pleaf);
-      hipLaunchKernelGGL(k_refit, dim3(grid_for(n)), dim3(BLK), 0, s, box6, out.order, n, left, right, pint, pleaf,
-                         flags, ibox);
-      hipLaunchKernelGGL(k_emit, dim3(grid_for(ni)), dim3(BLK), 0, s, box6, out.order, n, leaf_max, left, right,
-                         rfirst, rlast, ibox, out.nodes_d, out.nodes_f);
-      hipLaunchKernelGGL(k_depth, dim3(grid_for(ni)), dim3(BLK), 0, s, n, leaf_max, rfirst, rlast, pint, d_depth);
-      int depth = 0;
-      XDG_HIP_CHECK(hipMemcpyAsync(&depth, d_depth, sizeof(int), hipMemcpyDeviceToHost, s));
-      XDG_HIP_CHECK(hipMemcpyAsync(&hmax, d_max, sizeof(hmax), hipMemcpyDeviceToHost, s));
-      XDG_HIP_CHECK(hipStreamSynchronize(s));
-      out.depth = depth;
+      std::vector<DevBuf>& keep = keep_topology ? owned : tmp;
+      LbvhTopo t;
+      t.left = (int32_t*)dev_alloc(sizeof(int32_t) * ni, keep);
+      t.right = (int32_t*)dev_alloc(sizeof(int32_t) * ni, keep);
+      t.rfirst = (int32_t*)dev_alloc(sizeof(int32_t) * ni, keep);
+      t.rlast = (int32_t*)dev_alloc(sizeof(int32_t) * ni, keep);
+      t.pint = (int32_t*)dev_alloc(sizeof(int32_t) * ni, keep);
+      t.pleaf = (int32_t*)dev_alloc(sizeof(int32_t) * n, keep);
+      t.flags = (unsigned int*)dev_alloc(sizeof(unsigned int) * ni, keep);
+      t.ibox = (double*)dev_alloc(sizeof(double) * 6 * ni, keep);
+      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ni)), dim3(BLK), 0, s, t.flags, ni, 0u);
+      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ni)), dim3(BLK), 0, s, (unsigned int*)t.pint, ni, 0xffffffffu);
+      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(BLK), 0, s, (unsigned int*)t.pleaf, n, 0xffffffffu);
+      hipLaunchKernelGGL(k_karras, dim3(grid_for(ni)), dim3(BLK), 0, s, keys, n, t.left, t.right, t.rfirst, t.rlast,
+                         t.pint, t.pleaf);
+      hipLaunchKernelGGL(k_refit, dim3(grid_for(n)), dim3(BLK), 0, s, box6, out.order, n, t.left, t.right, t.pint,
+                         t.pleaf, t.flags, t.ibox);
+      hipLaunchKernelGGL(k_emit, dim3(grid_for(ni)), dim3(BLK), 0, s, box6, out.order, n, leaf_max, t.left, t.right,
+                         t.rfirst, t.rlast, t.ibox, out.nodes_d, out.nodes_f);
       out.root = n <= leaf_max ? leaf_code(0, (int32_t)n) : 0;
-      if (n <= leaf_max) out.depth = 0;
+      if (keep_topology) out.topo = t;
     }
-    if (n == 1) {
-      XDG_HIP_CHECK(hipMemcpyAsync(&hmax, d_max, sizeof(hmax), hipMemcpyDeviceToHost, s));
-      XDG_HIP_CHECK(hipStreamSynchronize(s));
-    }
+    XDG_HIP_CHECK(hipMemcpyAsync(&hmax, d_max, sizeof(hmax), hipMemcpyDeviceToHost, s));
+    XDG_HIP_CHECK(hipStreamSynchronize(s));
     double m;
     static_assert(sizeof(m) == sizeof(hmax), "size");
     memcpy(&m, &hmax, sizeof m);
     out.max_abs = m;
     XDG_HIP_CHECK(hipGetLastError());
-    const char* ve = std::getenv("XDG_HIP_VALIDATE");
-    if (!(ve && ve[0] == '0')) validate_tree(out, n);
+    // depth and structure from a breadth-first pass over the emitted tree
+    out.depth = validate_tree(out, n, s);
   } catch (...) {
     dev_free_all(tmp);
     throw;
@@ -446,37 +647,226 @@ void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std:
   dev_free_all(tmp);
 }
 
-}  // namespace xdgh
+void lbvh_refit(const double* box6, int64_t n, LbvhOut& out, hipStream_t s)
+{
+  if (n <= 1 || out.root < 0) return;
+  const LbvhTopo& t = out.topo;
+  if (!t.left) throw HipError{"lbvh_refit: the tree was built without its topology"};
+  const int64_t ni = n - 1;
+  std::vector<DevBuf> tmp;
+  try {
+    auto* d_max = (unsigned long long*)dev_alloc(sizeof(unsigned long long), tmp);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (unsigned int*)d_max, 2, 0u);
+    hipLaunchKernelGGL(k_maxabs, dim3(std::min<unsigned>(grid_for(6 * n), 4096)), dim3(BLK), 0, s, box6, n, d_max);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ni)), dim3(BLK), 0, s, t.flags, ni, 0u);
+    hipLaunchKernelGGL(k_refit, dim3(grid_for(n)), dim3(BLK), 0, s, box6, out.order, n, t.left, t.right, t.pint,
+                       t.pleaf, t.flags, t.ibox);
+    hipLaunchKernelGGL(k_emit, dim3(grid_for(ni)), dim3(BLK), 0, s, box6, out.order, n, out.leaf_max, t.left,
+                       t.right, t.rfirst, t.rlast, t.ibox, out.nodes_d, out.nodes_f);
+    unsigned long long hmax = 0;
+    XDG_HIP_CHECK(hipMemcpyAsync(&hmax, d_max, sizeof(hmax), hipMemcpyDeviceToHost, s));
+    XDG_HIP_CHECK(hipStreamSynchronize(s));
+    memcpy(&out.max_abs, &hmax, sizeof hmax);
+    XDG_HIP_CHECK(hipGetLastError());
+  } catch (...) {
+    dev_free_all(tmp);
+    throw;
+  }
+  dev_free_all(tmp);
+}
 
 // ---------------------------------------------------------------------------
-// BVH4 collapse (host): each 4-wide node takes a binary node's two children
-// and repeatedly opens the internal child with the largest surface area until
-// it holds four children (or only leaves remain).  Nodes are emitted
-// breadth-first so that the children of a node are contiguous.  Leaf codes
-// are unchanged, so the BVH4 addresses the same leaf-ordered primitives.
+// BVH4 collapse on the device, level by level from the root.  Each 4-wide
+// node takes a binary node's two children and repeatedly opens the internal
+// child with the largest surface area (first such child on ties) until it
+// holds four children or only leaves remain.  A level's items count their
+// internal children, an exclusive scan numbers them, and the next level is
+// written in that order, so node numbering is deterministic and the
+// children of a node are contiguous.  Leaf codes are unchanged, so the BVH4
+// addresses the same leaf-ordered primitives.  The pass also checks the
+// result: children in range, every primitive covered once by a reachable
+// leaf, the stack bound (pushes along any root-to-leaf path).
 // ---------------------------------------------------------------------------
-namespace xdgh {
-
 namespace {
 
-struct Cand {
-  int32_t code;
-  double box[6];
-  float fbox[6];
+struct C4Item {
+  int32_t bnode;  // binary node this BVH4 node is built from
+  int32_t slot;   // BVH4 node index
+  int32_t above;  // stack entries pushed above it on its path
 };
 
-double half_area(const double* b)
+__device__ __forceinline__ double half_area_d(const double* b)
 {
   const double dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
   return dx * dy + dy * dz + dz * dx;
 }
 
+__device__ __forceinline__ const double* src_box(const NodeD* nd, int32_t src)
+{
+  return (src & 1) ? nd[src >> 1].b1 : nd[src >> 1].b0;
+}
+
+// children of the BVH4 node over binary node b: codes and box sources
+__device__ __forceinline__ int c4_children(const NodeD* __restrict__ nd, int64_t n_nodes, int32_t b, int32_t* code,
+                                           int32_t* src, bool* bad)
+{
+  code[0] = nd[b].c0;
+  src[0] = 2 * b;
+  code[1] = nd[b].c1;
+  src[1] = 2 * b + 1;
+  int nc = 2;
+  while (nc < 4) {
+    int best = -1;
+    double area = -1.0;
+    for (int j = 0; j < nc; ++j)
+      if (code[j] >= 0) {
+        const double a = half_area_d(src_box(nd, src[j]));
+        if (a > area) {
+          area = a;
+          best = j;
+        }
+      }
+    if (best < 0) break;
+    const int32_t x = code[best];
+    if (x >= n_nodes) {
+      *bad = true;
+      return 0;
+    }
+    code[best] = nd[x].c0;
+    src[best] = 2 * x;
+    code[nc] = nd[x].c1;
+    src[nc] = 2 * x + 1;
+    ++nc;
+  }
+  return nc;
+}
+
+__global__ void k_c4_count(const NodeD* __restrict__ nd, int64_t n_nodes, const C4Item* __restrict__ items,
+                           unsigned int nitems, uint32_t* cnt, unsigned int* err)
+{
+  const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nitems) return;
+  int32_t code[4], src[4];
+  bool bad = items[i].bnode < 0 || items[i].bnode >= n_nodes;
+  int nc = bad ? 0 : c4_children(nd, n_nodes, items[i].bnode, code, src, &bad);
+  uint32_t k = 0;
+  for (int j = 0; j < nc; ++j) k += code[j] >= 0 ? 1u : 0u;
+  if (bad) {
+    atomicOr(err, 1u);
+    k = 0;
+  }
+  cnt[i] = k;
+}
+
+__global__ void k_c4_emit(const NodeD* __restrict__ nd, const NodeF* __restrict__ nf, int64_t n_nodes, int64_t n,
+                          const C4Item* __restrict__ items, unsigned int nitems, const uint32_t* __restrict__ offs,
+                          int32_t next_base, int64_t cap, Node4D* o4d, Node4F* o4f, int32_t* src4, C4Item* next,
+                          unsigned int* seen, int* max_stack, unsigned int* err)
+{
+  const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nitems) return;
+  const C4Item it = items[i];
+  if (it.slot < 0 || it.slot >= cap) {
+    atomicOr(err, 1u);
+    return;
+  }
+  int32_t code[4], src[4];
+  bool bad = it.bnode < 0 || it.bnode >= n_nodes;
+  const int nc = bad ? 0 : c4_children(nd, n_nodes, it.bnode, code, src, &bad);
+  if (bad) {
+    atomicOr(err, 1u);
+    return;
+  }
+  const int32_t above = it.above + (nc - 1);
+  atomicMax(max_stack, above);
+  Node4D d4;
+  Node4F f4;
+  memset((void*)&d4, 0, sizeof d4);
+  memset((void*)&f4, 0, sizeof f4);
+  uint32_t rank = 0;
+  for (int j = 0; j < 4; ++j) {
+    if (j < nc) {
+      const double* b = src_box(nd, src[j]);
+      const float* bf = (src[j] & 1) ? nf[src[j] >> 1].b1 : nf[src[j] >> 1].b0;
+      d4.lox[j] = b[0];
+      d4.loy[j] = b[1];
+      d4.loz[j] = b[2];
+      d4.hix[j] = b[3];
+      d4.hiy[j] = b[4];
+      d4.hiz[j] = b[5];
+      f4.lox[j] = bf[0];
+      f4.loy[j] = bf[1];
+      f4.loz[j] = bf[2];
+      f4.hix[j] = bf[3];
+      f4.hiy[j] = bf[4];
+      f4.hiz[j] = bf[5];
+      if (code[j] >= 0) {
+        const int32_t slot = next_base + (int32_t)(offs[i] + rank);
+        next[offs[i] + rank] = C4Item{code[j], slot, above};
+        ++rank;
+        d4.child[j] = f4.child[j] = slot;
+      } else {
+        const int64_t f = leaf_first(code[j]), k = leaf_count(code[j]);
+        if (f < 0 || f + k > n) {
+          atomicOr(err, 1u);
+        } else {
+          for (int64_t p = f; p < f + k; ++p) atomicAdd(seen + p, 1u);
+        }
+        d4.child[j] = f4.child[j] = code[j];
+      }
+      if (src4) src4[4 * (int64_t)it.slot + j] = src[j];
+    } else {
+      // inverted box: the near-plane entry distance is +inf for every ray
+      d4.lox[j] = d4.loy[j] = d4.loz[j] = HUGE_VAL;
+      d4.hix[j] = d4.hiy[j] = d4.hiz[j] = -HUGE_VAL;
+      f4.lox[j] = f4.loy[j] = f4.loz[j] = HUGE_VALF;
+      f4.hix[j] = f4.hiy[j] = f4.hiz[j] = -HUGE_VALF;
+      d4.child[j] = f4.child[j] = INT32_MIN;
+      if (src4) src4[4 * (int64_t)it.slot + j] = -1;
+    }
+  }
+  o4d[it.slot] = d4;
+  o4f[it.slot] = f4;
+}
+
+__global__ void k_c4_refit(const NodeD* __restrict__ nd, const NodeF* __restrict__ nf, const int32_t* __restrict__ src4,
+                           int64_t n4, Node4D* o4d, Node4F* o4f)
+{
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= 4 * n4) return;
+  const int64_t node = q >> 2;
+  const int j = (int)(q & 3);
+  const int32_t src = src4[q];
+  if (src < 0) return;  // empty slot keeps its inverted box
+  const double* b = src_box(nd, src);
+  const float* bf = (src & 1) ? nf[src >> 1].b1 : nf[src >> 1].b0;
+  Node4D& d4 = o4d[node];
+  Node4F& f4 = o4f[node];
+  d4.lox[j] = b[0];
+  d4.loy[j] = b[1];
+  d4.loz[j] = b[2];
+  d4.hix[j] = b[3];
+  d4.hiy[j] = b[4];
+  d4.hiz[j] = b[5];
+  f4.lox[j] = bf[0];
+  f4.loy[j] = bf[1];
+  f4.loz[j] = bf[2];
+  f4.hix[j] = bf[3];
+  f4.hiy[j] = bf[4];
+  f4.hiz[j] = bf[5];
+}
+
+__global__ void k_copy_bytes16(const uint4* __restrict__ src, uint4* dst, int64_t n16)
+{
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n16) dst[i] = src[i];
+}
+
 }  // namespace
 
-void bvh4_collapse(const LbvhOut& bin, int64_t n, std::vector<DevBuf>& owned, Bvh4Out& out)
+void bvh4_collapse(const LbvhOut& bin, int64_t n, hipStream_t s, std::vector<DevBuf>& owned, Bvh4Out& out,
+                   bool keep_sources)
 {
-  using namespace xdgd;
-  const int32_t EMPTY = INT32_MIN;
   if (bin.root < 0 || bin.n_nodes == 0) {  // a single leaf: no nodes
     out.root = bin.root;
     out.stack = 0;
@@ -485,143 +875,74 @@ void bvh4_collapse(const LbvhOut& bin, int64_t n, std::vector<DevBuf>& owned, Bv
     out.nodes_f = (Node4F*)dev_alloc(sizeof(Node4F), owned);
     return;
   }
-  std::vector<NodeD> bd((size_t)bin.n_nodes);
-  std::vector<NodeF> bf((size_t)bin.n_nodes);
-  XDG_HIP_CHECK(hipMemcpy(bd.data(), bin.nodes_d, sizeof(NodeD) * bin.n_nodes, hipMemcpyDeviceToHost));
-  XDG_HIP_CHECK(hipMemcpy(bf.data(), bin.nodes_f, sizeof(NodeF) * bin.n_nodes, hipMemcpyDeviceToHost));
+  std::vector<DevBuf> tmp;
+  try {
+    // a 4-wide node absorbs at least one binary node: n_nodes bounds the count
+    const int64_t cap = bin.n_nodes;
+    auto* t4d = (Node4D*)dev_alloc(sizeof(Node4D) * (size_t)cap, tmp);
+    auto* t4f = (Node4F*)dev_alloc(sizeof(Node4F) * (size_t)cap, tmp);
+    int32_t* tsrc = keep_sources ? (int32_t*)dev_alloc(sizeof(int32_t) * 4 * (size_t)cap, tmp) : nullptr;
+    auto* lv0 = (C4Item*)dev_alloc(sizeof(C4Item) * (size_t)cap, tmp);
+    auto* lv1 = (C4Item*)dev_alloc(sizeof(C4Item) * (size_t)cap, tmp);
+    auto* cnt = (uint32_t*)dev_alloc(sizeof(uint32_t) * (size_t)cap, tmp);
+    auto* seen = (unsigned int*)dev_alloc(sizeof(unsigned int) * (size_t)n, tmp);
+    auto* misc = (unsigned int*)dev_alloc(sizeof(unsigned int) * 4, tmp);  // total, err, max_stack
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(BLK), 0, s, seen, n, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, misc, 4, 0u);
+    const C4Item root{bin.root, 0, 0};
+    XDG_HIP_CHECK(hipMemcpyAsync(lv0, &root, sizeof root, hipMemcpyHostToDevice, s));
+    unsigned int nin = 1;
+    int64_t slots = 1;
+    for (int level = 0; nin > 0; ++level) {
+      if (level > 64) throw HipError{"bvh4: deeper than 64 levels"};
+      hipLaunchKernelGGL(k_c4_count, dim3(grid_for(nin)), dim3(BLK), 0, s, bin.nodes_d, bin.n_nodes, lv0, nin, cnt,
+                         misc + 1);
+      hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, cnt, (int64_t)nin, misc);
+      unsigned int total = 0;
+      XDG_HIP_CHECK(hipMemcpyAsync(&total, misc, sizeof total, hipMemcpyDeviceToHost, s));
+      XDG_HIP_CHECK(hipStreamSynchronize(s));
+      if (slots + (int64_t)total > cap) throw HipError{"bvh4: more nodes than the binary tree has"};
+      hipLaunchKernelGGL(k_c4_emit, dim3(grid_for(nin)), dim3(BLK), 0, s, bin.nodes_d, bin.nodes_f, bin.n_nodes, n,
+                         lv0, nin, cnt, (int32_t)slots, cap, t4d, t4f, tsrc, lv1, seen, (int*)(misc + 2), misc + 1);
+      slots += total;
+      nin = total;
+      std::swap(lv0, lv1);
+    }
+    hipLaunchKernelGGL(k_cover_check, dim3(grid_for(n)), dim3(BLK), 0, s, seen, (const int32_t*)nullptr,
+                       (unsigned int*)nullptr, n, misc + 1);
+    unsigned int h[3] = {0, 0, 0};
+    XDG_HIP_CHECK(hipMemcpyAsync(h, misc, sizeof h, hipMemcpyDeviceToHost, s));
+    XDG_HIP_CHECK(hipStreamSynchronize(s));
+    if (h[1] & 1u) throw HipError{"bvh4: child index or leaf range out of bounds"};
+    if (h[1] & 4u) throw HipError{"bvh4: a primitive is not covered exactly once"};
+    // right-sized final arrays, copied on the device
+    out.n_nodes = slots;
+    out.nodes_d = (Node4D*)dev_alloc(sizeof(Node4D) * (size_t)slots, owned);
+    out.nodes_f = (Node4F*)dev_alloc(sizeof(Node4F) * (size_t)slots, owned);
+    XDG_HIP_CHECK(hipMemcpyAsync(out.nodes_d, t4d, sizeof(Node4D) * (size_t)slots, hipMemcpyDeviceToDevice, s));
+    XDG_HIP_CHECK(hipMemcpyAsync(out.nodes_f, t4f, sizeof(Node4F) * (size_t)slots, hipMemcpyDeviceToDevice, s));
+    if (keep_sources) {
+      out.src = (int32_t*)dev_alloc(sizeof(int32_t) * 4 * (size_t)slots, owned);
+      XDG_HIP_CHECK(hipMemcpyAsync(out.src, tsrc, sizeof(int32_t) * 4 * (size_t)slots, hipMemcpyDeviceToDevice, s));
+    }
+    XDG_HIP_CHECK(hipStreamSynchronize(s));
+    // the stack never holds more than the pushes along one root-to-leaf path
+    out.stack = std::max((int32_t)h[2], 1);
+    out.root = 0;
+  } catch (...) {
+    dev_free_all(tmp);
+    throw;
+  }
+  dev_free_all(tmp);
+}
 
-  std::vector<Node4D> od;
-  std::vector<Node4F> of;
-  od.reserve((size_t)bin.n_nodes / 2 + 1);
-  of.reserve((size_t)bin.n_nodes / 2 + 1);
-  // work item: binary node index -> BVH4 slot to fill, and the stack entries
-  // pushed above it on the worst path
-  struct Item {
-    int32_t bnode;
-    int32_t slot;
-    int32_t above;
-  };
-  std::vector<Item> queue;
-  od.emplace_back();
-  of.emplace_back();
-  queue.push_back({bin.root, 0, 0});
-  int32_t max_stack = 0;
-  for (size_t q = 0; q < queue.size(); ++q) {
-    const Item it = queue[q];
-    const NodeD& b = bd[it.bnode];
-    const NodeF& f = bf[it.bnode];
-    std::vector<Cand> cs(2);
-    cs[0].code = b.c0;
-    cs[1].code = b.c1;
-    for (int k = 0; k < 6; ++k) {
-      cs[0].box[k] = b.b0[k];
-      cs[1].box[k] = b.b1[k];
-      cs[0].fbox[k] = f.b0[k];
-      cs[1].fbox[k] = f.b1[k];
-    }
-    while (cs.size() < 4) {
-      int best = -1;
-      double area = -1.0;
-      for (int j = 0; j < (int)cs.size(); ++j)
-        if (cs[j].code >= 0 && half_area(cs[j].box) > area) {
-          area = half_area(cs[j].box);
-          best = j;
-        }
-      if (best < 0) break;
-      const NodeD& cb = bd[cs[best].code];
-      const NodeF& cf = bf[cs[best].code];
-      Cand a, c;
-      a.code = cb.c0;
-      c.code = cb.c1;
-      for (int k = 0; k < 6; ++k) {
-        a.box[k] = cb.b0[k];
-        c.box[k] = cb.b1[k];
-        a.fbox[k] = cf.b0[k];
-        c.fbox[k] = cf.b1[k];
-      }
-      cs[best] = a;
-      cs.push_back(c);
-    }
-    const int32_t nc = (int32_t)cs.size();
-    const int32_t above = it.above + (nc - 1);
-    max_stack = std::max(max_stack, above);
-    Node4D nd;
-    Node4F nf;
-    std::memset((void*)&nd, 0, sizeof nd);
-    std::memset((void*)&nf, 0, sizeof nf);
-    for (int j = 0; j < 4; ++j) {
-      if (j < nc) {
-        nd.lox[j] = cs[j].box[0];
-        nd.loy[j] = cs[j].box[1];
-        nd.loz[j] = cs[j].box[2];
-        nd.hix[j] = cs[j].box[3];
-        nd.hiy[j] = cs[j].box[4];
-        nd.hiz[j] = cs[j].box[5];
-        nf.lox[j] = cs[j].fbox[0];
-        nf.loy[j] = cs[j].fbox[1];
-        nf.loz[j] = cs[j].fbox[2];
-        nf.hix[j] = cs[j].fbox[3];
-        nf.hiy[j] = cs[j].fbox[4];
-        nf.hiz[j] = cs[j].fbox[5];
-      } else {
-        // inverted box: the near-plane entry distance is +inf for every ray
-        nd.lox[j] = nd.loy[j] = nd.loz[j] = HUGE_VAL;
-        nd.hix[j] = nd.hiy[j] = nd.hiz[j] = -HUGE_VAL;
-        nf.lox[j] = nf.loy[j] = nf.loz[j] = HUGE_VALF;
-        nf.hix[j] = nf.hiy[j] = nf.hiz[j] = -HUGE_VALF;
-        nd.child[j] = nf.child[j] = EMPTY;
-      }
-    }
-    for (int j = 0; j < nc; ++j) {
-      if (cs[j].code >= 0) {
-        const int32_t slot = (int32_t)od.size();
-        od.emplace_back();
-        of.emplace_back();
-        queue.push_back({cs[j].code, slot, above});
-        nd.child[j] = nf.child[j] = slot;
-      } else {
-        nd.child[j] = nf.child[j] = cs[j].code;
-      }
-    }
-    od[it.slot] = nd;
-    of[it.slot] = nf;
-  }
-  // structural check before any kernel reads the BVH4: child indices in
-  // range, every primitive covered exactly once
-  {
-    std::vector<uint8_t> seen((size_t)n, 0);
-    int64_t covered = 0;
-    for (const Node4D& nd : od) {
-      for (int j = 0; j < 4; ++j) {
-        const int32_t c = nd.child[j];
-        if (c == EMPTY) {
-          if (!(nd.lox[j] == HUGE_VAL && nd.hix[j] == -HUGE_VAL)) throw HipError{"bvh4: empty slot not inverted"};
-          continue;
-        }
-        if (c >= 0) {
-          if ((size_t)c >= od.size()) throw HipError{"bvh4: child index out of range"};
-          continue;
-        }
-        const int64_t f = leaf_first(c), k = leaf_count(c);
-        if (f < 0 || f + k > n) throw HipError{"bvh4: leaf range out of bounds"};
-        for (int64_t p = f; p < f + k; ++p) {
-          if (seen[p]) throw HipError{"bvh4: primitive covered twice"};
-          seen[p] = 1;
-        }
-        covered += k;
-      }
-    }
-    if (covered != n) throw HipError{"bvh4: not every primitive is reachable"};
-  }
-  // the stack never holds more than the pushes along one root-to-leaf path
-  out.stack = std::max(max_stack, 1);
-  out.root = 0;
-  out.n_nodes = (int64_t)od.size();
-  out.nodes_d = (Node4D*)dev_alloc(sizeof(Node4D) * od.size(), owned);
-  out.nodes_f = (Node4F*)dev_alloc(sizeof(Node4F) * of.size(), owned);
-  XDG_HIP_CHECK(hipMemcpy(out.nodes_d, od.data(), sizeof(Node4D) * od.size(), hipMemcpyHostToDevice));
-  XDG_HIP_CHECK(hipMemcpy(out.nodes_f, of.data(), sizeof(Node4F) * of.size(), hipMemcpyHostToDevice));
-  (void)n;
+void bvh4_refit(const LbvhOut& bin, const Bvh4Out& b4, hipStream_t s)
+{
+  if (b4.n_nodes == 0) return;
+  if (!b4.src) throw HipError{"bvh4_refit: the tree was built without its box sources"};
+  hipLaunchKernelGGL(k_c4_refit, dim3(grid_for(4 * b4.n_nodes)), dim3(BLK), 0, s, bin.nodes_d, bin.nodes_f, b4.src,
+                     b4.n_nodes, b4.nodes_d, b4.nodes_f);
+  XDG_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace xdgh

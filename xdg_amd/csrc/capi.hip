@@ -28,6 +28,12 @@ struct SurfTreeHost {
   int64_t n_nodes = 0;
   int64_t n_nodes4 = 0;
   int64_t bytes = 0;
+  // kept for xdg_hip_refit_surface_tree (trees built with the "refit" option)
+  bool refittable = false;
+  LbvhOut lb;
+  Bvh4Out b4;
+  int32_t* tri_idx = nullptr;  // 3 vertex indices per triangle, caller's order
+  int64_t n_verts = 0;
 };
 
 // One registered transport problem: device copies of the topology, the
@@ -123,6 +129,7 @@ struct xdg_hip_ctx {
   hipEvent_t scratch_ev = nullptr;
   LaunchCfg cfg;
   int leaf_max = 2;
+  int refit = 0;  // surface trees built next keep what a refit needs
   int max_surf_depth = 1;
   int max_surf_stack4 = 1;
   int max_elem_depth = 1;
@@ -178,6 +185,19 @@ const ElemTreeHost& elem_tree(xdg_hip_ctx* c, int32_t tree)
 
 hipStream_t pick(xdg_hip_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
 
+// FP32 box test error for origin o: at most (3u max|x| + 5u |o|) / |d_i| in
+// ray-parameter units (u = 2^-24) against a margin of bump / |d_i|, so
+// bump * 2^20 - max|x| keeps it below 5/16 of the margin.  The 2^26 cap
+// keeps fl32(o) times the clamped reciprocal 1e30 of a near-axis direction
+// finite.  Rounded down to a float.
+float mixed_limit(double bump, double max_abs)
+{
+  const double lim = std::min(bump * 1048576.0 - max_abs, 67108864.0);
+  float f = lim > 0.0 ? (float)lim : 0.0f;
+  if ((double)f > lim) f = nextafterf(f, 0.0f);
+  return f;
+}
+
 bool capturing(hipStream_t s)
 {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -211,6 +231,65 @@ __global__ void k_tally_guard(const long long* __restrict__ tally, int64_t n, ui
     near |= v >= lim || v < -lim;
   }
   if (near) atomicOr(status, bit);
+}
+
+// ---- refit of a surface tree (xdg_hip_refit_surface_tree) ----------------
+__global__ void k_gather_v9(const double* __restrict__ verts, const int32_t* __restrict__ tri_idx, int64_t n,
+                            double* v9)
+{
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int j = 0; j < 3; ++j) {
+    const int64_t v = tri_idx[3 * p + j];
+    for (int k = 0; k < 3; ++k) v9[9 * p + 3 * j + k] = verts[3 * v + k];
+  }
+}
+
+// new vertices into the leaf-ordered records; face, surface and flip kept,
+// the edge endpoint orders recomputed from the new coordinates
+__global__ void k_refit_tris(const double* __restrict__ v9, const int32_t* __restrict__ order, int64_t n, TriRec* tris)
+{
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int64_t p = order[s];
+  TriRec r = tris[s];
+  for (int k = 0; k < 9; ++k) r.v[k] = v9[9 * p + k];
+  const unsigned fb = tri_first_bits(d3(r.v[0], r.v[1], r.v[2]), d3(r.v[3], r.v[4], r.v[5]), d3(r.v[6], r.v[7], r.v[8]));
+  r.surf2 = (r.surf2 & ~0xE) | (int32_t)(fb << 1);
+  tris[s] = r;
+}
+
+__device__ __forceinline__ unsigned long long ord_key(double x)
+{
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+__host__ __device__ inline double ord_val(unsigned long long u)
+{
+  const unsigned long long b = (u >> 63) ? (u & 0x7fffffffffffffffULL) : ~u;
+  double d;
+  memcpy(&d, &b, sizeof d);
+  return d;
+}
+
+// min / max vertex coordinate per axis (ordered-integer atomics); cb[0..2]
+// start at ~0, cb[3..5] at 0
+__global__ void k_v9_bounds(const double* __restrict__ v9, int64_t n, unsigned long long* cb)
+{
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int k = 0; k < 3; ++k) {
+    const double a = v9[9 * p + k], b = v9[9 * p + 3 + k], c = v9[9 * p + 6 + k];
+    atomicMin(cb + k, ord_key(fmin(fmin(a, b), c)));
+    atomicMax(cb + 3 + k, ord_key(fmax(fmax(a, b), c)));
+  }
+}
+
+__global__ void k_init_minmax(unsigned long long* cb)
+{
+  const int i = threadIdx.x;
+  if (i < 3) cb[i] = ~0ULL;
+  else if (i < 6) cb[i] = 0ULL;
 }
 
 // Sets `bit` of the status word when an int64 fixed-point tally has reached
@@ -494,12 +573,14 @@ int* opt_defer(xdg_hip_ctx* c) { return &c->cfg.defer; }
 int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
 int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
 int* opt_mw_wave(xdg_hip_ctx* c) { return &c->cfg.mw_wave; }
+int* opt_refit(xdg_hip_ctx* c) { return &c->refit; }
 
 const OptionRef OPTIONS[] = {
     {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 7, opt_persistent},
     {"ray_order", 0, 2, opt_order},         {"speculative", 0, 1, opt_spec}, {"wide", 0, 1, opt_wide},
     {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
     {"refill", 1, 64, opt_refill},          {"mw_wave", 0, 1, opt_mw_wave},
+    {"refit", 0, 1, opt_refit},
 };
 
 const OptionRef& find_option(const char* name)
@@ -594,8 +675,9 @@ int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
         XDG_HIP_CHECK(hipMemcpyAsync(d_flip, flip, (size_t)n_tris, hipMemcpyHostToDevice, s));
       }
       launch_tri_boxes(d_v9, n_tris, bump, d_box, s);
+      const bool keep = c->refit != 0;
       LbvhOut lb;
-      lbvh_build(d_box, n_tris, c->leaf_max, s, t.bufs, lb);
+      lbvh_build(d_box, n_tris, c->leaf_max, s, t.bufs, lb, keep);
       TriRec* d_tris = (TriRec*)dev_alloc(nt * sizeof(TriRec), t.bufs);
       launch_write_tris(d_v9, d_face, d_surf, d_flip, lb.order, n_tris, d_tris, s);
       double* d_nrm = (double*)dev_alloc(nt * 3 * sizeof(double), t.bufs);
@@ -604,7 +686,16 @@ int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
       XDG_HIP_CHECK(hipGetLastError());
       XDG_HIP_CHECK(hipStreamSynchronize(s));
       Bvh4Out b4;
-      bvh4_collapse(lb, n_tris, t.bufs, b4);
+      bvh4_collapse(lb, n_tris, s, t.bufs, b4, keep);
+      if (keep) {
+        t.refittable = true;
+        t.lb = lb;
+        t.b4 = b4;
+        t.n_verts = n_verts;
+        t.tri_idx = (int32_t*)dev_alloc(sizeof(int32_t) * 3 * nt, t.bufs);
+        if (n_tris > 0)
+          XDG_HIP_CHECK(hipMemcpyAsync(t.tri_idx, tris, sizeof(int32_t) * 3 * (size_t)n_tris, hipMemcpyHostToDevice, s));
+      }
       t.dev.nodes4_d = b4.nodes_d;
       t.dev.nodes4_f = b4.nodes_f;
       t.dev.root4 = b4.root;
@@ -632,9 +723,7 @@ int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
       // so bump * 2^20 - max|x| keeps it below 5/16 of the margin.  The 2^26
       // cap keeps fl32(o) times the clamped reciprocal 1e30 of a near-axis
       // direction finite.
-      double lim = std::min(bump * 1048576.0 - lb.max_abs, 67108864.0);
-      t.dev.o_limit = lim > 0.0 ? (float)lim : 0.0f;
-      if ((double)t.dev.o_limit > lim) t.dev.o_limit = nextafterf(t.dev.o_limit, 0.0f);
+      t.dev.o_limit = mixed_limit(bump, lb.max_abs);
       t.n_nodes = lb.n_nodes;
       t.bytes = bufs_bytes(t.bufs);
       t.min_face = min_face;
@@ -801,6 +890,59 @@ int xdg_hip_release_surface_tree(xdg_hip_ctx* c, int32_t tree)
     dev_free_all(t.bufs);
     t.dev = SurfTreeDev{};
     t.released = true;
+    sync_tables(c);
+    return XDG_HIP_OK;
+  });
+}
+
+int xdg_hip_refit_surface_tree(xdg_hip_ctx* c, int32_t tree, int64_t n_verts, const double* verts, double bump)
+{
+  return guarded(c, [&]() {
+    surf_tree(c, tree);
+    SurfTreeHost& t = c->surf[tree - 1];
+    require(t.refittable, "the tree was not built with the \"refit\" option set");
+    require(n_verts == t.n_verts, "n_verts differs from the build");
+    require(n_verts == 0 || verts != nullptr, "null verts");
+    require(std::isfinite(bump), "bump must be finite");
+    const double b = bump >= 0.0 ? bump : t.dev.bump;
+    const int64_t n = t.dev.n_prims;
+    hipStream_t s = c->stream;
+    XDG_HIP_CHECK(hipDeviceSynchronize());  // no query may read the tree meanwhile
+    if (n > 0) {
+      std::vector<DevBuf> tmp;
+      try {
+        double* d_v9 = (double*)dev_alloc(sizeof(double) * 9 * (size_t)n, tmp);
+        double* d_box = (double*)dev_alloc(sizeof(double) * 6 * (size_t)n, tmp);
+        auto* cb = (unsigned long long*)dev_alloc(sizeof(unsigned long long) * 6, tmp);
+        const unsigned g = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL(k_gather_v9, dim3(g), dim3(256), 0, s, verts, t.tri_idx, n, d_v9);
+        launch_tri_boxes(d_v9, n, b, d_box, s);
+        hipLaunchKernelGGL(k_refit_tris, dim3(g), dim3(256), 0, s, d_v9, t.lb.order, n, (TriRec*)t.dev.tris);
+        launch_tri_normals(t.dev.tris, n, (double*)t.dev.normals, s);
+        hipLaunchKernelGGL(k_init_minmax, dim3(1), dim3(64), 0, s, cb);
+        hipLaunchKernelGGL(k_v9_bounds, dim3(g), dim3(256), 0, s, d_v9, n, cb);
+        lbvh_refit(d_box, n, t.lb, s);
+        bvh4_refit(t.lb, t.b4, s);
+        unsigned long long h[6];
+        XDG_HIP_CHECK(hipMemcpyAsync(h, cb, sizeof h, hipMemcpyDeviceToHost, s));
+        XDG_HIP_CHECK(hipStreamSynchronize(s));
+        for (int k = 0; k < 3; ++k) {
+          t.dev.box_lo[k] = (float)ord_val(h[k]);
+          t.dev.box_hi[k] = (float)ord_val(h[3 + k]);
+        }
+        if (n == 1) {  // no internal nodes: the refit kept no max |x|
+          double mx = 0.0;
+          for (int k = 0; k < 6; ++k) mx = std::max(mx, std::fabs(ord_val(h[k])));
+          t.lb.max_abs = mx + b;
+        }
+      } catch (...) {
+        dev_free_all(tmp);
+        throw;
+      }
+      dev_free_all(tmp);
+    }
+    t.dev.bump = b;
+    t.dev.o_limit = mixed_limit(b, t.lb.max_abs);
     sync_tables(c);
     return XDG_HIP_OK;
   });

@@ -31,6 +31,20 @@ struct DevBuf {
 void* dev_alloc(size_t bytes, std::vector<DevBuf>& owned);
 void dev_free_all(std::vector<DevBuf>& owned);
 
+// Binary-tree topology kept after a build so the boxes can be refitted for
+// moved primitives (same connectivity): Karras children, parents and leaf
+// ranges, the refit arrival flags and the internal node boxes.
+struct LbvhTopo {
+  int32_t* left = nullptr;
+  int32_t* right = nullptr;
+  int32_t* pint = nullptr;   // parent of internal node i (-1 for the root)
+  int32_t* pleaf = nullptr;  // parent of leaf position s
+  int32_t* rfirst = nullptr;
+  int32_t* rlast = nullptr;
+  unsigned int* flags = nullptr;
+  double* ibox = nullptr;    // 6 per internal node
+};
+
 // Result of one LBVH build over n primitive boxes.
 struct LbvhOut {
   xdgd::NodeD* nodes_d = nullptr;  // n-1 slots (unreachable slots left unused)
@@ -39,22 +53,35 @@ struct LbvhOut {
   int32_t root = 0;
   int32_t depth = 0;
   int64_t n_nodes = 0;
+  int32_t leaf_max = 2;
   double max_abs = 0.0;            // max |coordinate| over all boxes
+  LbvhTopo topo;                   // kept only when a refit is wanted
 };
 
-// box6: device array of n boxes (lo.xyz, hi.xyz), already dilated.
+// box6: device array of n boxes (lo.xyz, hi.xyz), already dilated.  The
+// whole build runs on the device: 63-bit Morton codes, a stable LSD radix
+// sort, the Karras hierarchy, a bottom-up refit and a structural check by
+// breadth-first levels; only counts and the max |coordinate| are read back.
 void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std::vector<DevBuf>& owned,
-                LbvhOut& out);
+                LbvhOut& out, bool keep_topology = false);
+// New boxes for the same primitives (same order and connectivity): node
+// boxes recomputed bottom-up, topology unchanged (needs out.topo).
+void lbvh_refit(const double* box6, int64_t n, LbvhOut& out, hipStream_t s);
 
-// 4-wide tree over the same leaves, collapsed from a binary LBVH.
+// 4-wide tree over the same leaves, collapsed from a binary LBVH on the
+// device, level by level.
 struct Bvh4Out {
   xdgd::Node4D* nodes_d = nullptr;
   xdgd::Node4F* nodes_f = nullptr;
   int32_t root = 0;
   int32_t stack = 0;  // max stack entries any traversal can need
   int64_t n_nodes = 0;
+  int32_t* src = nullptr;  // 4 per node: binary (node << 1 | side) each child box comes from, -1 empty
 };
-void bvh4_collapse(const LbvhOut& bin, int64_t n, std::vector<DevBuf>& owned, Bvh4Out& out);
+void bvh4_collapse(const LbvhOut& bin, int64_t n, hipStream_t s, std::vector<DevBuf>& owned, Bvh4Out& out,
+                   bool keep_sources = false);
+// Child boxes of the BVH4 copied again from the (refitted) binary nodes.
+void bvh4_refit(const LbvhOut& bin, const Bvh4Out& b4, hipStream_t s);
 
 // ---- surface queries (surface_query.hip) ---------------------------------
 struct RayFireArgs {
