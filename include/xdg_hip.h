@@ -124,6 +124,8 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *                 (1), or each lane traces its own at once (0) (default 1)
  *   "refit"       surface trees built while set keep their topology, so
  *                 xdg_hip_refit_surface_tree can move their vertices (0)
+ *   "segment_order" segment batches of >= 65536: walked in the Morton order
+ *                 of their start points (1) or in the caller's order (0)
  * Unknown names give XDG_HIP_ERR_NOT_FOUND, out-of-range values
  * XDG_HIP_ERR_INVALID. */
 int xdg_hip_set_option(xdg_hip_ctx* ctx, const char* name, int value);

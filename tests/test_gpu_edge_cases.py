@@ -182,3 +182,32 @@ def test_lattice_degenerate_segments(ctx, torch_mod):
     c, h = ctx.tally_segments_volume(et, vol, dev(torch, p0), dev(torch, p1))
     assert np.array_equal(host(c), rc) and np.array_equal(host(h).view(np.uint64), rh)
     assert ctx.status() == (2 if (rf & 2).any() else 0)
+
+
+def test_segment_order_is_result_neutral(ctx, torch_mod):
+    """Walking a batch in Morton order of the start points (the
+    "segment_order" option) changes nothing: counts, sequence hashes and
+    tallies equal the caller-order walk and the oracle, with re-entry."""
+    torch = torch_mod
+    mesh = G.kuhn_cube(16, 0.2, seed=4)
+    et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
+    bm = mesh.boundary_model()
+    v9, f, s, fl = bm.tree_arrays(bm.ipc)
+    st = ctx.build_surface_tree(v9.reshape(-1, 3), np.arange(3 * v9.shape[0], dtype=np.int32).reshape(-1, 3),
+                                f, s, fl, bm.bump(bm.ipc))
+    ctx.attach_boundary(et, st, mesh.bface_elem)
+    n = 100000
+    p0 = P.box_points(21, P.STREAM_POS, n, [-0.1] * 3, [1.1] * 3)
+    p1 = P.box_points(21, P.STREAM_DIR, n, [-0.1] * 3, [1.1] * 3)
+    out = {}
+    for mode in (0, 1):
+        ctx.set_option("segment_order", mode)
+        tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device="cuda")
+        cnt, hsh = ctx.tally_segments(et, dev(torch, p0), dev(torch, p1), tally=tally)
+        out[mode] = (host(cnt), host(hsh), host(tally))
+    ctx.set_option("segment_order", 0)
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    rc, rh, rt = O.TetMesh(mesh).segments(p0, p1, tally=True)
+    assert np.array_equal(out[1][0], rc) and np.array_equal(out[1][1].view(np.uint64), rh)
+    assert np.array_equal(out[1][2], rt)

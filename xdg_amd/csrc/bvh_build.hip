@@ -141,7 +141,8 @@ constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_BINS = 16;
 
-__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+template <class K>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K* __restrict__ keys, int64_t n, int shift,
                                                        uint32_t* hist, int64_t ntiles)
 {
   __shared__ uint32_t c[RS_BINS];
@@ -205,12 +206,12 @@ __device__ __forceinline__ uint32_t field16(const uint64_t* w, uint32_t d)
   return (uint32_t)((w[d >> 2] >> (16 * (d & 3))) & 0xffffull);
 }
 
-__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t* __restrict__ kin,
-                                                          const int32_t* __restrict__ vin, uint64_t* kout,
-                                                          int32_t* vout, int64_t n, int shift,
+template <class K>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K* __restrict__ kin, const int32_t* __restrict__ vin,
+                                                          K* kout, int32_t* vout, int64_t n, int shift,
                                                           const uint32_t* __restrict__ off, int64_t ntiles)
 {
-  __shared__ uint64_t sk[RS_TILE];
+  __shared__ K sk[RS_TILE];
   __shared__ int32_t sv[RS_TILE];
   __shared__ uint64_t wtot[RS_THREADS / 64][4];
   const int64_t base = blockIdx.x * (int64_t)RS_TILE;
@@ -254,7 +255,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t* __res
   for (int k = 0; k < RS_ITEMS; ++k) {
     const int j = first + k;
     if (j >= cnt) break;
-    const uint64_t key = sk[j];
+    const K key = sk[j];
     const uint32_t d = (uint32_t)((key >> shift) & 15u);
     const int64_t pos = (int64_t)off[d * ntiles + blockIdx.x] + field16(ex, d) + field16(run, d);
     run[d >> 2] += 1ull << (16 * (d & 3));
@@ -263,24 +264,21 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t* __res
   }
 }
 
-// (keys, vals) sorted in place by the low `bits` bits of the keys; k2/v2
-// are scratch of the same size
-void radix_sort_pairs(uint64_t* keys, int32_t* vals, uint64_t* k2, int32_t* v2, int64_t n, int bits, hipStream_t s,
-                      std::vector<DevBuf>& tmp)
+template <class K>
+void radix_sort_impl(K* keys, int32_t* vals, K* k2, int32_t* v2, int64_t n, int bits, uint32_t* hist, hipStream_t s)
 {
   if (n <= 1) return;
   const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
-  auto* hist = (uint32_t*)dev_alloc(sizeof(uint32_t) * RS_BINS * ntiles, tmp);
   int passes = (bits + 3) / 4;
   if (passes & 1) ++passes;  // even: the result lands back in keys/vals
-  uint64_t *ka = keys, *kb = k2;
+  K *ka = keys, *kb = k2;
   int32_t *va = vals, *vb = v2;
   for (int p = 0; p < passes; ++p) {
     const int shift = 4 * p;
-    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, n, shift, hist, ntiles);
+    hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, n, shift, hist, ntiles);
     hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, hist, (int64_t)RS_BINS * ntiles, (uint32_t*)nullptr);
-    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n, shift, hist,
-                       ntiles);
+    hipLaunchKernelGGL(k_rs_scatter<K>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n, shift,
+                       hist, ntiles);
     std::swap(ka, kb);
     std::swap(va, vb);
   }
@@ -570,6 +568,20 @@ int validate_tree(const LbvhOut& out, int64_t n, hipStream_t s)
 
 }  // namespace
 
+int64_t radix_sort_hist_words(int64_t n) { return (int64_t)RS_BINS * std::max<int64_t>((n + RS_TILE - 1) / RS_TILE, 1); }
+
+void radix_sort_pairs(uint64_t* keys, int32_t* vals, uint64_t* k2, int32_t* v2, int64_t n, int bits, uint32_t* hist,
+                      hipStream_t s)
+{
+  radix_sort_impl(keys, vals, k2, v2, n, bits, hist, s);
+}
+
+void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* k2, int32_t* v2, int64_t n, int bits, uint32_t* hist,
+                      hipStream_t s)
+{
+  radix_sort_impl(keys, vals, k2, v2, n, bits, hist, s);
+}
+
 void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std::vector<DevBuf>& owned,
                 LbvhOut& out, bool keep_topology)
 {
@@ -601,7 +613,8 @@ void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std:
     auto* keys2 = (uint64_t*)dev_alloc(sizeof(uint64_t) * n, tmp);
     auto* vals2 = (int32_t*)dev_alloc(sizeof(int32_t) * n, tmp);
     hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(BLK), 0, s, box6, n, cb, keys, out.order);
-    radix_sort_pairs(keys, out.order, keys2, vals2, n, 63, s, tmp);
+    auto* hist = (uint32_t*)dev_alloc(sizeof(uint32_t) * (size_t)radix_sort_hist_words(n), tmp);
+    radix_sort_pairs(keys, out.order, keys2, vals2, n, 63, hist, s);
 
     unsigned long long hmax = 0;
     if (n == 1) {

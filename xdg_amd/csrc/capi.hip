@@ -574,13 +574,14 @@ int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
 int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
 int* opt_mw_wave(xdg_hip_ctx* c) { return &c->cfg.mw_wave; }
 int* opt_refit(xdg_hip_ctx* c) { return &c->refit; }
+int* opt_seg_order(xdg_hip_ctx* c) { return &c->cfg.seg_order; }
 
 const OptionRef OPTIONS[] = {
     {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 7, opt_persistent},
     {"ray_order", 0, 2, opt_order},         {"speculative", 0, 1, opt_spec}, {"wide", 0, 1, opt_wide},
     {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
     {"refill", 1, 64, opt_refill},          {"mw_wave", 0, 1, opt_mw_wave},
-    {"refit", 0, 1, opt_refit},
+    {"refit", 0, 1, opt_refit},            {"segment_order", 0, 2, opt_seg_order},
 };
 
 const OptionRef& find_option(const char* name)
@@ -806,6 +807,16 @@ int xdg_hip_build_element_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
       t.dev.depth = lb.depth;
       for (int f = 0; f < 12; ++f) t.dev.face_order[f] = face_order ? face_order[f] : 0;
       t.dev.ipc_tree = -1;
+      for (int k = 0; k < 3; ++k) {
+        double lo = 0.0, hi = 0.0;
+        for (int64_t v = 0; v < n_verts; ++v) {
+          const double x = verts[3 * v + k];
+          if (v == 0 || x < lo) lo = x;
+          if (v == 0 || x > hi) hi = x;
+        }
+        t.dev.box_lo[k] = (float)lo;
+        t.dev.box_hi[k] = (float)hi;
+      }
       t.dev.bface_elem = nullptr;
       t.n_nodes = lb.n_nodes;
       t.bytes = bufs_bytes(t.bufs);
@@ -1310,9 +1321,16 @@ int xdg_hip_tally_segments(xdg_hip_ctx* c, int32_t element_tree, int64_t n, cons
     a.seq_elem = seq_elem;
     a.seq_len = seq_len;
     a.status = c->d_status;
-    launch_segments(a, face_order_kind(t.dev.face_order), cfg_for(c, depth), pick(c, stream));
-    tally_guard(c, tally_fx, t.dev.n_prims, XDG_HIP_STATUS_TALLY_RANGE, pick(c, stream));
+    const hipStream_t s = pick(c, stream);
+    const bool ordered = c->cfg.seg_order && n >= ORDER_MIN_RAYS;
+    if (ordered) {
+      scratch_acquire(c, s);
+      a.order = launch_segment_order(a, c->cfg.seg_order, order_scratch(c, n), s);
+    }
+    launch_segments(a, face_order_kind(t.dev.face_order), cfg_for(c, depth), s);
+    tally_guard(c, tally_fx, t.dev.n_prims, XDG_HIP_STATUS_TALLY_RANGE, s);
     XDG_HIP_CHECK(hipGetLastError());
+    if (ordered) scratch_release(c, s);
     return XDG_HIP_OK;
   });
 }

@@ -414,6 +414,7 @@ __global__ void k_segments(SegmentsArgs a, int cap)
   extern __shared__ int32_t lds[];
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= a.n) return;
+  if (a.order) i = a.order[i];
   Sink s;
   s.count = 0;
   s.hash = HASH_SEED;
@@ -505,8 +506,9 @@ template <int FO, int OUT>
 __global__ void k_segments_vol(SegmentsArgs a, int cap)
 {
   extern __shared__ int32_t lds[];
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= a.n) return;
+  if (a.order) i = a.order[i];
   Sink s;
   s.count = 0;
   s.hash = HASH_SEED;
@@ -907,6 +909,64 @@ void launch_segments_t(const SegmentsArgs& a, const LaunchCfg& c, hipStream_t s)
     hipLaunchKernelGGL((k_segments_vol<FO, OUT>), grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
   else
     hipLaunchKernelGGL((k_segments<FO, OUT>), grid1(a.n, c.block), dim3(c.block), lds_bytes(c), s, a, c.stack);
+}
+
+namespace {
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v)  // 10 bits -> every third bit of 30
+{
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t q10(double x, float lo, float hi)
+{
+  const double w = (double)hi - (double)lo;
+  double q = w > 0.0 ? (x - (double)lo) / w * 1024.0 : 0.0;
+  q = q >= 0.0 ? (q > 1023.0 ? 1023.0 : q) : 0.0;  // NaN -> 0
+  return (uint32_t)q;
+}
+
+// mode 1: Morton code of the start point (10 bits per axis); mode 2: the
+// start cell and then the end cell (5 bits per axis each), so that
+// neighbouring segments share most of their path
+__global__ void k_segment_keys(SegmentsArgs a, int mode, uint32_t* keys, int32_t* idx)
+{
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  uint32_t m = 0;
+  if (a.tree >= 1 && a.tree <= a.nelem_trees) {
+    const ElemTreeDev& T = a.etable[a.tree];
+    const uint32_t qs[3] = {q10(a.p0[3 * i], T.box_lo[0], T.box_hi[0]), q10(a.p0[3 * i + 1], T.box_lo[1], T.box_hi[1]),
+                            q10(a.p0[3 * i + 2], T.box_lo[2], T.box_hi[2])};
+    if (mode == 2) {
+      const uint32_t qe[3] = {q10(a.p1[3 * i], T.box_lo[0], T.box_hi[0]),
+                              q10(a.p1[3 * i + 1], T.box_lo[1], T.box_hi[1]),
+                              q10(a.p1[3 * i + 2], T.box_lo[2], T.box_hi[2])};
+      const uint32_t ms = spread10(qs[0] >> 5) | (spread10(qs[1] >> 5) << 1) | (spread10(qs[2] >> 5) << 2);
+      const uint32_t me = spread10(qe[0] >> 5) | (spread10(qe[1] >> 5) << 1) | (spread10(qe[2] >> 5) << 2);
+      m = (ms << 15) | me;
+    } else {
+      m = spread10(qs[0]) | (spread10(qs[1]) << 1) | (spread10(qs[2]) << 2);
+    }
+  }
+  keys[i] = m;
+  idx[i] = (int32_t)i;
+}
+
+}  // namespace
+
+const int32_t* launch_segment_order(const SegmentsArgs& a, int mode, const RayOrderScratch& w, hipStream_t s)
+{
+  if (a.n <= 0) return nullptr;
+  hipLaunchKernelGGL(k_segment_keys, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a, mode, w.keys_in,
+                     w.idx_in);
+  radix_sort_pairs(w.keys_in, w.idx_in, w.keys_out, w.idx_out, a.n, 30, (uint32_t*)w.temp, s);
+  return w.idx_in;
 }
 
 void launch_segments(const SegmentsArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s)

@@ -58,6 +58,16 @@ struct LbvhOut {
   LbvhTopo topo;                   // kept only when a refit is wanted
 };
 
+// Stable LSD radix sort (4-bit digits, hand-written) of (key, int32 value)
+// pairs by the low `bits` bits of the keys.  The result lands in keys/vals;
+// k2/v2 are scratch of n entries and hist holds radix_sort_hist_words(n)
+// words.  Asynchronous on s; allocates nothing.
+int64_t radix_sort_hist_words(int64_t n);
+void radix_sort_pairs(uint64_t* keys, int32_t* vals, uint64_t* k2, int32_t* v2, int64_t n, int bits, uint32_t* hist,
+                      hipStream_t s);
+void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* k2, int32_t* v2, int64_t n, int bits, uint32_t* hist,
+                      hipStream_t s);
+
 // box6: device array of n boxes (lo.xyz, hi.xyz), already dilated.  The
 // whole build runs on the device: 63-bit Morton codes, a stable LSD radix
 // sort, the Karras hierarchy, a bottom-up refit and a structural check by
@@ -146,6 +156,7 @@ struct LaunchCfg {
   int spec = 1;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
   int refill = 40; // persistent ray_fire / find_volume: idle lanes of a wave needed before a refill
   int mw_wave = 1;  // mesh walk: batched re-entry rays (wavefront) instead of inline ones
+  int seg_order = 0;  // segment walks in Morton order of the start points
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
@@ -347,8 +358,16 @@ struct SegmentsArgs {
   // enters it through surface tree surf_tree (ENTERING)
   int volume_mode;
   int32_t surf_tree;
+  // nullable: slot j of the launch walks segment order[j] (outputs still
+  // land at the segment's own index)
+  const int32_t* order;
 };
 void launch_segments(const SegmentsArgs& a, int face_order_kind, const LaunchCfg& c, hipStream_t s);
+// Walk order of a segment batch (stable, so equal keys keep the caller's
+// order): mode 1 by the 30-bit Morton code of the start point inside the
+// element tree's bounds, mode 2 by start cell then end cell (15 bits each).
+// Returns the permutation (in w), valid on stream s.
+const int32_t* launch_segment_order(const SegmentsArgs& a, int mode, const RayOrderScratch& w, hipStream_t s);
 
 // Element-by-element particle walk (tools/walk_elements.h:53-117).
 struct MeshWalkArgs {

@@ -14,7 +14,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "internal.h"
 
@@ -75,13 +74,7 @@ __global__ void k_ray_keys(RayOrderArgs a, int tree_bits, int dir_bits, uint32_t
 
 }  // namespace
 
-size_t ray_order_temp_bytes(int64_t n)
-{
-  size_t tb = 0;
-  XDG_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
-                                          (int32_t*)nullptr, (unsigned)n, 0, 32, (hipStream_t)0));
-  return tb;
-}
+size_t ray_order_temp_bytes(int64_t n) { return sizeof(uint32_t) * (size_t)radix_sort_hist_words(n); }
 
 const int32_t* launch_ray_order(const RayOrderArgs& a, int mode, const RayOrderScratch& w, hipStream_t s)
 {
@@ -91,10 +84,9 @@ const int32_t* launch_ray_order(const RayOrderArgs& a, int mode, const RayOrderS
   const int dir_bits = mode >= 2 ? 1 : 0;
   const unsigned grid = (unsigned)((a.n + 255) / 256);
   hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(256), 0, s, a, tree_bits, dir_bits, w.keys_in, w.idx_in);
-  size_t tb = w.temp_bytes;
-  XDG_HIP_CHECK(rocprim::radix_sort_pairs(w.temp, tb, w.keys_in, w.keys_out, w.idx_in, w.idx_out, (unsigned)a.n, 0,
-                                          32, s));
-  return w.idx_out;
+  // stable sort: rays with equal keys keep the caller's order
+  radix_sort_pairs(w.keys_in, w.idx_in, w.keys_out, w.idx_out, a.n, 32, (uint32_t*)w.temp, s);
+  return w.idx_in;
 }
 
 }  // namespace xdgh

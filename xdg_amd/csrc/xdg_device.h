@@ -529,6 +529,8 @@ struct ElemTreeDev {
   // implicit-complement re-entry (XDG::segments, xdg.cpp:164-185)
   int32_t ipc_tree;          // surface tree id, -1 if none
   const int32_t* bface_elem; // boundary face id -> element
+  float box_lo[3];           // vertex bounds (walk ordering only)
+  float box_hi[3];
 };
 
 }  // namespace xdgd
