@@ -74,6 +74,11 @@ def parse():
     p.add_argument("--no-tet", action="store_true")
     p.add_argument("--no-transport", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--legs", default="",
+                   help="comma list of legs to run alone (ray_fire, sphere, tet, transport, adjoint), "
+                        "for profiling one workload at a time; prints a short JSON line")
+    p.add_argument("--transport-config", default="",
+                   help="block,lds_stack,chunk,refill for the transport leg instead of the autotuner")
     p.add_argument("--copy-probe", action="store_true",
                    help="also run a 256 MiB device copy (scale for PMC byte counters)")
     return p.parse_args()
@@ -383,9 +388,14 @@ def bench_transport(args, ctx, torch, stream, sh):
     nvol = len(vols)
     t0 = time.perf_counter()
     ns = min(n, 1 << 18)
-    tune_ms = ctx.transport_autotune(h, 0, ns, capi.MIXED)
-    cfg = ctx.transport_config(h)
-    log(f"autotune {time.perf_counter() - t0:.1f}s -> {cfg} ({tune_ms:.3f} ms / {ns} histories)")
+    if args.transport_config:
+        ctx.transport_set_config(h, *[int(x) for x in args.transport_config.split(",")])
+        cfg = ctx.transport_config(h)
+        log(f"transport config {cfg} (given)")
+    else:
+        tune_ms = ctx.transport_autotune(h, 0, ns, capi.MIXED)
+        cfg = ctx.transport_config(h)
+        log(f"autotune {time.perf_counter() - t0:.1f}s -> {cfg} ({tune_ms:.3f} ms / {ns} histories)")
     out = ctx.transport_outputs(nvol, n)
     g = ctx.transport_capture(h, 0, n, out, capi.MIXED)
     try:
@@ -609,6 +619,17 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+
+    if args.legs:
+        legs = {"ray_fire": bench_ray_fire, "sphere": bench_sphere, "tet": bench_tet,
+                "transport": bench_transport, "adjoint": bench_adjoint}
+        res = {}
+        for name in args.legs.split(","):
+            r = legs[name](args, ctx, torch, stream, sh)
+            res[name] = {k: v for k, v in r.items() if not k.startswith("_") and k not in ("model", "pts", "dirs")}
+        print(json.dumps({"legs": res}, default=str), flush=True)
+        ctx.close()
+        return
 
     hl = bench_ray_fire(args, ctx, torch, stream, sh)
     sph = None if args.no_sphere else bench_sphere(args, ctx, torch, stream, sh)

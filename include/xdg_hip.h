@@ -126,6 +126,16 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *                 xdg_hip_refit_surface_tree can move their vertices (0)
  *   "segment_order" segment batches of >= 65536: walked in the Morton order
  *                 of their start points (1) or in the caller's order (0)
+ *   "transport_start" transport starts (sampling, find_volume): inside the
+ *                 event kernel (0), as separate kernels in history order
+ *                 (1), or as separate kernels on histories sorted by start
+ *                 cell and direction, with find_volume as point_in_volume
+ *                 items grouped by volume (2, default)
+ *   "transport_slice" staged starts run in slices of 2^value histories
+ *                 (6..24, default 22; about 600 MB of start state per slice)
+ *   "transport_items" start-stage item buffer entries per history (1..64,
+ *                 default 8); histories whose items do not fit run the
+ *                 volume loop themselves
  * Unknown names give XDG_HIP_ERR_NOT_FOUND, out-of-range values
  * XDG_HIP_ERR_INVALID. */
 int xdg_hip_set_option(xdg_hip_ctx* ctx, const char* name, int value);
@@ -423,9 +433,11 @@ int xdg_hip_transport_run(xdg_hip_ctx* ctx, int32_t handle, uint64_t first, int6
  * xdg_hip_destroy).  The graph holds device pointers of the capture time:
  * the output arrays (kept alive by the caller), the tree table and trees,
  * and the transport's buffers.  Building, releasing or attaching any tree
- * on the context, destroying the transport, or a launch shape that needs a
+ * on the context, destroying the transport, or a launch that needs a
  * larger stack spill area (transport_set_config / autotune followed by a
- * run) invalidates it: xdg_hip_graph_launch then returns
+ * run) or a larger start buffer (more histories per slice, or the
+ * "transport_slice" / "transport_items" options raised, followed by a run)
+ * invalidates it: xdg_hip_graph_launch then returns
  * XDG_HIP_ERR_INVALID without launching; capture again. */
 int xdg_hip_transport_capture(xdg_hip_ctx* ctx, int32_t handle, uint64_t first, int64_t n, int precision,
                               uint64_t* tally_fx, int32_t* events, uint8_t* status, uint64_t* hash, void** graph);

@@ -150,6 +150,87 @@ def test_transport_graph_replays(ctx, torch_mod, lattice):
         ctx.transport_destroy(h)
 
 
+@pytest.fixture
+def start_form(ctx):
+    """Restores the transport start options after a test changes them."""
+    names = ("transport_start", "transport_slice", "transport_items")
+    keep = [ctx.get_option(k) for k in names]
+    yield
+    for k, v in zip(names, keep):
+        ctx.set_option(k, v)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_transport_start_forms(ctx, lattice, prec, start_form):
+    """Starts sampled inside the event kernel, or by the separate start
+    kernels (in index order, or sorted by the start's cell and direction)
+    over slices of 2^6, 2^10 and 2^22 histories (one slice, many slices, a
+    ragged last slice): every history and the tallies equal the oracle's."""
+    h, ref = _problem(ctx, lattice)
+    try:
+        n, first = 5000, 123
+        r = ref(first, n)
+        for staged, slice_log2 in [(0, 22), (1, 6), (1, 22), (2, 6), (2, 10), (2, 22)]:
+            ctx.set_option("transport_start", staged)
+            ctx.set_option("transport_slice", slice_log2)
+            out = ctx.transport_run(h, first, n, len(lattice[1]), precision=prec)
+            _check(out, r)
+    finally:
+        ctx.transport_destroy(h)
+
+
+def test_transport_start_item_overflow(ctx, lattice, start_form):
+    """With one start-stage item per history the point_in_volume items of a
+    slice overflow the buffer; the histories that lost items find their
+    volume in the resolve step instead, and every history still equals the
+    oracle's."""
+    h, ref = _problem(ctx, lattice)
+    try:
+        n, first = 4000, 9
+        r = ref(first, n)
+        for items in (1, 2):
+            ctx.set_option("transport_start", 2)
+            ctx.set_option("transport_items", items)
+            _check(ctx.transport_run(h, first, n, len(lattice[1])), r)
+    finally:
+        ctx.transport_destroy(h)
+
+
+def test_transport_sliced_graph_replays(ctx, torch_mod, lattice, start_form):
+    """A captured staged launch over 12 slices replays to the eager result;
+    a larger slice afterwards moves the start buffer, which makes the graph
+    stale (refused before any launch)."""
+    from xdg_amd import capi
+    torch = torch_mod
+    h, ref = _problem(ctx, lattice)
+    try:
+        ctx.set_option("transport_start", 2)
+        ctx.set_option("transport_slice", 9)
+        n, first = 6000, 5
+        nvol = len(lattice[1])
+        eager = ctx.transport_run(h, first, n, nvol)
+        out = ctx.transport_outputs(nvol, n)
+        g = ctx.transport_capture(h, first, n, out)
+        try:
+            for rep in range(3):
+                for v in out.values():
+                    v.fill_(3)
+                torch.cuda.synchronize()
+                ctx.graph_launch(g)
+                ctx.synchronize()
+                for k in eager:
+                    assert torch.equal(out[k], eager[k]), (rep, k)
+            ctx.set_option("transport_slice", 12)
+            ctx.transport_run(h, first, n, nvol)
+            with pytest.raises(capi.XdgHipError, match="stale"):
+                ctx.graph_launch(g)
+        finally:
+            ctx.graph_destroy(g)
+        _check(eager, ref(first, n))
+    finally:
+        ctx.transport_destroy(h)
+
+
 def test_transport_event_cap(ctx, lattice):
     h, ref = _problem(ctx, lattice, mfp=0.05, p_absorb=0.0, max_events=7)
     try:

@@ -582,6 +582,11 @@ void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* k2, int32_t* v2, 
   radix_sort_impl(keys, vals, k2, v2, n, bits, hist, s);
 }
 
+void launch_scan_excl(uint32_t* a, int64_t m, uint32_t* total, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, a, m, total);
+}
+
 void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std::vector<DevBuf>& owned,
                 LbvhOut& out, bool keep_topology)
 {

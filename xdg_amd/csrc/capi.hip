@@ -47,8 +47,12 @@ struct TransportHost {
   int32_t* spill = nullptr;
   size_t spill_entries = 0;
   unsigned long long* d_next = nullptr;
+  void* stage = nullptr;  // start state of one slice (staged form)
+  int64_t stage_cap = 0;
+  int stage_items = 0;
+  int64_t start_grid[2] = {0, 0};  // plain, mixed: resident grid of the start stage's PIV kernel
   std::vector<int32_t> host_trees;
-  uint64_t generation = 1;  // bumped when the spill area moves
+  uint64_t generation = 1;  // bumped when the spill area or the start buffer moves
 };
 
 // An instantiated transport graph and what it was captured against: a
@@ -511,6 +515,7 @@ int xdg_hip_destroy(xdg_hip_ctx* c)
     for (auto* t : c->transports) {
       if (!t) continue;
       if (t->spill) (void)hipFree(t->spill);
+      if (t->stage) (void)hipFree(t->stage);
       dev_free_all(t->bufs);
       delete t;
     }
@@ -575,6 +580,9 @@ int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
 int* opt_mw_wave(xdg_hip_ctx* c) { return &c->cfg.mw_wave; }
 int* opt_refit(xdg_hip_ctx* c) { return &c->refit; }
 int* opt_seg_order(xdg_hip_ctx* c) { return &c->cfg.seg_order; }
+int* opt_tr_start(xdg_hip_ctx* c) { return &c->cfg.tr_start; }
+int* opt_tr_slice(xdg_hip_ctx* c) { return &c->cfg.tr_slice; }
+int* opt_tr_items(xdg_hip_ctx* c) { return &c->cfg.tr_items; }
 
 const OptionRef OPTIONS[] = {
     {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 7, opt_persistent},
@@ -582,6 +590,8 @@ const OptionRef OPTIONS[] = {
     {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
     {"refill", 1, 64, opt_refill},          {"mw_wave", 0, 1, opt_mw_wave},
     {"refit", 0, 1, opt_refit},            {"segment_order", 0, 2, opt_seg_order},
+    {"transport_start", 0, 2, opt_tr_start}, {"transport_slice", 6, 24, opt_tr_slice},
+    {"transport_items", 1, 64, opt_tr_items},
 };
 
 const OptionRef& find_option(const char* name)
@@ -1523,8 +1533,27 @@ void transport_refresh(xdg_hip_ctx* c, TransportHost& t)
   if (t.lds_stack <= 0 || t.lds_stack > need) t.lds_stack = need;
 }
 
-// global stack share for the shape (outside any capture: may allocate)
-TransportLaunch transport_prepare(xdg_hip_ctx* c, TransportHost& t, bool mixed, bool may_alloc)
+size_t transport_stage_bytes(int64_t cap, int items)
+{
+  return (size_t)cap * (TRANSPORT_STAGE_BYTES + 8 * (size_t)items) + 4 * (size_t)radix_sort_hist_words(cap) + 256;
+}
+
+// histories per start slice: 2^transport_slice, and few enough that the
+// 32-bit item counts cannot wrap (one item per history and volume at most)
+int64_t transport_slice_cap(xdg_hip_ctx* c, const TransportHost& t)
+{
+  const int64_t by_count = (int64_t)(UINT32_MAX / (uint64_t)std::max(t.dev.nvol, 1));
+  return std::max<int64_t>(1, std::min<int64_t>((int64_t)1 << c->cfg.tr_slice, by_count));
+}
+
+bool transport_staged(xdg_hip_ctx* c, const TransportHost& t)
+{
+  return c->cfg.tr_start != 0 && transport_start_block(t.stack_need) > 0;
+}
+
+// global stack share for the shape, and the start buffer of the staged form
+// for n histories (outside any capture: may allocate)
+TransportLaunch transport_prepare(xdg_hip_ctx* c, TransportHost& t, bool mixed, bool may_alloc, int64_t n)
 {
   transport_refresh(c, t);
   const TransportLaunch L = transport_shape(mixed, t.block, t.lds_stack, t.dev.nvol);
@@ -1539,6 +1568,22 @@ TransportLaunch transport_prepare(xdg_hip_ctx* c, TransportHost& t, bool mixed, 
     t.spill_entries = need;
     ++t.generation;
   }
+  const int64_t cap = std::min<int64_t>(n, transport_slice_cap(c, t));
+  if (transport_staged(c, t) && (cap > t.stage_cap || c->cfg.tr_items > t.stage_items)) {
+    if (!may_alloc) throw InvalidArg{"transport launch needs a larger start buffer; run it once before capture"};
+    const int64_t nc = std::max(cap, t.stage_cap);
+    const int ni = std::max(c->cfg.tr_items, t.stage_items);
+    XDG_HIP_CHECK(hipDeviceSynchronize());
+    if (t.stage) XDG_HIP_CHECK(hipFree(t.stage));
+    t.stage = nullptr;
+    t.stage_cap = 0;
+    t.stage_items = 0;
+    XDG_HIP_CHECK(hipMalloc(&t.stage, transport_stage_bytes(nc, ni)));
+    t.stage_cap = nc;
+    t.stage_items = ni;
+    ++t.generation;
+  }
+  if (transport_staged(c, t)) t.start_grid[mixed ? 1 : 0] = transport_start_grid(mixed, t.stack_need);
   return L;
 }
 
@@ -1560,7 +1605,28 @@ void transport_launch(xdg_hip_ctx* c, TransportHost& t, const TransportLaunch& L
   R.spill_lanes = L.grid * L.block;
   R.lds_tally = t.dev.nvol <= TRANSPORT_LDS_TALLY_MAX ? 1 : 0;
   R.refill = t.refill;
-  (void)c;
+  R.staged = transport_staged(c, t) && t.stage_cap > 0 ? c->cfg.tr_start : 0;
+  if (R.staged) {
+    // component-major doubles first, then the 4-byte arrays, then bytes;
+    // slices of at most 2^transport_slice histories (the layout follows the
+    // slice, inside the buffer sized for stage_cap >= slice)
+    const int64_t cap = std::min<int64_t>(t.stage_cap, transport_slice_cap(c, t));
+    R.stage.cap = cap;
+    R.stage.ru = (double*)t.stage;
+    R.stage.items = (int2*)(R.stage.ru + 6 * cap);
+    R.stage.item_cap = (int64_t)std::min(c->cfg.tr_items, t.stage_items) * cap;
+    R.stage.vol = (int32_t*)(R.stage.items + R.stage.item_cap);
+    R.stage.dc = (uint32_t*)(R.stage.vol + cap);
+    R.stage.key = R.stage.dc + cap;
+    R.stage.idx = (int32_t*)(R.stage.key + cap);
+    R.stage.k2 = (uint32_t*)(R.stage.idx + cap);
+    R.stage.v2 = (int32_t*)(R.stage.k2 + cap);
+    R.stage.found = R.stage.v2 + cap;
+    R.stage.hist = (uint32_t*)(R.stage.found + cap);
+    R.stage.st = (uint8_t*)(R.stage.hist + radix_sort_hist_words(cap));
+    R.stage.over = R.stage.st + cap;
+    R.start_grid = t.start_grid[mixed ? 1 : 0];
+  }
   launch_transport(t.dev, R, mixed, L, s);
 }
 
@@ -1627,6 +1693,9 @@ int xdg_hip_transport_create(xdg_hip_ctx* c, const xdg_hip_transport_problem* p,
         t->dev.surf_bc = d;
       }
       t->d_next = (unsigned long long*)dev_alloc(64, t->bufs);
+      t->dev.pre_box = (double*)dev_alloc((size_t)p->nvol * 6 * sizeof(double), t->bufs);
+      t->dev.slot_cnt = (uint32_t*)dev_alloc(((size_t)p->nvol + 1) * sizeof(uint32_t), t->bufs);
+      t->dev.slot_cur = (uint32_t*)dev_alloc((size_t)p->nvol * sizeof(uint32_t), t->bufs);
       t->host_trees.assign(p->vol_trees, p->vol_trees + p->nvol);
       t->block = c->cfg.block > 256 ? 256 : c->cfg.block;
       transport_refresh(c, *t);
@@ -1647,6 +1716,7 @@ int xdg_hip_transport_destroy(xdg_hip_ctx* c, int32_t handle)
     TransportHost& t = transport_of(c, handle);
     XDG_HIP_CHECK(hipDeviceSynchronize());
     if (t.spill) XDG_HIP_CHECK(hipFree(t.spill));
+    if (t.stage) XDG_HIP_CHECK(hipFree(t.stage));
     dev_free_all(t.bufs);
     delete &t;
     c->transports[handle - 1] = nullptr;
@@ -1698,7 +1768,7 @@ int xdg_hip_transport_run(xdg_hip_ctx* c, int32_t handle, uint64_t first, int64_
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     XDG_HIP_CHECK(hipStreamIsCapturing(s, &cs));
     const bool mixed = precision == XDG_HIP_MIXED;
-    const TransportLaunch L = transport_prepare(c, t, mixed, cs == hipStreamCaptureStatusNone);
+    const TransportLaunch L = transport_prepare(c, t, mixed, cs == hipStreamCaptureStatusNone, n);
     transport_launch(c, t, L, mixed, first, n, tally_fx, events, status, hash, s);
     tally_guard(c, tally_fx, t.dev.nvol, XDG_HIP_STATUS_TALLY_RANGE, s);
     XDG_HIP_CHECK(hipGetLastError());
@@ -1715,7 +1785,7 @@ int xdg_hip_transport_capture(xdg_hip_ctx* c, int32_t handle, uint64_t first, in
     require(precision == XDG_HIP_PLAIN || precision == XDG_HIP_MIXED, "unknown precision");
     require(tally_fx != nullptr && graph != nullptr, "null argument");
     const bool mixed = precision == XDG_HIP_MIXED;
-    const TransportLaunch L = transport_prepare(c, t, mixed, true);
+    const TransportLaunch L = transport_prepare(c, t, mixed, true, n);
     XDG_HIP_CHECK(hipStreamSynchronize(c->stream));
     hipGraph_t g = nullptr;
     XDG_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1749,7 +1819,7 @@ int xdg_hip_graph_launch(xdg_hip_ctx* c, void* graph, void* stream)
     require(g->transport >= 1 && (size_t)g->transport <= c->transports.size() && c->transports[g->transport - 1],
             "stale graph: its transport problem was destroyed");
     require(c->transports[g->transport - 1]->generation == g->transport_generation,
-            "stale graph: the transport's stack spill area moved since the capture; capture it again");
+            "stale graph: the transport's stack spill area or start buffer moved since the capture; capture it again");
     XDG_HIP_CHECK(hipGraphLaunch(g->exec, pick(c, stream)));
     return XDG_HIP_OK;
   });
@@ -1799,7 +1869,7 @@ int xdg_hip_transport_autotune(xdg_hip_ctx* c, int32_t handle, uint64_t first, i
             t.refill = refill;
             TransportLaunch L;
             try {
-              L = transport_prepare(c, t, mixed, true);
+              L = transport_prepare(c, t, mixed, true, sample);
             } catch (const HipError&) {
               continue;  // shape does not fit a CU
             }
@@ -1838,7 +1908,7 @@ int xdg_hip_transport_autotune(xdg_hip_ctx* c, int32_t handle, uint64_t first, i
     t.lds_stack = bs;
     t.chunk = bc;
     t.refill = br;
-    transport_prepare(c, t, mixed, true);
+    transport_prepare(c, t, mixed, true, sample);
     if (best_ms) *best_ms = best;
     return XDG_HIP_OK;
   });

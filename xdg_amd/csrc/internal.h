@@ -67,6 +67,9 @@ void radix_sort_pairs(uint64_t* keys, int32_t* vals, uint64_t* k2, int32_t* v2, 
                       hipStream_t s);
 void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* k2, int32_t* v2, int64_t n, int bits, uint32_t* hist,
                       hipStream_t s);
+// exclusive scan of a[0..m) in place by one workgroup; the sum goes to
+// *total (nullable).  Asynchronous on s.
+void launch_scan_excl(uint32_t* a, int64_t m, uint32_t* total, hipStream_t s);
 
 // box6: device array of n boxes (lo.xyz, hi.xyz), already dilated.  The
 // whole build runs on the device: 63-bit Morton codes, a stable LSD radix
@@ -157,6 +160,9 @@ struct LaunchCfg {
   int refill = 40; // persistent ray_fire / find_volume: idle lanes of a wave needed before a refill
   int mw_wave = 1;  // mesh walk: batched re-entry rays (wavefront) instead of inline ones
   int seg_order = 0;  // segment walks in Morton order of the start points
+  int tr_start = 2;   // transport starts: 0 in the event kernel, 1 separate kernels, 2 separate and sorted
+  int tr_slice = 22;  // transport, staged form: log2 of the histories per slice (about 600 MB of state at 22)
+  int tr_items = 8;   // transport, staged form: start-stage item buffer entries per history
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
@@ -266,7 +272,31 @@ struct TransportDev {
   double mfp, p_absorb;
   int32_t max_events;
   const uint8_t* surf_bc;   // nullable: boundary condition per surface id [0, nsurf)
+  double* pre_box;          // nvol x (lo.xyz, hi.xyz): start-stage prefilter boxes, written per launch
+  uint32_t* slot_cnt;       // nvol + 1: start-stage items per volume slot, then their offsets (+ total)
+  uint32_t* slot_cur;       // nvol: scatter cursors
 };
+// Start state of one slice of histories, written by the start stage and
+// read by the event kernel (component-major: x, y, z, ux, uy, uz blocks of
+// cap doubles each), and the order in which both run them.
+struct TransportStage {
+  double* ru;
+  int32_t* vol;
+  uint32_t* dc;   // Philox counter after the start draws
+  uint8_t* st;
+  uint32_t* key;  // coherence key of the start
+  int32_t* idx;   // processing order: slice-relative history indices
+  uint32_t* k2;   // sort scratch
+  int32_t* v2;
+  uint32_t* hist;  // radix_sort_hist_words(cap) words
+  int32_t* found;  // lowest volume slot found to contain the start (INT32_MAX: none yet)
+  uint8_t* over;   // the history's candidate items did not fit the item buffer
+  int2* items;     // (history, volume slot) point_in_volume items, grouped by slot
+  int64_t item_cap;
+  int64_t cap;    // histories per slice
+};
+// per history, beside the sort histogram and the 8-byte items
+constexpr size_t TRANSPORT_STAGE_BYTES = 6 * 8 + 7 * 4 + 2;
 // One launch over histories [first, first + n).
 struct TransportRun {
   uint64_t first;
@@ -283,6 +313,9 @@ struct TransportRun {
   int64_t spill_lanes;
   int lds_tally;                 // per-workgroup LDS tally (nvol small enough)
   int refill;                    // idle lanes a wave waits for before starting histories
+  int staged;                    // starts as separate kernels per slice of stage.cap histories (2: sorted)
+  TransportStage stage;
+  int64_t start_grid;            // resident workgroups of the start stage's point_in_volume kernel
 };
 struct TransportLaunch {
   int block;
@@ -295,6 +328,11 @@ TransportLaunch transport_shape(bool mixed, int block, int lds_stack, int nvol);
 void launch_transport(const TransportDev& P, const TransportRun& R, bool mixed, const TransportLaunch& L,
                       hipStream_t s);
 constexpr int TRANSPORT_LDS_TALLY_MAX = 4096;
+// start stage: its traversal stack lives wholly in LDS, at most this much
+// per workgroup; 0 = no block fits the stack
+constexpr size_t TRANSPORT_START_LDS_MAX = 64 * 1024;
+int transport_start_block(int stack_cap);
+int64_t transport_start_grid(bool mixed, int stack_cap);
 
 // ---- adjoint of the ray distance (adjoint.hip) ----------------------------
 struct AdjointArgs {
