@@ -136,6 +136,10 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *   "transport_items" start-stage item buffer entries per history (1..64,
  *                 default 8); histories whose items do not fit run the
  *                 volume loop themselves
+ *   "transport_rounds" staged form: one-event rounds over the live
+ *                 histories, sorted by volume, cell and direction before each
+ *                 round, before the event kernel takes the rest (0..64,
+ *                 default 0)
  * Unknown names give XDG_HIP_ERR_NOT_FOUND, out-of-range values
  * XDG_HIP_ERR_INVALID. */
 int xdg_hip_set_option(xdg_hip_ctx* ctx, const char* name, int value);

@@ -153,7 +153,7 @@ def test_transport_graph_replays(ctx, torch_mod, lattice):
 @pytest.fixture
 def start_form(ctx):
     """Restores the transport start options after a test changes them."""
-    names = ("transport_start", "transport_slice", "transport_items")
+    names = ("transport_start", "transport_slice", "transport_items", "transport_rounds")
     keep = [ctx.get_option(k) for k in names]
     yield
     for k, v in zip(names, keep):
@@ -175,6 +175,24 @@ def test_transport_start_forms(ctx, lattice, prec, start_form):
             ctx.set_option("transport_slice", slice_log2)
             out = ctx.transport_run(h, first, n, len(lattice[1]), precision=prec)
             _check(out, r)
+    finally:
+        ctx.transport_destroy(h)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_transport_event_rounds(ctx, lattice, prec, start_form):
+    """Sorted one-event rounds before the event kernel (none, one, a few, and
+    more rounds than any history has events), also over several slices:
+    every history and the tallies equal the oracle's."""
+    h, ref = _problem(ctx, lattice)
+    try:
+        n, first = 5000, 31
+        r = ref(first, n)
+        ctx.set_option("transport_start", 2)
+        for rounds, slice_log2 in [(0, 22), (1, 22), (3, 22), (64, 22), (3, 7)]:
+            ctx.set_option("transport_rounds", rounds)
+            ctx.set_option("transport_slice", slice_log2)
+            _check(ctx.transport_run(h, first, n, len(lattice[1]), precision=prec), r)
     finally:
         ctx.transport_destroy(h)
 

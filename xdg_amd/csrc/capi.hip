@@ -47,6 +47,7 @@ struct TransportHost {
   int32_t* spill = nullptr;
   size_t spill_entries = 0;
   unsigned long long* d_next = nullptr;
+  uint32_t* d_live = nullptr;
   void* stage = nullptr;  // start state of one slice (staged form)
   int64_t stage_cap = 0;
   int stage_items = 0;
@@ -583,6 +584,7 @@ int* opt_seg_order(xdg_hip_ctx* c) { return &c->cfg.seg_order; }
 int* opt_tr_start(xdg_hip_ctx* c) { return &c->cfg.tr_start; }
 int* opt_tr_slice(xdg_hip_ctx* c) { return &c->cfg.tr_slice; }
 int* opt_tr_items(xdg_hip_ctx* c) { return &c->cfg.tr_items; }
+int* opt_tr_rounds(xdg_hip_ctx* c) { return &c->cfg.tr_rounds; }
 
 const OptionRef OPTIONS[] = {
     {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 7, opt_persistent},
@@ -591,7 +593,7 @@ const OptionRef OPTIONS[] = {
     {"refill", 1, 64, opt_refill},          {"mw_wave", 0, 1, opt_mw_wave},
     {"refit", 0, 1, opt_refit},            {"segment_order", 0, 2, opt_seg_order},
     {"transport_start", 0, 2, opt_tr_start}, {"transport_slice", 6, 24, opt_tr_slice},
-    {"transport_items", 1, 64, opt_tr_items},
+    {"transport_items", 1, 64, opt_tr_items}, {"transport_rounds", 0, 64, opt_tr_rounds},
 };
 
 const OptionRef& find_option(const char* name)
@@ -1533,9 +1535,16 @@ void transport_refresh(xdg_hip_ctx* c, TransportHost& t)
   if (t.lds_stack <= 0 || t.lds_stack > need) t.lds_stack = need;
 }
 
-size_t transport_stage_bytes(int64_t cap, int items)
+int transport_mask_words(const TransportHost& t)
 {
-  return (size_t)cap * (TRANSPORT_STAGE_BYTES + 8 * (size_t)items) + 4 * (size_t)radix_sort_hist_words(cap) + 256;
+  const int w = (t.dev.nvol + 63) / 64;
+  return w <= TRANSPORT_MASK_WORDS_MAX ? w : 0;
+}
+
+size_t transport_stage_bytes(int64_t cap, int items, int mask_words)
+{
+  return (size_t)cap * (TRANSPORT_STAGE_BYTES + 8 * (size_t)items + 8 * (size_t)mask_words) +
+         4 * (size_t)radix_sort_hist_words(cap) + 256;
 }
 
 // histories per start slice: 2^transport_slice, and few enough that the
@@ -1578,7 +1587,7 @@ TransportLaunch transport_prepare(xdg_hip_ctx* c, TransportHost& t, bool mixed, 
     t.stage = nullptr;
     t.stage_cap = 0;
     t.stage_items = 0;
-    XDG_HIP_CHECK(hipMalloc(&t.stage, transport_stage_bytes(nc, ni)));
+    XDG_HIP_CHECK(hipMalloc(&t.stage, transport_stage_bytes(nc, ni, transport_mask_words(t))));
     t.stage_cap = nc;
     t.stage_items = ni;
     ++t.generation;
@@ -1613,18 +1622,26 @@ void transport_launch(xdg_hip_ctx* c, TransportHost& t, const TransportLaunch& L
     const int64_t cap = std::min<int64_t>(t.stage_cap, transport_slice_cap(c, t));
     R.stage.cap = cap;
     R.stage.ru = (double*)t.stage;
-    R.stage.items = (int2*)(R.stage.ru + 6 * cap);
+    R.stage.hh = (unsigned long long*)(R.stage.ru + 6 * cap);
+    R.stage.items = (int2*)(R.stage.hh + cap);
     R.stage.item_cap = (int64_t)std::min(c->cfg.tr_items, t.stage_items) * cap;
-    R.stage.vol = (int32_t*)(R.stage.items + R.stage.item_cap);
+    R.stage.mask_words = transport_mask_words(t);
+    R.stage.masks = (unsigned long long*)(R.stage.items + R.stage.item_cap);
+    R.stage.vol = (int32_t*)(R.stage.masks + (int64_t)R.stage.mask_words * cap);
     R.stage.dc = (uint32_t*)(R.stage.vol + cap);
     R.stage.key = R.stage.dc + cap;
     R.stage.idx = (int32_t*)(R.stage.key + cap);
     R.stage.k2 = (uint32_t*)(R.stage.idx + cap);
     R.stage.v2 = (int32_t*)(R.stage.k2 + cap);
     R.stage.found = R.stage.v2 + cap;
-    R.stage.hist = (uint32_t*)(R.stage.found + cap);
+    R.stage.ev = R.stage.found + cap;
+    R.stage.hist8 = R.stage.ev + cap;
+    R.stage.hist = (uint32_t*)(R.stage.hist8 + 8 * cap);
     R.stage.st = (uint8_t*)(R.stage.hist + radix_sort_hist_words(cap));
     R.stage.over = R.stage.st + cap;
+    R.stage.hn = R.stage.over + cap;
+    R.stage.live = t.d_live;
+    R.rounds = c->cfg.tr_rounds;
     R.start_grid = t.start_grid[mixed ? 1 : 0];
   }
   launch_transport(t.dev, R, mixed, L, s);
@@ -1693,6 +1710,7 @@ int xdg_hip_transport_create(xdg_hip_ctx* c, const xdg_hip_transport_problem* p,
         t->dev.surf_bc = d;
       }
       t->d_next = (unsigned long long*)dev_alloc(64, t->bufs);
+      t->d_live = (uint32_t*)dev_alloc(64, t->bufs);
       t->dev.pre_box = (double*)dev_alloc((size_t)p->nvol * 6 * sizeof(double), t->bufs);
       t->dev.slot_cnt = (uint32_t*)dev_alloc(((size_t)p->nvol + 1) * sizeof(uint32_t), t->bufs);
       t->dev.slot_cur = (uint32_t*)dev_alloc((size_t)p->nvol * sizeof(uint32_t), t->bufs);

@@ -163,6 +163,7 @@ struct LaunchCfg {
   int tr_start = 2;   // transport starts: 0 in the event kernel, 1 separate kernels, 2 separate and sorted
   int tr_slice = 22;  // transport, staged form: log2 of the histories per slice (about 600 MB of state at 22)
   int tr_items = 8;   // transport, staged form: start-stage item buffer entries per history
+  int tr_rounds = 0;  // transport, staged form: sorted one-event rounds before the event kernel takes the rest
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
@@ -293,10 +294,19 @@ struct TransportStage {
   uint8_t* over;   // the history's candidate items did not fit the item buffer
   int2* items;     // (history, volume slot) point_in_volume items, grouped by slot
   int64_t item_cap;
+  unsigned long long* masks;  // candidate masks, mask_words x cap (chunk-major, by list position)
+  int mask_words;             // chunks of 64 volume slots, or 0 when the masks are not kept
+  // event state between rounds
+  int32_t* ev;
+  unsigned long long* hh;
+  uint8_t* hn;
+  int32_t* hist8;  // exclusion history, 8 x cap (entry-major)
+  uint32_t* live;  // live histories at the head of idx (one word)
   int64_t cap;    // histories per slice
 };
-// per history, beside the sort histogram and the 8-byte items
-constexpr size_t TRANSPORT_STAGE_BYTES = 6 * 8 + 7 * 4 + 2;
+// per history, beside the sort histogram and the 8-byte items: position and
+// direction, hash, 8 four-byte arrays, the exclusion history, 3 bytes
+constexpr size_t TRANSPORT_STAGE_BYTES = 6 * 8 + 8 + 8 * 4 + 8 * 4 + 3;
 // One launch over histories [first, first + n).
 struct TransportRun {
   uint64_t first;
@@ -316,6 +326,7 @@ struct TransportRun {
   int staged;                    // starts as separate kernels per slice of stage.cap histories (2: sorted)
   TransportStage stage;
   int64_t start_grid;            // resident workgroups of the start stage's point_in_volume kernel
+  int rounds;                    // staged form: sorted one-event rounds before the event kernel
 };
 struct TransportLaunch {
   int block;
@@ -332,6 +343,7 @@ constexpr int TRANSPORT_LDS_TALLY_MAX = 4096;
 // per workgroup; 0 = no block fits the stack
 constexpr size_t TRANSPORT_START_LDS_MAX = 64 * 1024;
 int transport_start_block(int stack_cap);
+constexpr int TRANSPORT_MASK_WORDS_MAX = 16;  // candidate masks kept for up to 1024 volume slots
 int64_t transport_start_grid(bool mixed, int stack_cap);
 
 // ---- adjoint of the ray distance (adjoint.hip) ----------------------------

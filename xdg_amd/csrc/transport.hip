@@ -270,7 +270,9 @@ __device__ __forceinline__ void load_start(const TransportStage& G, int64_t i, D
 // stages the chunk's prefilter boxes in LDS and each lane tests its ray
 // against all of them (broadcast reads).  Per volume of the chunk that any
 // lane's ray reaches, one lane of the wave counts (SCATTER false) or
-// reserves (SCATTER true) the wave's items for it.
+// reserves (SCATTER true) the wave's items for it.  When the stage keeps
+// candidate masks (few enough chunks), the counting pass saves each lane's
+// masks and the scatter pass reads them instead of testing again.
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void k_transport_cand(TransportDev P, TransportRun R)
 {
@@ -289,20 +291,26 @@ __global__ __launch_bounds__(256) void k_transport_cand(TransportDev P, Transpor
   }
   const D3 inv = d3(u.x != 0.0 ? 1.0 / u.x : 0.0, u.y != 0.0 ? 1.0 / u.y : 0.0, u.z != 0.0 ? 1.0 / u.z : 0.0);
   if (__ballot(open) == 0) return;  // wave-uniform
+  const bool keep = G.mask_words > 0;
   for (int v0 = 0; v0 < P.nvol; v0 += 64) {
     const int cn = P.nvol - v0 < 64 ? P.nvol - v0 : 64;
-    __builtin_amdgcn_wave_barrier();
-    if (lane < cn) {
-#pragma unroll
-      for (int k = 0; k < 6; ++k) wbox[6 * lane + k] = P.pre_box[6 * (int64_t)(v0 + lane) + k];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     uint64_t m = 0;
-    if (open)
-      for (int q = 0; q < cn; ++q)
-        if (may_hit_box(wbox + 6 * q, r, u, inv)) m |= 1ull << q;
+    if (SCATTER && keep) {
+      if (open) m = G.masks[(int64_t)(v0 >> 6) * G.cap + j];
+    } else {
+      __builtin_amdgcn_wave_barrier();
+      if (lane < cn) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wbox[6 * lane + k] = P.pre_box[6 * (int64_t)(v0 + lane) + k];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (open)
+        for (int q = 0; q < cn; ++q)
+          if (may_hit_box(wbox + 6 * q, r, u, inv)) m |= 1ull << q;
+      if (keep && open) G.masks[(int64_t)(v0 >> 6) * G.cap + j] = m;
+    }
     while (true) {
       const int lq = m ? __ffsll((long long)m) - 1 : 64;
       const int q = wave_min_i(lq);
@@ -396,47 +404,221 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
   G.vol[i] = vol;
   G.st[i] = st;
+  G.ev[i] = 0;
+  G.hh[i] = HASH_SEED;
+  G.hn[i] = 0;
+  if (st != TR_ALIVE) {  // the history ends at its start
+    if (R.events) R.events[i] = 0;
+    if (R.status) R.status[i] = st;
+    if (R.hash) R.hash[i] = HASH_SEED;
+  }
+}
+
+// One history's state between events.
+struct Hist {
+  D3 r, u;
+  int32_t vol, ev;
+  uint32_t dc;
+  int hn;
+  uint8_t st;
+  unsigned long long hh;
+  uint64_t item;
+};
+
+// One event of a live history (particle_sim.h:136-154): ray_fire to the
+// volume's boundary with the exclusion history, sample the flight, tally,
+// then collide or cross.  hist is the lane's exclusion list (HIST entries).
+template <bool MIXED, class STK>
+__device__ __forceinline__ void event_step(const TransportDev& P, const TransportRun& R,
+                                           unsigned long long* ltally, STK& S, int32_t* hist, Hist& H)
+{
+  const int32_t slot = (H.vol >= 0 && H.vol < P.n_ids) ? P.id2slot[H.vol] : -1;
+  const int32_t t = slot >= 0 ? P.vol_tree[slot] : -1;
+  if (t < 1 || t > P.ntrees) {
+    H.st = TR_ERROR;
+    return;
+  }
+  const SurfTreeDev& T = P.table[t];
+  // Particle::surf_dist (particle_sim.h:41-54)
+  const RayD ray = make_ray_d3(H.r, H.u);
+  HitState h{INFTY, INT32_MAX, ID_NONE, false};
+  S.reset();
+  trace_s<RAYFIRE, MIXED>(T, ray, ORIENT_EXITING, hist, H.hn, S, h);
+  if (h.surf == ID_NONE) {
+    H.st = TR_LOST;
+    return;
+  }
+  if (h.t == 0.0) {
+    H.st = TR_STUCK;
+    return;
+  }
+  // the hit face joins the exclusion history (ray_tracer.cpp:324)
+  if (H.hn == HIST) {
+    for (int j = 1; j < HIST; ++j) hist[j - 1] = hist[j];
+    --H.hn;
+  }
+  hist[H.hn++] = h.face;
+  // sample_collision_distance (particle_sim.h:56-58)
+  double ua, ub;
+  uniform2(P.seed, P.stream, H.item, H.dc++, ua, ub);
+  const double dcol = -log_series(1.0 - ua) * P.mfp;
+  // advance (particle_sim.h:70-83)
+  const double s = dcol < h.t ? dcol : h.t;
+  H.r = d3(H.r.x + s * H.u.x, H.r.y + s * H.u.y, H.r.z + s * H.u.z);
+  const unsigned long long q = (unsigned long long)__double2ll_rn(s * 4294967296.0);  // s * 2^32
+  if (R.lds_tally)
+    atomicAdd(ltally + slot, q);
+  else
+    atomicAdd(R.tally_fx + slot, q);
+  H.hh = mix64(H.hh ^ (unsigned long long)(uint32_t)H.vol);
+  H.hh = mix64(H.hh ^ (unsigned long long)__double_as_longlong(s));
+  ++H.ev;
+  if (dcol < h.t) {
+    // collide (particle_sim.h:60-68), absorption first
+    if (ub < P.p_absorb) {
+      H.st = TR_ABSORBED;
+    } else {
+      if (!iso_dir(P.seed, P.stream, H.item, H.dc, H.u)) H.st = TR_ERROR;
+      H.hn = 0;
+    }
+  } else {
+    // cross_surface (particle_sim.h:85-130): the hit surface's boundary
+    // condition (MeshManager::get_surface_property, default transmission)
+    const int32_t sid = h.surf;
+    const bool known = sid >= 0 && sid < P.nsurf;
+    const uint8_t bc = (P.surf_bc && known) ? P.surf_bc[sid] : (uint8_t)BC_TRANSMISSION;
+    if (bc == BC_VACUUM) {
+      H.st = TR_VACUUM;
+    } else if (bc == BC_REFLECTING) {
+      // particle_sim.h:90-104; XDG::surface_normal (xdg.cpp:272-290) is
+      // face_normal of the last history entry, the face just hit, whose
+      // unflipped normal the tree stores in leaf order
+      const double* pn = T.normals + 3 * (int64_t)h.leaf;
+      const D3 nrm = d3(pn[0], pn[1], pn[2]);
+      const double proj = nrm.x * H.u.x + nrm.y * H.u.y + nrm.z * H.u.z;
+      const double mag = sqrt(nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z);
+      const double sc = 2.0 * proj / mag;
+      H.u = normalize(d3(H.u.x - sc * nrm.x, H.u.y - sc * nrm.y, H.u.z - sc * nrm.z));
+      hist[0] = h.face;  // history_ = {history_.back()}
+      H.hn = 1;
+    } else {
+      // MeshManager::next_volume (mesh_manager_interface.cpp:202-214)
+      int32_t nv = ID_NONE;
+      if (known) {
+        if (P.sense_fwd[sid] == H.vol)
+          nv = P.sense_rev[sid];
+        else if (P.sense_rev[sid] == H.vol)
+          nv = P.sense_fwd[sid];
+      }
+      if (nv == ID_NONE) {
+        H.st = TR_ERROR;
+      } else {
+        H.vol = nv;
+        if (H.vol == P.ipc) H.st = TR_LEAKED;
+      }
+    }
+  }
+  if (H.st == TR_ALIVE && H.ev >= P.max_events) H.st = TR_CAPPED;
+}
+
+__device__ __forceinline__ void record_end(const TransportRun& R, int64_t i, const Hist& H)
+{
+  if (R.events) R.events[i] = H.ev;
+  if (R.status) R.status[i] = H.st;
+  if (R.hash) R.hash[i] = H.hh;
+}
+
+// saved state of the staged form (start stage or event rounds)
+__device__ __forceinline__ void load_hist(const TransportRun& R, int64_t i, Hist& H, int32_t* hist)
+{
+  const TransportStage& G = R.stage;
+  load_start(G, i, H.r, H.u);
+  H.vol = G.vol[i];
+  H.dc = G.dc[i];
+  H.st = G.st[i];
+  H.ev = G.ev[i];
+  H.hh = G.hh[i];
+  H.hn = G.hn[i];
+  for (int k = 0; k < H.hn; ++k) hist[k] = G.hist8[k * G.cap + i];
+  H.item = R.first + (uint64_t)i;
+}
+
+__device__ __forceinline__ void store_hist(const TransportRun& R, int64_t i, const Hist& H, const int32_t* hist)
+{
+  const TransportStage& G = R.stage;
+  G.ru[i] = H.r.x;
+  G.ru[G.cap + i] = H.r.y;
+  G.ru[2 * G.cap + i] = H.r.z;
+  G.ru[3 * G.cap + i] = H.u.x;
+  G.ru[4 * G.cap + i] = H.u.y;
+  G.ru[5 * G.cap + i] = H.u.z;
+  G.vol[i] = H.vol;
+  G.dc[i] = H.dc;
+  G.st[i] = H.st;
+  G.ev[i] = H.ev;
+  G.hh[i] = H.hh;
+  G.hn[i] = (uint8_t)H.hn;
+  for (int k = 0; k < H.hn; ++k) G.hist8[k * G.cap + i] = hist[k];
+}
+
+// LDS of the event kernels: per-workgroup tallies, then the stack share,
+// then the exclusion histories
+struct EventLds {
+  unsigned long long* ltally;
+  int32_t* hist;
+  HybridStack S;
+};
+
+__device__ __forceinline__ EventLds event_lds(unsigned char* smem, const TransportDev& P, const TransportRun& R)
+{
+  EventLds E;
+  E.ltally = (unsigned long long*)smem;
+  const size_t tally_bytes = R.lds_tally ? (((size_t)P.nvol * 8 + 15) & ~(size_t)15) : 0;
+  int32_t* lstack = (int32_t*)(smem + tally_bytes);
+  E.hist = lstack + (size_t)R.lds_stack * blockDim.x + threadIdx.x * HIST;
+  if (R.lds_tally) {
+    for (int i = threadIdx.x; i < P.nvol; i += blockDim.x) E.ltally[i] = 0ull;
+    __syncthreads();
+  }
+  const int64_t gl = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  E.S.lbase = lstack + threadIdx.x;
+  E.S.lstride = blockDim.x;
+  E.S.gbase = R.spill ? R.spill + gl : nullptr;
+  E.S.gstride = R.spill_lanes;
+  E.S.lcap = R.lds_stack;
+  E.S.cap = R.spill ? R.stack_cap : R.lds_stack;
+  E.S.reset();
+  return E;
+}
+
+__device__ __forceinline__ void flush_tally(const TransportDev& P, const TransportRun& R, const EventLds& E)
+{
+  if (R.lds_tally) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < P.nvol; i += blockDim.x)
+      if (E.ltally[i]) atomicAdd(R.tally_fx + i, E.ltally[i]);
+  }
 }
 
 // Whole histories keep more state live than one query: workgroups of at
 // most 256 lanes let the kernel use more than 128 VGPRs instead of spilling
 // to scratch; three waves per SIMD (168 VGPRs) is the target.
-// STAGED: the start state comes from k_transport_start instead of being
-// sampled here.
+// STAGED: the histories are the staged form's list R.stage.idx[0 .. n),
+// n = *R.stage.live after event rounds and R.n otherwise, resumed from
+// their saved state (one that has ended already wrote its record);
+// otherwise histories [0, R.n) are sampled and started here.
 template <bool MIXED, bool STAGED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_transport(TransportDev P, TransportRun R)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  unsigned long long* ltally = (unsigned long long*)smem;
-  const size_t tally_bytes = R.lds_tally ? (((size_t)P.nvol * 8 + 15) & ~(size_t)15) : 0;
-  int32_t* lstack = (int32_t*)(smem + tally_bytes);
-  int32_t* lhist = lstack + (size_t)R.lds_stack * blockDim.x;
-  if (R.lds_tally) {
-    for (int i = threadIdx.x; i < P.nvol; i += blockDim.x) ltally[i] = 0ull;
-    __syncthreads();
-  }
-  const int64_t gl = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  HybridStack S;
-  S.lbase = lstack + threadIdx.x;
-  S.lstride = blockDim.x;
-  S.gbase = R.spill ? R.spill + gl : nullptr;
-  S.gstride = R.spill_lanes;
-  S.lcap = R.lds_stack;
-  S.cap = R.spill ? R.stack_cap : R.lds_stack;
-  S.reset();
-  int32_t* hist = lhist + threadIdx.x * HIST;
+  EventLds E = event_lds(smem, P, R);
   const int lane = threadIdx.x & 63;
-
-  uint64_t cur = 0, end = 0;  // claimed history range (relative indices)
+  const uint64_t n = (STAGED && R.rounds > 0) ? (uint64_t)*R.stage.live : (uint64_t)R.n;
+  uint64_t cur = 0, end = 0;  // claimed range of list positions
   bool alive = false, have = false;
   int64_t hix = 0;
-  uint64_t item = 0;
-  uint32_t dc = 0;
-  D3 r = d3(0, 0, 0), u = d3(0, 0, 0);
-  int32_t vol = ID_NONE, ev = 0;
-  int hn = 0;
-  uint8_t st = TR_ALIVE;
-  unsigned long long hh = HASH_SEED;
+  Hist H;
+  H.st = TR_ALIVE;
   while (true) {
     // a lane whose history ended waits until at least R.refill lanes of the
     // wave (or all remaining ones) are idle: starting a history (sampling,
@@ -444,9 +626,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // lanes that start together
     const bool idle = !alive;
     if (idle && have) {
-      if (R.events) R.events[hix] = ev;
-      if (R.status) R.status[hix] = st;
-      if (R.hash) R.hash[hix] = hh;
+      record_end(R, hix, H);
       have = false;
     }
     const int nact = __popcll(__ballot(1));
@@ -464,124 +644,103 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         if (need) {
           cur = b + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)) * (unsigned long long)R.chunk;
           end = cur + (uint64_t)R.chunk;
-          if (end > (uint64_t)R.n) end = (uint64_t)R.n;
+          if (end > n) end = n;
         }
       }
-      if (cur >= (uint64_t)R.n) break;
-      hix = STAGED ? (int64_t)R.stage.idx[cur] : (int64_t)cur;
-      ++cur;
+      if (cur >= n) break;
       have = true;
-      item = R.first + (uint64_t)hix;
-      ev = 0;
-      hh = HASH_SEED;
-      hn = 0;
       if (STAGED) {
-        const TransportStage& G = R.stage;
-        r = d3(G.ru[hix], G.ru[G.cap + hix], G.ru[2 * G.cap + hix]);
-        u = d3(G.ru[3 * G.cap + hix], G.ru[4 * G.cap + hix], G.ru[5 * G.cap + hix]);
-        vol = G.vol[hix];
-        dc = G.dc[hix];
-        st = G.st[hix];
+        hix = (int64_t)R.stage.idx[cur];
+        load_hist(R, hix, H, E.hist);
+        have = H.st == TR_ALIVE;
       } else {
-        sample_start(P, item, dc, r, u, st);
-        vol = start_volume<MIXED>(P, r, u, S, st);
+        hix = (int64_t)cur;
+        H.item = R.first + (uint64_t)hix;
+        H.ev = 0;
+        H.hh = HASH_SEED;
+        H.hn = 0;
+        sample_start(P, H.item, H.dc, H.r, H.u, H.st);
+        E.S.reset();
+        H.vol = start_volume<MIXED>(P, H.r, H.u, E.S, H.st);
       }
-      alive = st == TR_ALIVE;
+      ++cur;
+      alive = H.st == TR_ALIVE;
       continue;
     }
-    // one event
-    const int32_t slot = (vol >= 0 && vol < P.n_ids) ? P.id2slot[vol] : -1;
-    const int32_t t = slot >= 0 ? P.vol_tree[slot] : -1;
-    if (t < 1 || t > P.ntrees) {
-      st = TR_ERROR;
-      alive = false;
-      continue;
-    }
-    const SurfTreeDev& T = P.table[t];
-    // Particle::surf_dist (particle_sim.h:41-54)
-    const RayD ray = make_ray_d3(r, u);
-    HitState h{INFTY, INT32_MAX, ID_NONE, false};
-    trace_s<RAYFIRE, MIXED>(T, ray, ORIENT_EXITING, hist, hn, S, h);
-    if (h.surf == ID_NONE) {
-      st = TR_LOST;
-    } else if (h.t == 0.0) {
-      st = TR_STUCK;
-    } else {
-      // the hit face joins the exclusion history (ray_tracer.cpp:324)
-      if (hn == HIST) {
-        for (int j = 1; j < HIST; ++j) hist[j - 1] = hist[j];
-        --hn;
-      }
-      hist[hn++] = h.face;
-      // sample_collision_distance (particle_sim.h:56-58)
-      double ua, ub;
-      uniform2(P.seed, P.stream, item, dc++, ua, ub);
-      const double dcol = -log_series(1.0 - ua) * P.mfp;
-      // advance (particle_sim.h:70-83)
-      const double s = dcol < h.t ? dcol : h.t;
-      r = d3(r.x + s * u.x, r.y + s * u.y, r.z + s * u.z);
-      const unsigned long long q = (unsigned long long)__double2ll_rn(s * 4294967296.0);  // s * 2^32
-      if (R.lds_tally)
-        atomicAdd(ltally + slot, q);
-      else
-        atomicAdd(R.tally_fx + slot, q);
-      hh = mix64(hh ^ (unsigned long long)(uint32_t)vol);
-      hh = mix64(hh ^ (unsigned long long)__double_as_longlong(s));
-      ++ev;
-      if (dcol < h.t) {
-        // collide (particle_sim.h:60-68), absorption first
-        if (ub < P.p_absorb) {
-          st = TR_ABSORBED;
-        } else {
-          if (!iso_dir(P.seed, P.stream, item, dc, u)) st = TR_ERROR;
-          hn = 0;
-        }
-      } else {
-        // cross_surface (particle_sim.h:85-130): the hit surface's boundary
-        // condition (MeshManager::get_surface_property, default transmission)
-        const int32_t sid = h.surf;
-        const bool known = sid >= 0 && sid < P.nsurf;
-        const uint8_t bc = (P.surf_bc && known) ? P.surf_bc[sid] : (uint8_t)BC_TRANSMISSION;
-        if (bc == BC_VACUUM) {
-          st = TR_VACUUM;
-        } else if (bc == BC_REFLECTING) {
-          // particle_sim.h:90-104; XDG::surface_normal (xdg.cpp:272-290) is
-          // face_normal of the last history entry, the face just hit, whose
-          // unflipped normal the tree stores in leaf order
-          const double* pn = T.normals + 3 * (int64_t)h.leaf;
-          const D3 nrm = d3(pn[0], pn[1], pn[2]);
-          const double proj = nrm.x * u.x + nrm.y * u.y + nrm.z * u.z;
-          const double mag = sqrt(nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z);
-          const double sc = 2.0 * proj / mag;
-          u = normalize(d3(u.x - sc * nrm.x, u.y - sc * nrm.y, u.z - sc * nrm.z));
-          hist[0] = h.face;  // history_ = {history_.back()}
-          hn = 1;
-        } else {
-          // MeshManager::next_volume (mesh_manager_interface.cpp:202-214)
-          int32_t nv = ID_NONE;
-          if (known) {
-            if (P.sense_fwd[sid] == vol)
-              nv = P.sense_rev[sid];
-            else if (P.sense_rev[sid] == vol)
-              nv = P.sense_fwd[sid];
-          }
-          if (nv == ID_NONE) {
-            st = TR_ERROR;
-          } else {
-            vol = nv;
-            if (vol == P.ipc) st = TR_LEAKED;
-          }
-        }
-      }
-      if (st == TR_ALIVE && ev >= P.max_events) st = TR_CAPPED;
-    }
-    alive = st == TR_ALIVE;
+    event_step<MIXED>(P, R, E.ltally, E.S, E.hist, H);
+    alive = H.st == TR_ALIVE;
   }
-  if (R.lds_tally) {
-    __syncthreads();
-    for (int i = threadIdx.x; i < P.nvol; i += blockDim.x)
-      if (ltally[i]) atomicAdd(R.tally_fx + i, ltally[i]);
+  flush_tally(P, R, E);
+}
+
+// One event round of the staged form: each live history of the sorted list
+// R.stage.idx[0 .. *R.stage.live) runs one event (grid-stride, so a wave's
+// lanes take neighbours in the sorted order: the same volume, nearby starts
+// and directions).  The state goes back to HBM; a history that ends writes
+// its record.
+template <bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_transport_round(TransportDev P,
+                                                                                                TransportRun R)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  EventLds E = event_lds(smem, P, R);
+  const uint64_t n = *R.stage.live;
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    const int64_t i = R.stage.idx[j];
+    Hist H;
+    load_hist(R, i, H, E.hist);
+    event_step<MIXED>(P, R, E.ltally, E.S, E.hist, H);
+    store_hist(R, i, H, E.hist);
+    if (H.st != TR_ALIVE) record_end(R, i, H);
   }
+  flush_tally(P, R, E);
+}
+
+// Order of the next round: live histories first, by volume slot, then by
+// cell and direction (start_key's bits); ended histories get the largest
+// key (dead_key), so after the sort the live ones lead.
+__device__ __forceinline__ uint32_t dead_key(int key_bits)
+{
+  return key_bits >= 32 ? 0xffffffffu : ((1u << key_bits) - 1u);
+}
+
+__global__ __launch_bounds__(256) void k_transport_keys(TransportDev P, TransportRun R, int slot_bits, int key_bits)
+{
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const TransportStage& G = R.stage;
+  if (i < R.n) {
+    uint32_t key = dead_key(key_bits);
+    if (G.st[i] == TR_ALIVE) {
+      const int32_t v = G.vol[i];
+      const int32_t slot = (v >= 0 && v < P.n_ids) ? P.id2slot[v] : -1;
+      D3 r, u;
+      load_start(G, i, r, u);
+      // up to 9 spatial bits: the cell on a 4 x 4 x 4 grid (the top 6
+      // Morton bits of start_key) and the direction's face
+      const uint32_t k16 = start_key(P, r, u);
+      const uint32_t k9 = ((k16 >> 10) << 3) | ((k16 >> 4) & 7u);
+      const int sp = key_bits - slot_bits;
+      key = ((uint32_t)(slot < 0 ? 0 : slot) << sp) | (k9 >> (9 - sp));
+    }
+    G.key[i] = key;
+    G.idx[i] = (int32_t)i;
+  }
+}
+
+// live count = position of the first dead key in the sorted keys
+__global__ void k_transport_live(TransportRun R, int key_bits)
+{
+  if (threadIdx.x != 0) return;
+  const uint32_t dead = dead_key(key_bits);
+  int64_t lo = 0, hi = R.n;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (R.stage.key[mid] < dead)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  *R.stage.live = (uint32_t)lo;
 }
 
 __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
@@ -698,6 +857,23 @@ void launch_transport(const TransportDev& P, const TransportRun& R, bool mixed, 
     const int64_t need = (Q.n + (int64_t)L.block * Q.chunk - 1) / ((int64_t)L.block * Q.chunk);
     const unsigned g = (unsigned)std::max<int64_t>(1, std::min(L.grid, need));
     if (R.staged) {
+      // R.rounds sorted one-event rounds, then the event kernel takes the
+      // live histories, each until it ends
+      int slot_bits = 1;
+      while (((int64_t)1 << slot_bits) <= (int64_t)P.nvol) ++slot_bits;
+      const int key_bits = std::min(32, slot_bits + 9);
+      const dim3 gk((unsigned)((Q.n + 255) / 256));
+      const unsigned gr = (unsigned)std::max<int64_t>(1, L.grid);
+      for (int rd = 0; R.rounds > 0; ++rd) {
+        hipLaunchKernelGGL(k_transport_keys, gk, dim3(256), 0, s, P, Q, slot_bits, key_bits);
+        radix_sort_pairs(R.stage.key, R.stage.idx, R.stage.k2, R.stage.v2, Q.n, key_bits, R.stage.hist, s);
+        hipLaunchKernelGGL(k_transport_live, dim3(1), dim3(64), 0, s, Q, key_bits);
+        if (rd >= R.rounds) break;
+        if (mixed)
+          hipLaunchKernelGGL(k_transport_round<true>, dim3(gr), dim3(L.block), L.lds_bytes, s, P, Q);
+        else
+          hipLaunchKernelGGL(k_transport_round<false>, dim3(gr), dim3(L.block), L.lds_bytes, s, P, Q);
+      }
       if (mixed)
         hipLaunchKernelGGL((k_transport<true, true>), dim3(g), dim3(L.block), L.lds_bytes, s, P, Q);
       else
