@@ -136,6 +136,12 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *   "transport_items" start-stage item buffer entries per history (1..64,
  *                 default 8); histories whose items do not fit run the
  *                 volume loop themselves
+ *   "transport_events" staged form: events (and the start stage's
+ *                 point_in_volume items) by kernels whose traversal is a
+ *                 per-lane state machine, so a lane whose ray has ended goes
+ *                 on to its next event without waiting for the wave's
+ *                 longest traversal (1, default), or by whole-event kernels
+ *                 (0)
  *   "transport_rounds" staged form: one-event rounds over the live
  *                 histories, sorted by volume, cell and direction before each
  *                 round, before the event kernel takes the rest (0..64,

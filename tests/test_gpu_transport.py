@@ -153,7 +153,7 @@ def test_transport_graph_replays(ctx, torch_mod, lattice):
 @pytest.fixture
 def start_form(ctx):
     """Restores the transport start options after a test changes them."""
-    names = ("transport_start", "transport_slice", "transport_items", "transport_rounds")
+    names = ("transport_start", "transport_slice", "transport_items", "transport_rounds", "transport_events")
     keep = [ctx.get_option(k) for k in names]
     yield
     for k, v in zip(names, keep):
@@ -165,14 +165,17 @@ def test_transport_start_forms(ctx, lattice, prec, start_form):
     """Starts sampled inside the event kernel, or by the separate start
     kernels (in index order, or sorted by the start's cell and direction)
     over slices of 2^6, 2^10 and 2^22 histories (one slice, many slices, a
-    ragged last slice): every history and the tallies equal the oracle's."""
+    ragged last slice), events by the whole-event or the traversal-level
+    kernel: every history and the tallies equal the oracle's."""
     h, ref = _problem(ctx, lattice)
     try:
         n, first = 5000, 123
         r = ref(first, n)
-        for staged, slice_log2 in [(0, 22), (1, 6), (1, 22), (2, 6), (2, 10), (2, 22)]:
+        for staged, slice_log2, events in [(0, 22, 0), (1, 6, 1), (1, 22, 0), (2, 6, 1), (2, 10, 0), (2, 22, 1),
+                                           (2, 22, 0)]:
             ctx.set_option("transport_start", staged)
             ctx.set_option("transport_slice", slice_log2)
+            ctx.set_option("transport_events", events)
             out = ctx.transport_run(h, first, n, len(lattice[1]), precision=prec)
             _check(out, r)
     finally:
@@ -189,9 +192,10 @@ def test_transport_event_rounds(ctx, lattice, prec, start_form):
         n, first = 5000, 31
         r = ref(first, n)
         ctx.set_option("transport_start", 2)
-        for rounds, slice_log2 in [(0, 22), (1, 22), (3, 22), (64, 22), (3, 7)]:
+        for rounds, slice_log2, events in [(0, 22, 1), (1, 22, 1), (3, 22, 0), (64, 22, 1), (3, 7, 1), (3, 7, 0)]:
             ctx.set_option("transport_rounds", rounds)
             ctx.set_option("transport_slice", slice_log2)
+            ctx.set_option("transport_events", events)
             _check(ctx.transport_run(h, first, n, len(lattice[1]), precision=prec), r)
     finally:
         ctx.transport_destroy(h)

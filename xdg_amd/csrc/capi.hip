@@ -585,6 +585,7 @@ int* opt_tr_start(xdg_hip_ctx* c) { return &c->cfg.tr_start; }
 int* opt_tr_slice(xdg_hip_ctx* c) { return &c->cfg.tr_slice; }
 int* opt_tr_items(xdg_hip_ctx* c) { return &c->cfg.tr_items; }
 int* opt_tr_rounds(xdg_hip_ctx* c) { return &c->cfg.tr_rounds; }
+int* opt_tr_events(xdg_hip_ctx* c) { return &c->cfg.tr_events; }
 
 const OptionRef OPTIONS[] = {
     {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 7, opt_persistent},
@@ -594,6 +595,7 @@ const OptionRef OPTIONS[] = {
     {"refit", 0, 1, opt_refit},            {"segment_order", 0, 2, opt_seg_order},
     {"transport_start", 0, 2, opt_tr_start}, {"transport_slice", 6, 24, opt_tr_slice},
     {"transport_items", 1, 64, opt_tr_items}, {"transport_rounds", 0, 64, opt_tr_rounds},
+    {"transport_events", 0, 1, opt_tr_events},
 };
 
 const OptionRef& find_option(const char* name)
@@ -1641,7 +1643,9 @@ void transport_launch(xdg_hip_ctx* c, TransportHost& t, const TransportLaunch& L
     R.stage.over = R.stage.st + cap;
     R.stage.hn = R.stage.over + cap;
     R.stage.live = t.d_live;
+    R.stage.claim = (unsigned long long*)(t.d_live + 2);  // 8-byte aligned, inside the 64-byte buffer
     R.rounds = c->cfg.tr_rounds;
+    R.events_pt = c->cfg.tr_events;
     R.start_grid = t.start_grid[mixed ? 1 : 0];
   }
   launch_transport(t.dev, R, mixed, L, s);

@@ -164,6 +164,7 @@ struct LaunchCfg {
   int tr_slice = 22;  // transport, staged form: log2 of the histories per slice (about 600 MB of state at 22)
   int tr_items = 8;   // transport, staged form: start-stage item buffer entries per history
   int tr_rounds = 0;  // transport, staged form: sorted one-event rounds before the event kernel takes the rest
+  int tr_events = 1;  // transport, staged form: events by the traversal-level kernel (1) or whole-event kernel (0)
 };
 
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s);
@@ -302,6 +303,7 @@ struct TransportStage {
   uint8_t* hn;
   int32_t* hist8;  // exclusion history, 8 x cap (entry-major)
   uint32_t* live;  // live histories at the head of idx (one word)
+  unsigned long long* claim;  // item claim counter of the start stage's persistent kernel
   int64_t cap;    // histories per slice
 };
 // per history, beside the sort histogram and the 8-byte items: position and
@@ -327,10 +329,12 @@ struct TransportRun {
   TransportStage stage;
   int64_t start_grid;            // resident workgroups of the start stage's point_in_volume kernel
   int rounds;                    // staged form: sorted one-event rounds before the event kernel
+  int events_pt;                 // staged form: traversal-level event kernel (k_transport_pt)
 };
 struct TransportLaunch {
   int block;
-  int64_t grid;
+  int64_t grid;     // resident workgroups of the whole-event kernel (at least grid_pt)
+  int64_t grid_pt;  // of the traversal-level event kernel
   size_t lds_bytes;
 };
 // Shape of the persistent grid for (block, lds_stack, nvol): resident

@@ -373,6 +373,128 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
 }
 
+// The same items with the traversal as a per-lane state machine (as in the
+// persistent ray_fire): a lane whose point_in_volume has ended waits until
+// R.refill lanes of its wave (or all remaining ones) have too, then those
+// lanes record their results and claim the next items together, while the
+// other lanes keep descending.  Items are claimed in list order from the
+// counter *G.claim (zeroed before the launch).
+template <bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_transport_piv_pt(TransportDev P,
+                                                                                                 TransportRun R)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  LdsStack S((int32_t*)smem + threadIdx.x, blockDim.x, R.stack_cap);
+  const int lane = threadIdx.x & 63;
+  const int32_t DONE = CHILD_EMPTY;
+  const TransportStage& G = R.stage;
+  const uint64_t total = min((uint64_t)P.slot_cnt[P.nvol], (uint64_t)G.item_cap);
+  bool traced = false;
+  int2 it = make_int2(0, 0);
+  TreeView T{nullptr, nullptr, nullptr, nullptr};
+  RayD ray;
+  BoxRayF bf{0, 0, 0, 0, 0, 0};
+  BoxRayD bd{0, 0, 0, 0, 0, 0};
+  PlaneOfs po{0, 0, 0, 0, 0, 0};
+  HitState h{INFTY, INT32_MAX, ID_NONE, false};
+  float tf = 0.0f;
+  int32_t node = DONE, leaf = DONE;
+  while (true) {
+    const bool done = node == DONE && leaf == DONE;
+    const unsigned long long dmask = __ballot(done);
+    const int nact = __popcll(__ballot(1));
+    const bool go = done && __popcll(dmask) >= (R.refill < nact ? R.refill : nact);
+    if (go && traced) {
+      traced = false;
+      if (h.surf != ID_NONE && h.inside) atomicMin(G.found + it.x, it.y);
+    }
+    const unsigned long long cmask = __ballot(go);
+    if (cmask) {
+      const int leader = __ffsll((long long)cmask) - 1;
+      unsigned long long b = 0;
+      if (lane == leader) b = atomicAdd(G.claim, (unsigned long long)__popcll(cmask));
+      b = __shfl(b, leader);
+      if (go) {
+        const uint64_t k = b + (unsigned long long)__popcll(cmask & ((1ull << lane) - 1ull));
+        if (k >= total) break;
+        it = G.items[k];
+        // an item whose history holds a lower containing slot cannot lower
+        // the minimum: skipped (no result depends on the skip)
+        if (G.over[it.x] || __hip_atomic_load(G.found + it.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < it.y)
+          continue;
+        const SurfTreeDev& TT = P.table[P.vol_tree[it.y]];
+        D3 r, u;
+        load_start(G, it.x, r, u);
+        ray = make_ray_d3(r, u);
+        h = HitState{INFTY, INT32_MAX, ID_NONE, false};
+        T.nodes4_d = TT.nodes4_d;
+        T.nodes4_f = TT.nodes4_f;
+        T.tris = TT.tris;
+        T.normals = TT.normals;
+        traced = true;
+        S.reset();
+        if (MIXED && !mixed_safe(TT, ray)) {
+          trace_s<PIV, false>(TT, ray, ORIENT_ANY, nullptr, 0, S, h);
+          continue;
+        }
+        if (MIXED) {
+          const float ox = (float)ray.o.x, oy = (float)ray.o.y, oz = (float)ray.o.z;
+          bf.ix = safe_inv_f((float)ray.d.x);
+          bf.iy = safe_inv_f((float)ray.d.y);
+          bf.iz = safe_inv_f((float)ray.d.z);
+          bf.nx = -ox * bf.ix;
+          bf.ny = -oy * bf.iy;
+          bf.nz = -oz * bf.iz;
+        } else {
+          bd.ix = safe_inv_d(ray.d.x);
+          bd.iy = safe_inv_d(ray.d.y);
+          bd.iz = safe_inv_d(ray.d.z);
+          bd.nx = -ray.o.x * bd.ix;
+          bd.ny = -ray.o.y * bd.iy;
+          bd.nz = -ray.o.z * bd.iz;
+        }
+        po = MIXED ? plane_ofs_f(bf) : plane_ofs_d(bd);
+        tf = f32_up(h.t);
+        node = TT.n_prims > 0 ? TT.root4 : DONE;
+      }
+    }
+    while (node >= 0) {
+      float te[4];
+      int32_t ch[4];
+      node4_test<MIXED>(T, node, bf, bd, po, tf, h.t, te, ch);
+      cswap(te[0], ch[0], te[1], ch[1]);
+      cswap(te[2], ch[2], te[3], ch[3]);
+      cswap(te[0], ch[0], te[2], ch[2]);
+      cswap(te[1], ch[1], te[3], ch[3]);
+      cswap(te[1], ch[1], te[2], ch[2]);
+      if (ch[0] != CHILD_EMPTY) {
+        if (ch[3] != CHILD_EMPTY) S.push(ch[3]);
+        if (ch[2] != CHILD_EMPTY) S.push(ch[2]);
+        if (ch[1] != CHILD_EMPTY) S.push(ch[1]);
+        node = ch[0];
+      } else {
+        node = S.empty() ? DONE : S.pop();
+      }
+      if (node < 0 && node != DONE && leaf == DONE) {
+        leaf = node;
+        node = S.empty() ? DONE : S.pop();
+      }
+      if (__all(leaf != DONE)) break;
+    }
+    if (leaf == DONE && node != DONE) {
+      leaf = node;
+      node = S.empty() ? DONE : S.pop();
+    }
+    if (leaf != DONE) {
+      const int32_t first = leaf_first(leaf);
+      const int32_t cnt = leaf_count(leaf);
+      leaf = DONE;
+      for (int32_t j = 0; j < cnt; ++j) tri_closest_hit<PIV>(T, first + j, ray, ORIENT_ANY, nullptr, 0, h);
+      tf = f32_up(h.t);
+    }
+  }
+}
+
 // Start volume of each history from its lowest containing slot; histories
 // whose items overflowed the buffer run start_volume.
 template <bool MIXED>
@@ -425,25 +547,13 @@ struct Hist {
   uint64_t item;
 };
 
-// One event of a live history (particle_sim.h:136-154): ray_fire to the
-// volume's boundary with the exclusion history, sample the flight, tally,
-// then collide or cross.  hist is the lane's exclusion list (HIST entries).
-template <bool MIXED, class STK>
-__device__ __forceinline__ void event_step(const TransportDev& P, const TransportRun& R,
-                                           unsigned long long* ltally, STK& S, int32_t* hist, Hist& H)
+// The rest of an event once its ray_fire hit h in volume slot `slot` is
+// known: sample the flight, tally, then collide or cross
+// (particle_sim.h:56-130).  normals: the traced tree's stored normals.
+__device__ __forceinline__ void event_finish(const TransportDev& P, const TransportRun& R,
+                                             unsigned long long* ltally, int32_t* hist, Hist& H, const HitState& h,
+                                             const double* normals, int32_t slot)
 {
-  const int32_t slot = (H.vol >= 0 && H.vol < P.n_ids) ? P.id2slot[H.vol] : -1;
-  const int32_t t = slot >= 0 ? P.vol_tree[slot] : -1;
-  if (t < 1 || t > P.ntrees) {
-    H.st = TR_ERROR;
-    return;
-  }
-  const SurfTreeDev& T = P.table[t];
-  // Particle::surf_dist (particle_sim.h:41-54)
-  const RayD ray = make_ray_d3(H.r, H.u);
-  HitState h{INFTY, INT32_MAX, ID_NONE, false};
-  S.reset();
-  trace_s<RAYFIRE, MIXED>(T, ray, ORIENT_EXITING, hist, H.hn, S, h);
   if (h.surf == ID_NONE) {
     H.st = TR_LOST;
     return;
@@ -493,7 +603,7 @@ __device__ __forceinline__ void event_step(const TransportDev& P, const Transpor
       // particle_sim.h:90-104; XDG::surface_normal (xdg.cpp:272-290) is
       // face_normal of the last history entry, the face just hit, whose
       // unflipped normal the tree stores in leaf order
-      const double* pn = T.normals + 3 * (int64_t)h.leaf;
+      const double* pn = normals + 3 * (int64_t)h.leaf;
       const D3 nrm = d3(pn[0], pn[1], pn[2]);
       const double proj = nrm.x * H.u.x + nrm.y * H.u.y + nrm.z * H.u.z;
       const double mag = sqrt(nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z);
@@ -519,6 +629,36 @@ __device__ __forceinline__ void event_step(const TransportDev& P, const Transpor
     }
   }
   if (H.st == TR_ALIVE && H.ev >= P.max_events) H.st = TR_CAPPED;
+}
+
+// volume slot and surface tree of the history's current volume (-1: none)
+__device__ __forceinline__ int32_t event_slot(const TransportDev& P, const Hist& H, int32_t& tree)
+{
+  const int32_t slot = (H.vol >= 0 && H.vol < P.n_ids) ? P.id2slot[H.vol] : -1;
+  tree = slot >= 0 ? P.vol_tree[slot] : -1;
+  return (tree < 1 || tree > P.ntrees) ? -1 : slot;
+}
+
+// One event of a live history (particle_sim.h:136-154): ray_fire to the
+// volume's boundary with the exclusion history (Particle::surf_dist,
+// particle_sim.h:41-54), then event_finish.  hist is the lane's exclusion
+// list (HIST entries).
+template <bool MIXED, class STK>
+__device__ __forceinline__ void event_step(const TransportDev& P, const TransportRun& R,
+                                           unsigned long long* ltally, STK& S, int32_t* hist, Hist& H)
+{
+  int32_t t;
+  const int32_t slot = event_slot(P, H, t);
+  if (slot < 0) {
+    H.st = TR_ERROR;
+    return;
+  }
+  const SurfTreeDev& T = P.table[t];
+  const RayD ray = make_ray_d3(H.r, H.u);
+  HitState h{INFTY, INT32_MAX, ID_NONE, false};
+  S.reset();
+  trace_s<RAYFIRE, MIXED>(T, ray, ORIENT_EXITING, hist, H.hn, S, h);
+  event_finish(P, R, ltally, hist, H, h, T.normals, slot);
 }
 
 __device__ __forceinline__ void record_end(const TransportRun& R, int64_t i, const Hist& H)
@@ -673,6 +813,148 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   flush_tally(P, R, E);
 }
 
+// The staged form's event kernel with the traversal as a per-lane state
+// machine, as in the persistent ray_fire: a lane whose ray_fire has ended
+// waits until R.refill lanes of its wave (or all remaining ones) have too,
+// then those lanes finish their events together (flight, tally, collide or
+// cross), claim new histories where theirs ended, and start their next
+// rays; meanwhile the other lanes keep descending.  A wave therefore never
+// waits for its longest traversal before its other lanes go on.  The
+// histories are R.stage.idx[0 .. n), resumed from their saved state.
+template <bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_transport_pt(TransportDev P,
+                                                                                             TransportRun R)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  EventLds E = event_lds(smem, P, R);
+  const int lane = threadIdx.x & 63;
+  const int32_t DONE = CHILD_EMPTY;
+  const uint64_t n = R.rounds > 0 ? (uint64_t)*R.stage.live : (uint64_t)R.n;
+  bool have = false;    // the lane holds a live history
+  bool traced = false;  // its current event's ray is traced (or being traced)
+  int64_t hix = 0;
+  Hist H;
+  H.st = TR_ALIVE;
+  TreeView T{nullptr, nullptr, nullptr, nullptr};
+  RayD ray;
+  BoxRayF bf{0, 0, 0, 0, 0, 0};
+  BoxRayD bd{0, 0, 0, 0, 0, 0};
+  PlaneOfs po{0, 0, 0, 0, 0, 0};
+  HitState h{INFTY, INT32_MAX, ID_NONE, false};
+  float tf = 0.0f;
+  int32_t node = DONE, leaf = DONE, slot = -1;
+  while (true) {
+    const bool done = node == DONE && leaf == DONE;
+    const unsigned long long dmask = __ballot(done);
+    const int nact = __popcll(__ballot(1));
+    const bool go = done && __popcll(dmask) >= (R.refill < nact ? R.refill : nact);
+    if (go && traced) {
+      traced = false;
+      event_finish(P, R, E.ltally, E.hist, H, h, T.normals, slot);
+      if (H.st != TR_ALIVE) {
+        record_end(R, hix, H);
+        have = false;
+      }
+    }
+    // lanes without a history claim the next ones of the list
+    const bool need = go && !have;
+    const unsigned long long cmask = __ballot(need);
+    if (cmask) {
+      const int leader = __ffsll((long long)cmask) - 1;
+      unsigned long long b = 0;
+      if (lane == leader) b = atomicAdd(R.next, (unsigned long long)__popcll(cmask));
+      b = __shfl(b, leader);
+      if (need) {
+        const uint64_t pos = b + (unsigned long long)__popcll(cmask & ((1ull << lane) - 1ull));
+        if (pos >= n) break;
+        hix = (int64_t)R.stage.idx[pos];
+        load_hist(R, hix, H, E.hist);
+        have = H.st == TR_ALIVE;  // one that ended at its start has its record already
+      }
+    }
+    // lanes with a live history and no ray start the next event's ray
+    if (go && have) {
+      int32_t t;
+      slot = event_slot(P, H, t);
+      if (slot < 0) {
+        H.st = TR_ERROR;
+        record_end(R, hix, H);
+        have = false;
+        continue;
+      }
+      const SurfTreeDev& TT = P.table[t];
+      ray = make_ray_d3(H.r, H.u);
+      h = HitState{INFTY, INT32_MAX, ID_NONE, false};
+      T.nodes4_d = TT.nodes4_d;
+      T.nodes4_f = TT.nodes4_f;
+      T.tris = TT.tris;
+      T.normals = TT.normals;
+      traced = true;
+      E.S.reset();
+      if (MIXED && !mixed_safe(TT, ray)) {
+        // an origin beyond the tree's FP32 limit: FP64 boxes, traced at once
+        trace_s<RAYFIRE, false>(TT, ray, ORIENT_EXITING, E.hist, H.hn, E.S, h);
+        continue;
+      }
+      if (MIXED) {
+        const float ox = (float)ray.o.x, oy = (float)ray.o.y, oz = (float)ray.o.z;
+        bf.ix = safe_inv_f((float)ray.d.x);
+        bf.iy = safe_inv_f((float)ray.d.y);
+        bf.iz = safe_inv_f((float)ray.d.z);
+        bf.nx = -ox * bf.ix;
+        bf.ny = -oy * bf.iy;
+        bf.nz = -oz * bf.iz;
+      } else {
+        bd.ix = safe_inv_d(ray.d.x);
+        bd.iy = safe_inv_d(ray.d.y);
+        bd.iz = safe_inv_d(ray.d.z);
+        bd.nx = -ray.o.x * bd.ix;
+        bd.ny = -ray.o.y * bd.iy;
+        bd.nz = -ray.o.z * bd.iz;
+      }
+      po = MIXED ? plane_ofs_f(bf) : plane_ofs_d(bd);
+      tf = f32_up(h.t);
+      node = TT.n_prims > 0 ? TT.root4 : DONE;
+    }
+    while (node >= 0) {
+      float te[4];
+      int32_t ch[4];
+      node4_test<MIXED>(T, node, bf, bd, po, tf, h.t, te, ch);
+      cswap(te[0], ch[0], te[1], ch[1]);
+      cswap(te[2], ch[2], te[3], ch[3]);
+      cswap(te[0], ch[0], te[2], ch[2]);
+      cswap(te[1], ch[1], te[3], ch[3]);
+      cswap(te[1], ch[1], te[2], ch[2]);
+      if (ch[0] != CHILD_EMPTY) {
+        if (ch[3] != CHILD_EMPTY) E.S.push(ch[3]);
+        if (ch[2] != CHILD_EMPTY) E.S.push(ch[2]);
+        if (ch[1] != CHILD_EMPTY) E.S.push(ch[1]);
+        node = ch[0];
+      } else {
+        node = E.S.empty() ? DONE : E.S.pop();
+      }
+      if (node < 0 && node != DONE && leaf == DONE) {
+        leaf = node;
+        node = E.S.empty() ? DONE : E.S.pop();
+      }
+      if (__all(leaf != DONE)) break;
+    }
+    if (leaf == DONE && node != DONE) {
+      leaf = node;
+      node = E.S.empty() ? DONE : E.S.pop();
+    }
+    if (leaf != DONE) {
+      const int32_t first = leaf_first(leaf);
+      const int32_t cnt = leaf_count(leaf);
+      leaf = DONE;
+      for (int32_t j = 0; j < cnt; ++j)
+        tri_closest_hit<RAYFIRE>(T, first + j, ray, ORIENT_EXITING, E.hist, H.hn, h);
+      tf = f32_up(h.t);
+    }
+  }
+  flush_tally(P, R, E);
+}
+
 // One event round of the staged form: each live history of the sorted list
 // R.stage.idx[0 .. *R.stage.live) runs one event (grid-stride, so a wave's
 // lanes take neighbours in the sorted order: the same volume, nearby starts
@@ -778,6 +1060,13 @@ TransportLaunch transport_shape(bool mixed, int block, int lds_stack, int nvol)
     XDG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<false, false>, block, L.lds_bytes));
   if (nb <= 0) throw HipError{"transport launch shape does not fit one CU (block " + std::to_string(block) + ")"};
   L.grid = (int64_t)nb * cu;
+  int np = 0;
+  if (mixed)
+    XDG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&np, k_transport_pt<true>, block, L.lds_bytes));
+  else
+    XDG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&np, k_transport_pt<false>, block, L.lds_bytes));
+  L.grid_pt = (int64_t)std::max(np, 1) * cu;
+  L.grid = std::max(L.grid, L.grid_pt);  // the stack spill area covers either grid
   return L;
 }
 
@@ -798,11 +1087,15 @@ int64_t transport_start_grid(bool mixed, int stack_cap)
   XDG_HIP_CHECK(hipGetDevice(&dev));
   XDG_HIP_CHECK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
   const size_t sh = transport_start_lds(sb, stack_cap);
-  if (mixed)
+  int nb2 = 0;
+  if (mixed) {
     XDG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_piv<true>, sb, sh));
-  else
+    XDG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, k_transport_piv_pt<true>, sb, sh));
+  } else {
     XDG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport_piv<false>, sb, sh));
-  return (int64_t)std::max(nb, 1) * cu;
+    XDG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, k_transport_piv_pt<false>, sb, sh));
+  }
+  return (int64_t)std::max(std::min(nb, nb2), 1) * cu;
 }
 
 void launch_transport(const TransportDev& P, const TransportRun& R, bool mixed, const TransportLaunch& L,
@@ -844,10 +1137,17 @@ void launch_transport(const TransportDev& P, const TransportRun& R, bool mixed, 
       hipLaunchKernelGGL(k_transport_cand<true>, gn, dim3(256), 0, s, P, Q);
       const size_t sh = transport_start_lds(sb, R.stack_cap);
       const unsigned gp = (unsigned)std::max<int64_t>(1, R.start_grid);
-      if (mixed)
+      if (R.events_pt) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, R.stage.claim, (int64_t)1, 0ull);
+        if (mixed)
+          hipLaunchKernelGGL(k_transport_piv_pt<true>, dim3(gp), dim3(sb), sh, s, P, Q);
+        else
+          hipLaunchKernelGGL(k_transport_piv_pt<false>, dim3(gp), dim3(sb), sh, s, P, Q);
+      } else if (mixed) {
         hipLaunchKernelGGL(k_transport_piv<true>, dim3(gp), dim3(sb), sh, s, P, Q);
-      else
+      } else {
         hipLaunchKernelGGL(k_transport_piv<false>, dim3(gp), dim3(sb), sh, s, P, Q);
+      }
       const dim3 gs((unsigned)((Q.n + sb - 1) / sb));
       if (mixed)
         hipLaunchKernelGGL(k_transport_resolve<true>, gs, dim3(sb), sh, s, P, Q);
@@ -874,10 +1174,17 @@ void launch_transport(const TransportDev& P, const TransportRun& R, bool mixed, 
         else
           hipLaunchKernelGGL(k_transport_round<false>, dim3(gr), dim3(L.block), L.lds_bytes, s, P, Q);
       }
-      if (mixed)
+      if (R.events_pt) {
+        const unsigned gp = (unsigned)std::max<int64_t>(1, std::min(L.grid_pt, need));
+        if (mixed)
+          hipLaunchKernelGGL(k_transport_pt<true>, dim3(gp), dim3(L.block), L.lds_bytes, s, P, Q);
+        else
+          hipLaunchKernelGGL(k_transport_pt<false>, dim3(gp), dim3(L.block), L.lds_bytes, s, P, Q);
+      } else if (mixed) {
         hipLaunchKernelGGL((k_transport<true, true>), dim3(g), dim3(L.block), L.lds_bytes, s, P, Q);
-      else
+      } else {
         hipLaunchKernelGGL((k_transport<false, true>), dim3(g), dim3(L.block), L.lds_bytes, s, P, Q);
+      }
     } else {
       if (mixed)
         hipLaunchKernelGGL((k_transport<true, false>), dim3(g), dim3(L.block), L.lds_bytes, s, P, Q);
