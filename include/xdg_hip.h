@@ -99,7 +99,7 @@ int xdg_hip_get_launch(xdg_hip_ctx* ctx, int* block, int* stack);
  * binary LBVH; defer != 0 (default) traces far-origin mixed ray_fire rays in
  * a separate FP64 pass; persistent selects resident-grid kernels whose
  * lanes claim new work from a counter as they finish (bit 0 plain
- * ray_fire, bit 1 mixed ray_fire, bit 2 find_volume; default 3).  Results
+ * ray_fire, bit 1 mixed ray_fire, bit 2 find_volume; default 7).  Results
  * are identical either way. */
 int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent);
 /* Tuning options by name (no counterpart in the reference; results are
@@ -108,7 +108,7 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *   "stack"       minimum LDS stack entries per lane (default 1; raised to
  *                 what each tree needs)
  *   "persistent"  resident-grid kernels: bit 0 plain ray_fire, bit 1 mixed
- *                 ray_fire, bit 2 find_volume (default 3)
+ *                 ray_fire, bit 2 find_volume (default 7)
  *   "ray_order"   ray_fire batches of >= 65536 rays: 0 caller's order,
  *                 1 sorted by (tree, direction octant, origin Morton code),
  *                 2 as 1 plus one coarse direction bit per axis (0)
@@ -117,8 +117,10 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *   "wide"        BVH4 traversal (1) or the binary LBVH (0)
  *   "defer"       mixed ray_fire traces far-origin rays in an FP64 pass (1)
  *   "leaf_size"   max triangles per leaf for trees built afterwards (2)
- *   "refill"      persistent ray_fire and find_volume: idle lanes a wave
- *                 waits for before it claims new work (1..64, default 40)
+ *   "refill"      persistent ray_fire: idle lanes a wave waits for before
+ *                 it claims new work (1..64, default 40)
+ *   "fv_refill"   the same for the persistent find_volume (1..64; 0, the
+ *                 default: 56 mixed, 40 plain)
  *   "mw_wave"     mesh walk: particles that leave the mesh are parked and
  *                 their re-entry rays traced together in a separate kernel
  *                 (1), or each lane traces its own at once (0) (default 1)

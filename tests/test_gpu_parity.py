@@ -456,12 +456,12 @@ def test_nested_find_volume_and_fire(ctx, torch_mod):
             assert np.array_equal(bits(t[sel]), bits(rt))
 
 
-@pytest.mark.parametrize("persist", [True, False])
-def test_find_volume_far_points_and_variants(ctx, torch_mod, persist):
+@pytest.mark.parametrize("persist,fv_refill", [(True, 0), (False, 0), (True, 1), (True, 23), (True, 64)])
+def test_find_volume_far_points_and_variants(ctx, torch_mod, persist, fv_refill):
     """Points beyond the FP32 limit of the trees (deferred to the FP64 pass
-    by the persistent mixed kernel) mixed with near ones; persistent and
-    per-point kernels, both precisions, two volume orders, equal the
-    oracle."""
+    by the persistent mixed kernel) mixed with near ones; persistent (at the
+    default and at other refill thresholds) and per-point kernels, both
+    precisions, two volume orders, equal the oracle."""
     torch = torch_mod
     m = G.nested_spheres(2)
     vols = list(m.volumes)
@@ -475,6 +475,7 @@ def test_find_volume_far_points_and_variants(ctx, torch_mod, persist):
     pts[far] = pts[far] / 19.0 * (3.0 * lim)
     ref = O.find_volume(otrees, vols, m.ipc, pts, dirs)
     ctx.set_traversal(persistent=persist)
+    ctx.set_option("fv_refill", fv_refill)
     try:
         for prec in (0, 1):
             got = host(ctx.find_volume(trees, vols, m.ipc, dev(torch, pts), dev(torch, dirs), precision=prec))
@@ -488,6 +489,7 @@ def test_find_volume_far_points_and_variants(ctx, torch_mod, persist):
             assert np.array_equal(got2, ref2)
     finally:
         ctx.set_traversal()
+        ctx.set_option("fv_refill", 0)
 
 
 def test_global_surface_tree(ctx, torch_mod):

@@ -26,7 +26,7 @@ ERR_CAPPED = -5
 
 PLAIN = 0
 MIXED = 1
-PERSIST_DEFAULT = 3  # persistent ray_fire for both precisions
+PERSIST_DEFAULT = 7  # persistent ray_fire for both precisions, and find_volume
 ANY = -1
 EXITING = 0
 ENTERING = 1

@@ -485,15 +485,16 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
     delete c;
     return XDG_HIP_ERR_HIP;
   }
-  c->cfg.block = env_int("XDG_HIP_BLOCK", 64);
+  // environment overrides of the defaults in LaunchCfg
+  c->cfg.block = env_int("XDG_HIP_BLOCK", c->cfg.block);
   c->cfg.stack = env_int("XDG_HIP_STACK", 1);
-  c->leaf_max = env_int("XDG_HIP_LEAF", 2);
-  c->cfg.defer = env_int("XDG_HIP_DEFER", 1) != 0;
-  c->cfg.wide = env_int("XDG_HIP_WIDE", 1) != 0;
-  c->cfg.spec = env_int("XDG_HIP_SPEC", 1) != 0;
-  c->cfg.persistent = env_int("XDG_HIP_PERSIST", 3) & 7;
-  c->cfg.order = env_int("XDG_HIP_ORDER", 0);
-  c->cfg.refill = env_int("XDG_HIP_REFILL", 40);
+  c->leaf_max = env_int("XDG_HIP_LEAF", c->leaf_max);
+  c->cfg.defer = env_int("XDG_HIP_DEFER", c->cfg.defer) != 0;
+  c->cfg.wide = env_int("XDG_HIP_WIDE", c->cfg.wide) != 0;
+  c->cfg.spec = env_int("XDG_HIP_SPEC", c->cfg.spec) != 0;
+  c->cfg.persistent = env_int("XDG_HIP_PERSIST", c->cfg.persistent) & 7;
+  c->cfg.order = env_int("XDG_HIP_ORDER", c->cfg.order);
+  c->cfg.refill = env_int("XDG_HIP_REFILL", c->cfg.refill);
   try {
     sync_tables(c);
   } catch (...) {
@@ -578,6 +579,7 @@ int* opt_wide(xdg_hip_ctx* c) { return &c->cfg.wide; }
 int* opt_defer(xdg_hip_ctx* c) { return &c->cfg.defer; }
 int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
 int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
+int* opt_fv_refill(xdg_hip_ctx* c) { return &c->cfg.fv_refill; }
 int* opt_mw_wave(xdg_hip_ctx* c) { return &c->cfg.mw_wave; }
 int* opt_refit(xdg_hip_ctx* c) { return &c->refit; }
 int* opt_seg_order(xdg_hip_ctx* c) { return &c->cfg.seg_order; }
@@ -595,7 +597,7 @@ const OptionRef OPTIONS[] = {
     {"refit", 0, 1, opt_refit},            {"segment_order", 0, 2, opt_seg_order},
     {"transport_start", 0, 2, opt_tr_start}, {"transport_slice", 6, 24, opt_tr_slice},
     {"transport_items", 1, 64, opt_tr_items}, {"transport_rounds", 0, 64, opt_tr_rounds},
-    {"transport_events", 0, 1, opt_tr_events},
+    {"transport_events", 0, 1, opt_tr_events}, {"fv_refill", 0, 64, opt_fv_refill},
 };
 
 const OptionRef& find_option(const char* name)

@@ -154,10 +154,11 @@ struct LaunchCfg {
   int wide = 1;   // surface queries traverse the BVH4
   // resident-grid kernels with per-lane work refill: bit 0 plain ray_fire,
   // bit 1 mixed ray_fire, bit 2 find_volume
-  int persistent = 3;
+  int persistent = 7;
   int order = 0;  // ray_fire coherence ordering mode (0 = off)
   int spec = 1;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
-  int refill = 40; // persistent ray_fire / find_volume: idle lanes of a wave needed before a refill
+  int refill = 40; // persistent ray_fire: idle lanes of a wave needed before a refill
+  int fv_refill = 0;  // persistent find_volume: the same threshold (0: 56 mixed, 40 plain)
   int mw_wave = 1;  // mesh walk: batched re-entry rays (wavefront) instead of inline ones
   int seg_order = 0;  // segment walks in Morton order of the start points
   int tr_start = 2;   // transport starts: 0 in the event kernel, 1 separate kernels, 2 separate and sorted

@@ -390,8 +390,11 @@ __global__ void k_find_volume_deferred(FindVolumeArgs a, int cap)
 
 // Persistent find_volume over the BVH4: a lane holds one point and runs its
 // point_in_volume traversals volume after volume (a new traversal on the
-// same ray costs only a reset); it claims a new point, lazily as in
-// ray_fire, once the answer is known.
+// same ray costs only a reset), as a per-lane state machine like the
+// persistent ray_fire: lanes whose traversal has ended record the answer or
+// start the next volume, and claim new points, together once refill_min of
+// the wave are ready, while the other lanes keep descending (with
+// speculative leaf parking).
 template <bool MIXED>
 __device__ __forceinline__ void find_volume_pt_body(const FindVolumeArgs& a, int cap, int refill_min)
 {
@@ -400,7 +403,8 @@ __device__ __forceinline__ void find_volume_pt_body(const FindVolumeArgs& a, int
   const int lane = threadIdx.x & 63;
   const int32_t DONE = CHILD_EMPTY;
   int64_t pi = 0;
-  bool hasp = false, trav = false;
+  bool hasp = false;   // the lane holds a point whose volume is not known yet
+  bool traced = false; // a point_in_volume traversal of volume v is under way or ended
   int v = 0;
   TreeView T{nullptr, nullptr, nullptr, nullptr};
   RayD ray;
@@ -409,10 +413,17 @@ __device__ __forceinline__ void find_volume_pt_body(const FindVolumeArgs& a, int
   PlaneOfs po{0, 0, 0, 0, 0, 0};
   HitState h{INFTY, INT32_MAX, ID_NONE, false};
   float tf = INFINITY;
-  int32_t node = DONE;
+  int32_t node = DONE, leaf = DONE;
   while (true) {
-    if (trav && node == DONE) {  // a point_in_volume traversal ended
-      trav = false;
+    // lanes whose traversal has ended wait until refill_min of the wave (or
+    // all remaining ones) have too; then they act together: record or
+    // advance to the next volume, claim new points, start traversals
+    const bool done = node == DONE && leaf == DONE;
+    const unsigned long long dmask = __ballot(done);
+    const int nact = __popcll(__ballot(1));
+    const bool go = done && __popcll(dmask) >= (refill_min < nact ? refill_min : nact);
+    if (go && traced) {
+      traced = false;
       if (h.surf != ID_NONE && h.inside) {
         a.vol_out[pi] = a.vol_ids[v];
         hasp = false;
@@ -420,7 +431,40 @@ __device__ __forceinline__ void find_volume_pt_body(const FindVolumeArgs& a, int
         ++v;
       }
     }
-    if (hasp && !trav) {  // next volume of the same point
+    const bool need = go && !hasp;
+    const unsigned long long cmask = __ballot(need);
+    if (cmask) {
+      const int leader = __ffsll((long long)cmask) - 1;
+      unsigned long long b = 0;
+      if (lane == leader) b = atomicAdd(a.next, (unsigned long long)__popcll(cmask));
+      b = __shfl(b, leader);
+      if (need) {
+        pi = (int64_t)(b + (unsigned long long)__popcll(cmask & ((1ull << lane) - 1ull)));
+        if (pi >= a.n) break;
+        const double* d = a.dir ? a.dir + 3 * pi : a.dflt;
+        ray = make_ray(a.pts + 3 * pi, d);
+        if (MIXED) {
+          const float ox = (float)ray.o.x, oy = (float)ray.o.y, oz = (float)ray.o.z;
+          bf.ix = safe_inv_f((float)ray.d.x);
+          bf.iy = safe_inv_f((float)ray.d.y);
+          bf.iz = safe_inv_f((float)ray.d.z);
+          bf.nx = -ox * bf.ix;
+          bf.ny = -oy * bf.iy;
+          bf.nz = -oz * bf.iz;
+        } else {
+          bd.ix = safe_inv_d(ray.d.x);
+          bd.iy = safe_inv_d(ray.d.y);
+          bd.iz = safe_inv_d(ray.d.z);
+          bd.nx = -ray.o.x * bd.ix;
+          bd.ny = -ray.o.y * bd.iy;
+          bd.nz = -ray.o.z * bd.iz;
+        }
+        po = MIXED ? plane_ofs_f(bf) : plane_ofs_d(bd);
+        v = 0;
+        hasp = true;
+      }
+    }
+    if (go && hasp) {  // the point's next volume
       while (v < a.nvol && (a.vol_ids[v] == a.ipc || a.vol_trees[v] < 1 || a.vol_trees[v] > a.ntrees)) ++v;
       if (v >= a.nvol) {
         a.vol_out[pi] = a.ipc;
@@ -440,44 +484,9 @@ __device__ __forceinline__ void find_volume_pt_body(const FindVolumeArgs& a, int
           tf = INFINITY;
           S.reset();
           node = TT.n_prims > 0 ? TT.root4 : DONE;
-          trav = true;
+          traced = true;
         }
       }
-    }
-    const bool idle = !hasp;
-    const unsigned long long imask = __ballot(idle);
-    const int nact = __popcll(__ballot(1));
-    const bool go = __popcll(imask) >= (refill_min < nact ? refill_min : nact);
-    if (go && idle) {
-      const unsigned long long mask = imask;
-      const int leader = __ffsll((long long)mask) - 1;
-      unsigned long long b = 0;
-      if (lane == leader) b = atomicAdd(a.next, (unsigned long long)__popcll(mask));
-      b = __shfl(b, leader);
-      pi = (int64_t)(b + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
-      if (pi >= a.n) break;
-      const double* d = a.dir ? a.dir + 3 * pi : a.dflt;
-      ray = make_ray(a.pts + 3 * pi, d);
-      if (MIXED) {
-        const float ox = (float)ray.o.x, oy = (float)ray.o.y, oz = (float)ray.o.z;
-        bf.ix = safe_inv_f((float)ray.d.x);
-        bf.iy = safe_inv_f((float)ray.d.y);
-        bf.iz = safe_inv_f((float)ray.d.z);
-        bf.nx = -ox * bf.ix;
-        bf.ny = -oy * bf.iy;
-        bf.nz = -oz * bf.iz;
-      } else {
-        bd.ix = safe_inv_d(ray.d.x);
-        bd.iy = safe_inv_d(ray.d.y);
-        bd.iz = safe_inv_d(ray.d.z);
-        bd.nx = -ray.o.x * bd.ix;
-        bd.ny = -ray.o.y * bd.iy;
-        bd.nz = -ray.o.z * bd.iz;
-      }
-      po = MIXED ? plane_ofs_f(bf) : plane_ofs_d(bd);
-      v = 0;
-      hasp = true;
-      continue;  // the next iteration starts the first traversal
     }
     while (node >= 0) {
       float te[4];
@@ -496,13 +505,22 @@ __device__ __forceinline__ void find_volume_pt_body(const FindVolumeArgs& a, int
       } else {
         node = S.empty() ? DONE : S.pop();
       }
+      if (node < 0 && node != DONE && leaf == DONE) {
+        leaf = node;
+        node = S.empty() ? DONE : S.pop();
+      }
+      if (__all(leaf != DONE)) break;
     }
-    if (node != DONE) {
-      const int32_t first = leaf_first(node);
-      const int32_t cnt = leaf_count(node);
+    if (leaf == DONE && node != DONE) {
+      leaf = node;
+      node = S.empty() ? DONE : S.pop();
+    }
+    if (leaf != DONE) {
+      const int32_t first = leaf_first(leaf);
+      const int32_t cnt = leaf_count(leaf);
+      leaf = DONE;
       for (int32_t j = 0; j < cnt; ++j) tri_closest_hit<PIV>(T, first + j, ray, ORIENT_ANY, nullptr, 0, h);
       tf = f32_up(h.t);
-      node = S.empty() ? DONE : S.pop();
     }
   }
 }
@@ -719,9 +737,14 @@ void launch_point_in_volume(const PivArgs& a, bool mixed, const LaunchCfg& c, hi
 void launch_find_volume(const FindVolumeArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)
 {
   if (a.n <= 0) return;
-  // bit 2 of the persistent mask: the per-point kernel keeps a wave's lanes
-  // on the same tree volume after volume, which the resident grid loses
+  // bit 2 of the persistent mask: the resident grid whose lanes move on to
+  // their next volume or point while others still traverse (default), or
+  // the per-point kernel that keeps a wave's lanes on the same tree volume
+  // after volume but waits for its longest traversal at each.  The refill
+  // threshold that measured fastest differs by precision (nested spheres,
+  // 10 M points: mixed 7.7 ms at 56, plain 10.4 ms at 40-44).
   const bool persist = (c.persistent & 4) != 0;
+  const int refill = c.fv_refill ? c.fv_refill : (mixed ? 56 : 40);
   if (persist && c.wide && a.next && (!mixed || (a.fb_list && a.fb_count))) {
     const size_t sh = lds_bytes(c);
     const int64_t need = (a.n + c.block - 1) / c.block;
@@ -730,12 +753,12 @@ void launch_find_volume(const FindVolumeArgs& a, bool mixed, const LaunchCfg& c,
     if (mixed) {
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, a.fb_count);
       const int64_t gp = std::min(need, resident_grid(k_find_volume_pt_mixed, c.block, sh));
-      hipLaunchKernelGGL(k_find_volume_pt_mixed, dim3((unsigned)gp), b, sh, s, a, c.stack, c.refill);
+      hipLaunchKernelGGL(k_find_volume_pt_mixed, dim3((unsigned)gp), b, sh, s, a, c.stack, refill);
       const unsigned gd = (unsigned)std::min<int64_t>(need, 1024);
       hipLaunchKernelGGL(k_find_volume_deferred, dim3(gd), b, sh, s, a, c.stack);
     } else {
       const int64_t gp = std::min(need, resident_grid(k_find_volume_pt_plain, c.block, sh));
-      hipLaunchKernelGGL(k_find_volume_pt_plain, dim3((unsigned)gp), b, sh, s, a, c.stack, c.refill);
+      hipLaunchKernelGGL(k_find_volume_pt_plain, dim3((unsigned)gp), b, sh, s, a, c.stack, refill);
     }
     return;
   }
