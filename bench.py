@@ -414,7 +414,8 @@ def bench_transport(args, ctx, torch, stream, sh):
            "absorbed": int(st[1]), "leaked": int(st[2]),
            "lost_stuck_capped_error": int(st[3:].sum()),
            "total_track_length": float(tally.sum()), "config": cfg,
-           "launch": "captured hipGraph, one persistent kernel"}
+           "launch": "captured hipGraph: start stage (sample, sort, candidate volumes, "
+                     "point_in_volume items), then the persistent event kernel"}
     log(f"transport {res['mhistories_s']:.3f} Mhistories/s, {ev / avg / 1e6:.1f} Mevents/s, "
         f"status {st.tolist()}")
     return res
@@ -444,7 +445,8 @@ def bench_adjoint(args, ctx, torch, stream, sh):
     gd = torch.empty_like(dirs)
 
     def adj():
-        gv.zero_()
+        with torch.cuda.stream(stream):  # the zeroing is ordered before the atomics
+            gv.zero_()
         ctx.ray_distance_adjoint(org, dirs, t, f, w, verts, surf.tris, gv, go, gd, stream=sh)
 
     per, _ = time_launches(torch, stream, adj, max(2, args.steps // 2), 1)

@@ -121,6 +121,10 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *                 it claims new work (1..64, default 40)
  *   "fv_refill"   the same for the persistent find_volume (1..64; 0, the
  *                 default: 56 mixed, 40 plain)
+ *   "adjoint_sort" adjoint batches of >= 65536 rays with dldv: rays ordered
+ *                 by hit face and each wave's run on one face summed before
+ *                 the vertex atomics (1, default), or nine atomics per ray (0);
+ *                 dldv's FP64 summation order differs, ray gradients do not
  *   "mw_wave"     mesh walk: particles that leave the mesh are parked and
  *                 their re-entry rays traced together in a separate kernel
  *                 (1), or each lane traces its own at once (0) (default 1)
@@ -134,7 +138,7 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *                 cell and direction, with find_volume as point_in_volume
  *                 items grouped by volume (2, default)
  *   "transport_slice" staged starts run in slices of 2^value histories
- *                 (6..24, default 22; about 600 MB of start state per slice)
+ *                 (6..24, default 22: 0.8 to 1.3 GB of state per slice)
  *   "transport_items" start-stage item buffer entries per history (1..64,
  *                 default 8); histories whose items do not fit run the
  *                 volume loop themselves
@@ -467,7 +471,9 @@ int xdg_hip_transport_autotune(xdg_hip_ctx* ctx, int32_t handle, uint64_t first,
  * dldv[3 nv] += sum_i dldt[i] dt_i/dverts (FP64 atomics) and writes
  * dldo[3 n] = dldt dt/do, dldd[3 n] = dldt dt/dd (zero for misses), with
  * t = n.(v0 - o)/(n.d), n = (v1 - v0) x (v2 - v0).  All arrays device,
- * dldv/dldo/dldd nullable.  Asynchronous on stream. */
+ * dldv/dldo/dldd nullable.  Asynchronous on stream.  Batches of >= 65536
+ * rays with dldv use the context's ray-order scratch (see the
+ * "adjoint_sort" option; ordered across streams like the other users). */
 int xdg_hip_ray_distance_adjoint(xdg_hip_ctx* ctx, int64_t n, const double* org, const double* dir,
                                  const double* t, const int32_t* face, const double* dldt, int64_t nv,
                                  const double* verts, int64_t ntri, const int32_t* tris, double* dldv,

@@ -40,10 +40,15 @@ def problem():
     return m, org, dirs, w
 
 
-def test_adjoint_matches_oracle(ctx, torch_mod, problem):
+@pytest.mark.parametrize("adj_sort", [1, 0])
+def test_adjoint_matches_oracle(ctx, torch_mod, problem, adj_sort):
+    """Both forms of the kernel (rays ordered by hit face with per-wave
+    vertex sums, and one atomic set per ray), also with rays that missed
+    mixed in: ray gradients bit for bit, vertex gradients to 1e-12."""
     torch = torch_mod
     from xdg_amd import adjoint as A
     m, org, dirs, w = problem
+    ctx.set_option("adjoint_sort", adj_sort)
     surf = A.DifferentiableSurface(ctx, m.tris)
     verts = dev(torch, m.verts)
     tree = surf.tree_for(verts)
@@ -60,6 +65,16 @@ def test_adjoint_matches_oracle(ctx, torch_mod, problem):
     assert np.array_equal(bits(host(gd)), bits(od))
     g = host(gv)
     assert np.allclose(g, ov, rtol=1e-12, atol=1e-12 * np.abs(ov).max())
+    # every seventh ray reads as a miss
+    tm, fm = rt.copy(), rf.copy()
+    tm[::7], fm[::7] = np.inf, -1
+    gv.zero_()
+    ctx.ray_distance_adjoint(o, d, dev(torch, tm), dev(torch, fm), dev(torch, w), verts, surf.tris, gv, go, gd)
+    ov, oo, od = O.adjoint(m.verts, m.tris, org, dirs, tm, fm, w)
+    assert np.array_equal(bits(host(go)), bits(oo)) and np.array_equal(bits(host(gd)), bits(od))
+    assert not host(go)[::7].any()
+    assert np.allclose(host(gv), ov, rtol=1e-12, atol=1e-12 * np.abs(ov).max())
+    ctx.set_option("adjoint_sort", 1)
 
 
 def test_autograd_matches_central_differences(ctx, torch_mod, problem):

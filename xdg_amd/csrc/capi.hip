@@ -580,6 +580,7 @@ int* opt_defer(xdg_hip_ctx* c) { return &c->cfg.defer; }
 int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
 int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
 int* opt_fv_refill(xdg_hip_ctx* c) { return &c->cfg.fv_refill; }
+int* opt_adj_sort(xdg_hip_ctx* c) { return &c->cfg.adj_sort; }
 int* opt_mw_wave(xdg_hip_ctx* c) { return &c->cfg.mw_wave; }
 int* opt_refit(xdg_hip_ctx* c) { return &c->refit; }
 int* opt_seg_order(xdg_hip_ctx* c) { return &c->cfg.seg_order; }
@@ -598,6 +599,7 @@ const OptionRef OPTIONS[] = {
     {"transport_start", 0, 2, opt_tr_start}, {"transport_slice", 6, 24, opt_tr_slice},
     {"transport_items", 1, 64, opt_tr_items}, {"transport_rounds", 0, 64, opt_tr_rounds},
     {"transport_events", 0, 1, opt_tr_events}, {"fv_refill", 0, 64, opt_fv_refill},
+    {"adjoint_sort", 0, 1, opt_adj_sort},
 };
 
 const OptionRef& find_option(const char* name)
@@ -1476,8 +1478,16 @@ int xdg_hip_ray_distance_adjoint(xdg_hip_ctx* c, int64_t n, const double* org, c
     require(ntri == 0 || tris, "null tris");
     AdjointArgs a{n, org, dir, t, face, dldt, nv, verts, ntri, tris, dldv, dldo, dldd};
     if (!verts) a.ntri = 0;  // every ray reads as a miss without the mesh
-    launch_adjoint(a, 256, pick(c, stream));
-    XDG_HIP_CHECK(hipGetLastError());
+    const hipStream_t s = pick(c, stream);
+    if (c->cfg.adj_sort && dldv && n >= ORDER_MIN_RAYS) {
+      scratch_acquire(c, s);
+      launch_adjoint_sorted(a, order_scratch(c, n), s);
+      XDG_HIP_CHECK(hipGetLastError());
+      scratch_release(c, s);
+    } else {
+      launch_adjoint(a, 256, s);
+      XDG_HIP_CHECK(hipGetLastError());
+    }
     return XDG_HIP_OK;
   });
 }

@@ -159,10 +159,11 @@ struct LaunchCfg {
   int spec = 1;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
   int refill = 40; // persistent ray_fire: idle lanes of a wave needed before a refill
   int fv_refill = 0;  // persistent find_volume: the same threshold (0: 56 mixed, 40 plain)
+  int adj_sort = 1;   // adjoint batches of >= 65536 rays: ordered by hit face, vertex sums per wave
   int mw_wave = 1;  // mesh walk: batched re-entry rays (wavefront) instead of inline ones
   int seg_order = 0;  // segment walks in Morton order of the start points
   int tr_start = 2;   // transport starts: 0 in the event kernel, 1 separate kernels, 2 separate and sorted
-  int tr_slice = 22;  // transport, staged form: log2 of the histories per slice (about 600 MB of state at 22)
+  int tr_slice = 22;  // transport, staged form: log2 of the histories per slice (0.8-1.3 GB of state at 22)
   int tr_items = 8;   // transport, staged form: start-stage item buffer entries per history
   int tr_rounds = 0;  // transport, staged form: sorted one-event rounds before the event kernel takes the rest
   int tr_events = 1;  // transport, staged form: events by the traversal-level kernel (1) or whole-event kernel (0)
@@ -368,6 +369,9 @@ struct AdjointArgs {
   double* dldd;  // nullable, written
 };
 void launch_adjoint(const AdjointArgs& a, int block, hipStream_t s);
+// the same with the rays ordered by hit face first (w: ray-order scratch of
+// at least a.n entries); the vertex sums run per wave and face
+void launch_adjoint_sorted(const AdjointArgs& a, const RayOrderScratch& w, hipStream_t s);
 
 // ---- element queries (element_query.hip) ---------------------------------
 struct FindElementArgs {
