@@ -77,6 +77,8 @@ def parse():
     p.add_argument("--legs", default="",
                    help="comma list of legs to run alone (ray_fire, sphere, tet, transport, adjoint), "
                         "for profiling one workload at a time; prints a short JSON line")
+    p.add_argument("--segment-order", type=int, default=0,
+                   help="segment_order option for the tet walk (0 caller's order, 1 Morton of the start)")
     p.add_argument("--transport-config", default="",
                    help="block,lds_stack,chunk,refill for the transport leg instead of the autotuner")
     p.add_argument("--copy-probe", action="store_true",
@@ -263,6 +265,7 @@ def bench_tet(args, ctx, torch, stream, sh):
         v9.reshape(-1, 3), np.arange(3 * v9.shape[0], dtype=np.int32).reshape(-1, 3),
         f, s, fl, bm.bump(bm.ipc))
     ctx.attach_boundary(et, stree, mesh.bface_elem)
+    ctx.set_option("segment_order", args.segment_order)
     ns_ = args.segments
     p0 = torch.from_numpy(P.box_points(4, P.STREAM_POS, ns_, [0.0] * 3, [1.0] * 3)).to(dev)
     p1 = torch.from_numpy(P.box_points(4, P.STREAM_DIR, ns_, [0.0] * 3, [1.0] * 3)).to(dev)

@@ -1896,7 +1896,7 @@ int xdg_hip_transport_autotune(xdg_hip_ctx* c, int32_t handle, uint64_t first, i
       for (int block : {64, 128, 256})
         for (int ls : stacks)
           for (int chunk : {1, 4, 16})
-            for (int refill : {1, 32}) {
+            for (int refill : {1, 32, 48}) {
             t.block = block;
             t.lds_stack = ls;
             t.chunk = chunk;
