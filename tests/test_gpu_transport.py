@@ -253,6 +253,69 @@ def test_transport_sliced_graph_replays(ctx, torch_mod, lattice, start_form):
         ctx.transport_destroy(h)
 
 
+def test_transport_over_1024_volumes(ctx, start_form):
+    """11^3 spheres (subdivision 1) and their box: more volume slots than
+    the start stage keeps candidate masks and LDS counters for, so its
+    candidate passes test the boxes twice and reserve items with global
+    atomics per wave; every history and the tallies equal the oracle's."""
+    m = G.sphere_lattice(11, 1, 0.8, 2.0)
+    vols = [v for v in m.volumes if v != m.ipc]
+    assert len(vols) > 1024
+    trees = [build_tree(ctx, m, v) for v in vols]
+    otrees = [O.Tree.from_model(m, v) for v in vols]
+    hi = [22.0] * 3
+    h = ctx.transport_create(vols, trees, m.ipc, m.senses, SEED, STREAM, [0.0] * 3, hi, 2.0, 0.3, 10000)
+    try:
+        n, first = 3000, 11
+        r = O.transport(m, otrees, vols, SEED, STREAM, [0.0] * 3, hi, 2.0, 0.3, 10000, first, n)
+        assert (r["status"] == O.TR_LEAKED).sum() > 100
+        ctx.set_option("transport_start", 2)
+        for events in (1, 0):
+            ctx.set_option("transport_events", events)
+            _check(ctx.transport_run(h, first, n, len(vols)), r)
+    finally:
+        ctx.transport_destroy(h)
+        for t in trees:
+            ctx.release_surface_tree(t)
+
+
+def test_transport_prefilter_far_from_origin(ctx, start_form):
+    """A sphere 3e5 from the origin, its trees built with a 1e-3 bump (the
+    C-ABI takes any bump), and every start sampled 0.005 inside the
+    sphere's extreme vertex.  At that offset the vertex's x rounds up by
+    0.015 to the nearest FP32 value, past the start and the start stage's
+    slack; the trees' FP32 bounds are rounded outward, so the prefilter
+    keeps the sphere and every history starts in it, as in the oracle."""
+    base = G.sphere_lattice(1, 2, 0.8, 2.0)
+    off = 300000.0035  # float32(off + 0.2) - (off + 0.2) = 0.01525
+    verts = base.verts + off
+    m = G.SurfaceModel(verts, base.tris, base.tri_surf, base.surfaces, base.senses, base.volumes, ipc=base.ipc)
+    xmin = verts[np.unique(m.tris[m.surface_faces(1)]), 0].min()
+    assert float(np.float32(xmin)) - xmin > 0.01
+    vols = [v for v in m.volumes if v != m.ipc]
+    trees = []
+    for v in vols:
+        v9, f, sid, fl = m.tree_arrays(v)
+        nt = v9.shape[0]
+        trees.append(ctx.build_surface_tree(v9.reshape(-1, 3), np.arange(3 * nt, dtype=np.int32).reshape(-1, 3),
+                                            f, sid, fl, 1e-3))
+    otrees = [O.Tree.from_model(m, v) for v in vols]
+    p = np.array([xmin + 0.005, off + 1.0, off + 1.0])
+    lo, hi = list(p - 1e-7), list(p + 1e-7)
+    h = ctx.transport_create(vols, trees, m.ipc, m.senses, SEED, STREAM, lo, hi, 2.0, 0.3, 10000)
+    try:
+        n = 4000
+        r = O.transport(m, otrees, vols, SEED, STREAM, lo, hi, 2.0, 0.3, 10000, 0, n)
+        for staged in (2, 1, 0):
+            ctx.set_option("transport_start", staged)
+            out = ctx.transport_run(h, 0, n, len(vols))
+            _check(out, r)
+    finally:
+        ctx.transport_destroy(h)
+        for t in trees:
+            ctx.release_surface_tree(t)
+
+
 def test_transport_event_cap(ctx, lattice):
     h, ref = _problem(ctx, lattice, mfp=0.05, p_absorb=0.0, max_events=7)
     try:

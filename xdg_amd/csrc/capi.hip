@@ -195,6 +195,22 @@ hipStream_t pick(xdg_hip_ctx* c, void* s) { return s ? (hipStream_t)s : c->strea
 // bump * 2^20 - max|x| keeps it below 5/16 of the margin.  The 2^26 cap
 // keeps fl32(o) times the clamped reciprocal 1e30 of a near-axis direction
 // finite.  Rounded down to a float.
+// FP32 bounds rounded outward: the box holds every vertex exactly (the
+// transport's volume prefilter relies on that, whatever the coordinates'
+// magnitude)
+float f32_lo(double x)
+{
+  float f = (float)x;
+  if ((double)f > x) f = nextafterf(f, -INFINITY);
+  return f;
+}
+float f32_hi(double x)
+{
+  float f = (float)x;
+  if ((double)f < x) f = nextafterf(f, INFINITY);
+  return f;
+}
+
 float mixed_limit(double bump, double max_abs)
 {
   const double lim = std::min(bump * 1048576.0 - max_abs, 67108864.0);
@@ -734,8 +750,8 @@ int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
           if (i == 0 || x < lo) lo = x;
           if (i == 0 || x > hi) hi = x;
         }
-        t.dev.box_lo[k] = (float)lo;
-        t.dev.box_hi[k] = (float)hi;
+        t.dev.box_lo[k] = f32_lo(lo);
+        t.dev.box_hi[k] = f32_hi(hi);
       }
       // FP32 box test error for origin o: at most (3u max|x| + 5u |o|) / |d_i|
       // in ray-parameter units (u = 2^-24) against a margin of bump / |d_i|,
@@ -832,8 +848,8 @@ int xdg_hip_build_element_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
           if (v == 0 || x < lo) lo = x;
           if (v == 0 || x > hi) hi = x;
         }
-        t.dev.box_lo[k] = (float)lo;
-        t.dev.box_hi[k] = (float)hi;
+        t.dev.box_lo[k] = f32_lo(lo);
+        t.dev.box_hi[k] = f32_hi(hi);
       }
       t.dev.bface_elem = nullptr;
       t.n_nodes = lb.n_nodes;
@@ -956,8 +972,8 @@ int xdg_hip_refit_surface_tree(xdg_hip_ctx* c, int32_t tree, int64_t n_verts, co
         XDG_HIP_CHECK(hipMemcpyAsync(h, cb, sizeof h, hipMemcpyDeviceToHost, s));
         XDG_HIP_CHECK(hipStreamSynchronize(s));
         for (int k = 0; k < 3; ++k) {
-          t.dev.box_lo[k] = (float)ord_val(h[k]);
-          t.dev.box_hi[k] = (float)ord_val(h[3 + k]);
+          t.dev.box_lo[k] = f32_lo(ord_val(h[k]));
+          t.dev.box_hi[k] = f32_hi(ord_val(h[3 + k]));
         }
         if (n == 1) {  // no internal nodes: the refit kept no max |x|
           double mx = 0.0;

@@ -273,44 +273,55 @@ __device__ __forceinline__ void load_start(const TransportStage& G, int64_t i, D
 // reserves (SCATTER true) the wave's items for it.  When the stage keeps
 // candidate masks (few enough chunks), the counting pass saves each lane's
 // masks and the scatter pass reads them instead of testing again.
-template <bool SCATTER>
-__global__ __launch_bounds__(256) void k_transport_cand(TransportDev P, TransportRun R)
+//
+// With at most CAND_LDS_SLOTS volumes the counts and reservations go
+// through LDS first: the workgroup's waves add into per-slot LDS counters,
+// one lane per slot adds the workgroup's total to the global counter (the
+// scatter pass reserves the workgroup's range there), and the waves then
+// take their places from the LDS cursor.  A volume that nearly every ray
+// reaches (the box around a lattice) then sees one global atomic per
+// workgroup instead of one per wave.
+constexpr int CAND_BLOCK = 1024;
+constexpr int CAND_LDS_SLOTS = 1024;
+
+// one chunk's candidate mask of this lane: read from the kept masks
+// (from_masks), or tested and then kept if the stage keeps masks
+__device__ __forceinline__ uint64_t cand_mask(const TransportDev& P, const TransportStage& G, bool from_masks,
+                                              bool keep, bool open, int v0, int cn, int64_t j, int lane,
+                                              double* wbox, const D3& r, const D3& u, const D3& inv)
 {
-  __shared__ double sbox[256 * 6];
-  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const bool valid = j < R.n;  // lanes past the end still take part in the wave's steps
-  const int lane = threadIdx.x & 63;
-  double* wbox = sbox + (size_t)(threadIdx.x >> 6) * (64 * 6);
-  const TransportStage& G = R.stage;
-  const int64_t i = valid ? (int64_t)G.idx[j] : 0;
-  D3 r = d3(0, 0, 0), u = d3(0, 0, 1);
-  bool open = false;
-  if (valid && G.st[i] == TR_ALIVE) {
-    load_start(G, i, r, u);
-    open = true;
+  uint64_t m = 0;
+  if (from_masks) {
+    if (open) m = G.masks[(int64_t)(v0 >> 6) * G.cap + j];
+    return m;
   }
-  const D3 inv = d3(u.x != 0.0 ? 1.0 / u.x : 0.0, u.y != 0.0 ? 1.0 / u.y : 0.0, u.z != 0.0 ? 1.0 / u.z : 0.0);
-  if (__ballot(open) == 0) return;  // wave-uniform
-  const bool keep = G.mask_words > 0;
+  __builtin_amdgcn_wave_barrier();
+  if (lane < cn) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) wbox[6 * lane + k] = P.pre_box[6 * (int64_t)(v0 + lane) + k];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (open)
+    for (int q = 0; q < cn; ++q)
+      if (may_hit_box(wbox + 6 * q, r, u, inv)) m |= 1ull << q;
+  if (keep && open) G.masks[(int64_t)(v0 >> 6) * G.cap + j] = m;
+  __builtin_amdgcn_wave_barrier();
+  return m;
+}
+
+// MODE 0: count into the global counters; 1: count into the LDS counters;
+// 2: reserve from the global cursors and write items; 3: reserve from the
+// LDS cursors and write items
+template <int MODE>
+__device__ __forceinline__ void cand_walk(const TransportDev& P, const TransportStage& G, bool from_masks,
+                                          bool keep, bool open, int64_t i, int64_t j, int lane, double* wbox,
+                                          uint32_t* lcnt, const D3& r, const D3& u, const D3& inv)
+{
   for (int v0 = 0; v0 < P.nvol; v0 += 64) {
     const int cn = P.nvol - v0 < 64 ? P.nvol - v0 : 64;
-    uint64_t m = 0;
-    if (SCATTER && keep) {
-      if (open) m = G.masks[(int64_t)(v0 >> 6) * G.cap + j];
-    } else {
-      __builtin_amdgcn_wave_barrier();
-      if (lane < cn) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) wbox[6 * lane + k] = P.pre_box[6 * (int64_t)(v0 + lane) + k];
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (open)
-        for (int q = 0; q < cn; ++q)
-          if (may_hit_box(wbox + 6 * q, r, u, inv)) m |= 1ull << q;
-      if (keep && open) G.masks[(int64_t)(v0 >> 6) * G.cap + j] = m;
-    }
+    uint64_t m = cand_mask(P, G, from_masks, keep, open, v0, cn, j, lane, wbox, r, u, inv);
     while (true) {
       const int lq = m ? __ffsll((long long)m) - 1 : 64;
       const int q = wave_min_i(lq);
@@ -319,11 +330,15 @@ __global__ __launch_bounds__(256) void k_transport_cand(TransportDev P, Transpor
       const unsigned long long bal = __ballot(mine);
       const int cnt = __popcll(bal);
       const int leader = __ffsll((long long)bal) - 1;
-      if (!SCATTER) {
+      if (MODE == 0) {
         if (lane == leader) atomicAdd(P.slot_cnt + v0 + q, (uint32_t)cnt);
+      } else if (MODE == 1) {
+        if (lane == leader) atomicAdd(lcnt + v0 + q, (uint32_t)cnt);
       } else {
         uint32_t base = 0;
-        if (lane == leader) base = P.slot_cnt[v0 + q] + atomicAdd(P.slot_cur + v0 + q, (uint32_t)cnt);
+        if (lane == leader)
+          base = MODE == 2 ? P.slot_cnt[v0 + q] + atomicAdd(P.slot_cur + v0 + q, (uint32_t)cnt)
+                           : atomicAdd(lcnt + v0 + q, (uint32_t)cnt);
         base = __shfl(base, leader);
         if (mine) {
           const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(bal & ((1ull << lane) - 1ull));
@@ -335,8 +350,51 @@ __global__ __launch_bounds__(256) void k_transport_cand(TransportDev P, Transpor
       }
       if (mine) m &= m - 1ull;
     }
-    __builtin_amdgcn_wave_barrier();
   }
+}
+
+template <bool SCATTER>
+__global__ __launch_bounds__(CAND_BLOCK) void k_transport_cand(TransportDev P, TransportRun R)
+{
+  __shared__ double sbox[CAND_BLOCK * 6];
+  __shared__ uint32_t lcnt[CAND_LDS_SLOTS];
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool valid = j < R.n;  // lanes past the end still take part in the wave's and workgroup's steps
+  const int lane = threadIdx.x & 63;
+  double* wbox = sbox + (size_t)(threadIdx.x >> 6) * (64 * 6);
+  const TransportStage& G = R.stage;
+  const int64_t i = valid ? (int64_t)G.idx[j] : 0;
+  D3 r = d3(0, 0, 0), u = d3(0, 0, 1);
+  bool open = false;
+  if (valid && G.st[i] == TR_ALIVE) {
+    load_start(G, i, r, u);
+    open = true;
+  }
+  const D3 inv = d3(u.x != 0.0 ? 1.0 / u.x : 0.0, u.y != 0.0 ? 1.0 / u.y : 0.0, u.z != 0.0 ? 1.0 / u.z : 0.0);
+  const bool any = __ballot(open) != 0;  // wave-uniform
+  const bool keep = G.mask_words > 0;
+  // block-uniform: LDS aggregation, which the scatter pass does from the
+  // kept masks only (its count and write walks must see the same masks)
+  const bool blk = P.nvol <= CAND_LDS_SLOTS && (!SCATTER || keep);
+  // the scatter pass reads the masks the counting pass kept
+  const bool from_masks = SCATTER && keep;
+  if (!blk) {
+    if (any) cand_walk<SCATTER ? 2 : 0>(P, G, from_masks, keep, open, i, j, lane, wbox, lcnt, r, u, inv);
+    return;
+  }
+  for (int q = threadIdx.x; q < P.nvol; q += blockDim.x) lcnt[q] = 0u;
+  __syncthreads();
+  if (any) cand_walk<1>(P, G, from_masks, keep, open, i, j, lane, wbox, lcnt, r, u, inv);
+  __syncthreads();
+  if (!SCATTER) {
+    for (int q = threadIdx.x; q < P.nvol; q += blockDim.x)
+      if (lcnt[q]) atomicAdd(P.slot_cnt + q, lcnt[q]);
+    return;
+  }
+  for (int q = threadIdx.x; q < P.nvol; q += blockDim.x)
+    if (lcnt[q]) lcnt[q] = P.slot_cnt[q] + atomicAdd(P.slot_cur + q, lcnt[q]);
+  __syncthreads();
+  if (any) cand_walk<3>(P, G, true, keep, open, i, j, lane, wbox, lcnt, r, u, inv);
 }
 
 // point_in_volume of each (history, volume slot) item, grid-stride over the
@@ -1132,9 +1190,10 @@ void launch_transport(const TransportDev& P, const TransportRun& R, bool mixed, 
       const dim3 gv((unsigned)((P.nvol + 256) / 256));
       hipLaunchKernelGGL(k_fill_u32, gv, dim3(256), 0, s, P.slot_cnt, (int64_t)P.nvol + 1, 0u);
       hipLaunchKernelGGL(k_fill_u32, gv, dim3(256), 0, s, P.slot_cur, (int64_t)P.nvol, 0u);
-      hipLaunchKernelGGL(k_transport_cand<false>, gn, dim3(256), 0, s, P, Q);
+      const dim3 gc((unsigned)((Q.n + CAND_BLOCK - 1) / CAND_BLOCK));
+      hipLaunchKernelGGL(k_transport_cand<false>, gc, dim3(CAND_BLOCK), 0, s, P, Q);
       launch_scan_excl(P.slot_cnt, P.nvol, P.slot_cnt + P.nvol, s);
-      hipLaunchKernelGGL(k_transport_cand<true>, gn, dim3(256), 0, s, P, Q);
+      hipLaunchKernelGGL(k_transport_cand<true>, gc, dim3(CAND_BLOCK), 0, s, P, Q);
       const size_t sh = transport_start_lds(sb, R.stack_cap);
       const unsigned gp = (unsigned)std::max<int64_t>(1, R.start_grid);
       if (R.events_pt) {

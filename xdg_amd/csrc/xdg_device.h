@@ -512,7 +512,8 @@ struct SurfTreeDev {
   // face_normal of each triangle (leaf order, 3 doubles), computed on the
   // device at build time with the same expression as the reference
   const double* normals;
-  // bounds of the tree's vertices (ray ordering only; not used by queries)
+  // bounds of the tree's vertices, rounded outward to FP32 (ray ordering
+  // keys, and the transport start stage's conservative volume prefilter)
   float box_lo[3];
   float box_hi[3];
 };
