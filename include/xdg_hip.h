@@ -128,6 +128,10 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *   "mw_wave"     mesh walk: particles that leave the mesh are parked and
  *                 their re-entry rays traced together in a separate kernel
  *                 (1), or each lane traces its own at once (0) (default 1)
+ *   "mw_refill"   mesh walk with mw_wave 1: the parking rounds run on a
+ *                 resident grid whose lanes claim the next particles once
+ *                 this many lanes of a wave are idle (1..64, default 32),
+ *                 or one particle per lane (0)
  *   "refit"       surface trees built while set keep their topology, so
  *                 xdg_hip_refit_surface_tree can move their vertices (0)
  *   "segment_order" segment batches of >= 65536: walked in the Morton order

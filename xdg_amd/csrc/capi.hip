@@ -391,6 +391,7 @@ const MeshWalkScratch& mw_scratch_for(xdg_hip_ctx* c, int64_t n)
     c->mw_w.list[0] = (int32_t*)(c->d_mw + sv);
     c->mw_w.list[1] = (int32_t*)(c->d_mw + sv + sl);
     c->mw_w.count = (unsigned int*)(c->d_mw + sv + 2 * sl);
+    c->mw_w.claim = (unsigned long long*)(c->mw_w.count + 2 * MW_ROUNDS + 2);  // 8-byte aligned, within the 256 bytes
   }
   return c->mw_w;
 }
@@ -598,6 +599,7 @@ int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
 int* opt_fv_refill(xdg_hip_ctx* c) { return &c->cfg.fv_refill; }
 int* opt_adj_sort(xdg_hip_ctx* c) { return &c->cfg.adj_sort; }
 int* opt_mw_wave(xdg_hip_ctx* c) { return &c->cfg.mw_wave; }
+int* opt_mw_refill(xdg_hip_ctx* c) { return &c->cfg.mw_refill; }
 int* opt_refit(xdg_hip_ctx* c) { return &c->refit; }
 int* opt_seg_order(xdg_hip_ctx* c) { return &c->cfg.seg_order; }
 int* opt_tr_start(xdg_hip_ctx* c) { return &c->cfg.tr_start; }
@@ -611,6 +613,7 @@ const OptionRef OPTIONS[] = {
     {"ray_order", 0, 2, opt_order},         {"speculative", 0, 1, opt_spec}, {"wide", 0, 1, opt_wide},
     {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
     {"refill", 1, 64, opt_refill},          {"mw_wave", 0, 1, opt_mw_wave},
+    {"mw_refill", 0, 64, opt_mw_refill},
     {"refit", 0, 1, opt_refit},            {"segment_order", 0, 2, opt_seg_order},
     {"transport_start", 0, 2, opt_tr_start}, {"transport_slice", 6, 24, opt_tr_slice},
     {"transport_items", 1, 64, opt_tr_items}, {"transport_rounds", 0, 64, opt_tr_rounds},

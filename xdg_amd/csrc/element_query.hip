@@ -796,8 +796,92 @@ __global__ __launch_bounds__(256) void k_mw_walk(MeshWalkArgs a, int cap, MwSave
   mw_finish(a, i, S);
 }
 
+// The parking walk on a resident grid: a lane whose particle ended or was
+// parked waits until `refill` lanes of its wave (or all remaining ones) are
+// idle; those lanes then claim the next particles of the list together
+// (one atomic on *claim) and start them (sampling and find_element, or the
+// saved state), while the other lanes keep stepping.  A wave therefore no
+// longer runs until its longest particle ends, and the long start path runs
+// for a batch of lanes at a time.  Events are those of k_mw_walk.
+// The ballots of an iteration are turned into each lane's claim position
+// inside the wave-uniform claim block, and the position is pinned to a
+// vector register there.  The compiler may run the stepping lanes in an
+// inner loop that the claiming lanes leave early; a wave-uniform (scalar)
+// mask read after that loop would hold a later iteration's value, whereas a
+// lane's vector registers keep its own.
+// Parked particles are listed by claim position: out_list[pos] holds the
+// particle of position pos if it parked, else -1, and *out_count is set to
+// the number of positions.  A shared append counter would take one atomic
+// with return per parking iteration, and those serialise on one address.
+template <int FO>
+__global__ __launch_bounds__(256) void k_mw_walk_pt(MeshWalkArgs a, int cap, MwSave* save, const int32_t* in_list,
+                                                    const unsigned int* in_count, int32_t* out_list,
+                                                    unsigned int* out_count, unsigned long long* claim, int refill)
+{
+  extern __shared__ int32_t lds[];
+  const int64_t n_in = in_count ? (int64_t)*in_count : a.n;
+  int32_t* stk = lds + threadIdx.x;
+  const ElemTreeDev T = a.etable[a.tree];
+  const bool has_ipc = T.ipc_tree >= 1 && T.ipc_tree <= a.nsurf_trees;
+  const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = (unsigned int)n_in;
+  bool have = false;
+  int64_t i = 0, cpos = 0;
+  MwState S;
+  S.st = MW_ALIVE;
+  ElemLoad L;
+  while (true) {
+    const bool idle = !have;
+    const unsigned long long imask = __ballot(idle);
+    const int nact = __popcll(__ballot(1));
+    const bool go = __popcll(imask) >= (refill < nact ? refill : nact);
+    const unsigned long long mask = go ? imask : 0ull;
+    int64_t pos = -1;  // this lane's claimed position, -1 when it claims none
+    if (mask) {        // wave-uniform: every lane of the wave takes part
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long b = 0;
+      if (lane == leader) b = atomicAdd(claim, (unsigned long long)__popcll(mask));
+      const unsigned long long base = __shfl(b, leader);
+      if (idle) pos = (int64_t)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
+    }
+    asm volatile("" : "+v"(pos));
+    if (pos >= n_in) break;
+    if (pos >= 0) {
+      cpos = pos;
+      i = in_list ? (int64_t)in_list[pos] : pos;
+      if (in_list)
+        mw_load(save[i], a.first + (uint64_t)i, S);
+      else
+        mw_start(a, T, i, S, stk, cap);
+      if (S.st == MW_ALIVE) {
+        elem_load<FO>(T, S.e, L);
+        have = true;
+      } else {
+        mw_finish(a, i, S);
+        out_list[cpos] = -1;
+      }
+    }
+    if (have) {
+      mw_step<FO>(a, T, S, L);
+      if (S.st == MW_ALIVE && S.e == ID_NONE && has_ipc) {
+        mw_store(save[i], S);
+        out_list[cpos] = (int32_t)i;
+        have = false;
+      } else {
+        mw_close(a, S);
+        if (S.st != MW_ALIVE) {
+          mw_finish(a, i, S);
+          out_list[cpos] = -1;
+          have = false;
+        }
+      }
+    }
+  }
+}
+
 // Re-entry kernel of the wavefront: the parked particles' re-entry rays;
-// particles back in the mesh go on in out_list.
+// particles back in the mesh go on in out_list.  Negative entries of
+// in_list are positions of the resident walk whose particle did not park.
 __global__ void k_mw_reenter(MeshWalkArgs a, int cap, MwSave* save, const int32_t* in_list,
                              const unsigned int* in_count, int32_t* out_list, unsigned int* out_count)
 {
@@ -805,6 +889,7 @@ __global__ void k_mw_reenter(MeshWalkArgs a, int cap, MwSave* save, const int32_
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j >= (int64_t)*in_count) return;
   const int64_t i = in_list[j];
+  if (i < 0) return;
   int32_t* stk = lds + threadIdx.x;
   int32_t* prims = lds + (size_t)cap * blockDim.x + MW_PRIMS * threadIdx.x;
   const ElemTreeDev T = a.etable[a.tree];
@@ -868,12 +953,18 @@ void launch_mesh_walk_t(const MeshWalkArgs& a, const LaunchCfg& c, const MeshWal
   }
   // rounds of walk -> batched re-entry; the last walk re-enters inline, so
   // every particle ends whatever the number of re-entries
-  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, w->count, 2 * MW_ROUNDS);
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, w->count, MW_COUNT_WORDS);
   const int32_t* in_list = nullptr;
   const unsigned int* in_count = nullptr;
+  const size_t shp = sizeof(int32_t) * (size_t)c.stack * block;  // no re-entry primitives in the resident walk
+  const int64_t gpt = c.mw_refill ? std::min<int64_t>(g.x, resident_grid(k_mw_walk_pt<FO>, block, shp)) : 0;
   for (int r = 0; r < MW_ROUNDS; ++r) {
-    hipLaunchKernelGGL((k_mw_walk<FO, false>), g, b, sh, s, a, c.stack, w->save, in_list, in_count, w->list[0],
-                       w->count + 2 * r);
+    if (c.mw_refill)
+      hipLaunchKernelGGL(k_mw_walk_pt<FO>, dim3((unsigned)std::max<int64_t>(gpt, 1)), b, shp, s, a, c.stack, w->save,
+                         in_list, in_count, w->list[0], w->count + 2 * r, w->claim + r, c.mw_refill);
+    else
+      hipLaunchKernelGGL((k_mw_walk<FO, false>), g, b, sh, s, a, c.stack, w->save, in_list, in_count, w->list[0],
+                         w->count + 2 * r);
     hipLaunchKernelGGL(k_mw_reenter, g, b, sh, s, a, c.stack, w->save, (const int32_t*)w->list[0],
                        (const unsigned int*)(w->count + 2 * r), w->list[1], w->count + 2 * r + 1);
     // the next walk reads list 1 and parks into list 0 again; the stream

@@ -161,6 +161,7 @@ struct LaunchCfg {
   int fv_refill = 0;  // persistent find_volume: the same threshold (0: 56 mixed, 40 plain)
   int adj_sort = 1;   // adjoint batches of >= 65536 rays: ordered by hit face, vertex sums per wave
   int mw_wave = 1;  // mesh walk: batched re-entry rays (wavefront) instead of inline ones
+  int mw_refill = 32;  // mesh walk rounds on a resident grid, lanes claiming particles in batches of this many (0: off)
   int seg_order = 0;  // segment walks in Morton order of the start points
   int tr_start = 2;   // transport starts: 0 in the event kernel, 1 separate kernels, 2 separate and sorted
   int tr_slice = 22;  // transport, staged form: log2 of the histories per slice (0.8-1.3 GB of state at 22)
@@ -459,8 +460,10 @@ constexpr int MW_ROUNDS = 3;  // walk / batched re-entry rounds before the inlin
 struct MeshWalkScratch {
   MwSave* save;               // n particles
   int32_t* list[2];           // parked and re-entered particle lists, n each
-  unsigned int* count;        // 2 * MW_ROUNDS list counters
+  unsigned int* count;        // 2 * MW_ROUNDS list counters, then the claim counters
+  unsigned long long* claim;  // MW_ROUNDS claim counters of the resident-grid walk (inside count's words)
 };
+constexpr int MW_COUNT_WORDS = 2 * MW_ROUNDS + 2 + 2 * MW_ROUNDS;  // 32-bit words zeroed per launch
 // w (nullable): scratch of the wavefront walk; null runs every particle
 // inline, one per lane
 void launch_mesh_walk(const MeshWalkArgs& a, int face_order_kind, const LaunchCfg& c, const MeshWalkScratch* w,
@@ -480,5 +483,21 @@ void launch_tri_normals(const xdgd::TriRec* tris, int64_t n, double* out, hipStr
 // leaf-ordered triangle records
 void launch_write_tris(const double* verts9, const int32_t* face, const int32_t* surf, const uint8_t* flip,
                        const int32_t* order, int64_t n, xdgd::TriRec* out, hipStream_t s);
+
+// Resident workgroups of a kernel on the whole device for (block, lds).
+template <class K>
+inline int64_t resident_grid(K kernel, int block, size_t lds)
+{
+  static int cu = 0;
+  if (cu == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
+      cu = 256;
+  }
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, lds) != hipSuccess || nb <= 0) nb = 1;
+  return (int64_t)nb * cu;
+}
 
 }  // namespace xdgh

@@ -651,26 +651,6 @@ inline size_t lds_bytes(const LaunchCfg& c) { return sizeof(int32_t) * (size_t)c
 
 }  // namespace
 
-namespace {
-
-// resident blocks of a kernel on the whole device (cached per shape)
-template <class K>
-int64_t resident_grid(K kernel, int block, size_t lds)
-{
-  static int cu = 0;
-  if (cu == 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
-      cu = 256;
-  }
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, lds) != hipSuccess || nb <= 0) nb = 1;
-  return (int64_t)nb * cu;
-}
-
-}  // namespace
-
 void launch_ray_fire(const RayFireArgs& a, bool mixed, const LaunchCfg& c, hipStream_t s)
 {
   if (a.n <= 0) return;
