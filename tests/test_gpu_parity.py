@@ -287,6 +287,64 @@ def test_persistent_refill_thresholds(ctx, torch_mod, sphere5, refill):
         ctx.set_option("refill", 40)  # the default
 
 
+@pytest.mark.parametrize("refill", [1, 40])
+def test_persistent_ray_fire_many_per_lane(ctx, torch_mod, sphere5, refill):
+    """2^21 rays, several for each lane of the resident grid, so lanes claim
+    new rays while others of their wave are still tracing: persistent
+    ray_fire in both precisions equals the per-ray kernel bit for bit, and
+    its first 20000 rays equal the oracle."""
+    torch = torch_mod
+    m, tree, ot = sphere5
+    n, k = 1 << 21, 20000
+    org = P.ball_points(37, P.STREAM_POS, n, 9.0)
+    dirs = P.iso_dirs(37, P.STREAM_DIR, n)
+    o_d, d_d = dev(torch, org), dev(torch, dirs)
+    rt, rs, rf = ot.ray_fire(org[:k], dirs[:k])
+    try:
+        for prec in (0, 1):
+            ctx.set_traversal(persistent=False)
+            ref = [host(x) for x in ctx.ray_fire(tree, o_d, d_d, precision=prec)]
+            ctx.set_traversal(persistent=True)
+            ctx.set_option("refill", refill)
+            got = [host(x) for x in ctx.ray_fire(tree, o_d, d_d, precision=prec)]
+            assert np.array_equal(bits(got[0]), bits(ref[0]))
+            assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2])
+            assert np.array_equal(bits(got[0][:k]), bits(rt))
+            assert np.array_equal(got[1][:k], rs) and np.array_equal(got[2][:k], rf)
+    finally:
+        ctx.set_traversal()
+        ctx.set_option("refill", 40)  # the default
+
+
+@pytest.mark.parametrize("fv_refill", [0, 1])
+def test_persistent_find_volume_many_per_lane(ctx, torch_mod, fv_refill):
+    """2^21 points, several for each lane of the resident grid: persistent
+    find_volume (lanes moving on to the next volume and claiming new points
+    while others trace) equals the per-point kernel in both precisions, and
+    its first 20000 points equal the oracle."""
+    torch = torch_mod
+    m = G.nested_spheres(2)
+    vols = list(m.volumes)
+    trees = [build_tree(ctx, m, v) for v in vols]
+    n, k = 1 << 21, 20000
+    pts = P.box_points(43, P.STREAM_POS, n, [-19] * 3, [19] * 3)
+    dirs = P.iso_dirs(43, P.STREAM_DIR, n)
+    ref_k = O.find_volume([O.Tree.from_model(m, v) for v in vols], vols, m.ipc, pts[:k], dirs[:k])
+    p_d, d_d = dev(torch, pts), dev(torch, dirs)
+    try:
+        ctx.set_option("fv_refill", fv_refill)
+        for prec in (0, 1):
+            ctx.set_traversal(persistent=False)
+            ref = host(ctx.find_volume(trees, vols, m.ipc, p_d, d_d, precision=prec))
+            ctx.set_traversal(persistent=True)
+            got = host(ctx.find_volume(trees, vols, m.ipc, p_d, d_d, precision=prec))
+            assert np.array_equal(got, ref)
+            assert np.array_equal(got[:k], ref_k)
+    finally:
+        ctx.set_traversal()
+        ctx.set_option("fv_refill", 0)
+
+
 @pytest.mark.parametrize("wide", [True, False])
 def test_nested_piv_occluded_variants(ctx, torch_mod, wide):
     """Binary and 4-wide traversal, plain and mixed, give identical

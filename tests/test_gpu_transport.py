@@ -182,6 +182,34 @@ def test_transport_start_forms(ctx, lattice, prec, start_form):
         ctx.transport_destroy(h)
 
 
+def test_transport_forms_many_per_lane(ctx, lattice, start_form):
+    """2^20 histories, several for each lane of the resident event grids, so
+    lanes claim new histories (and start-stage items) while others of their
+    wave are still tracing: the traversal-level event kernel, the
+    whole-event kernel and starts inside the event kernel give the same
+    histories and tallies, and the first 5000 histories equal the oracle."""
+    h, ref = _problem(ctx, lattice)
+    try:
+        n, first, k = 1 << 20, 11, 5000
+        r = ref(first, k)
+        outs = []
+        for staged, events in [(2, 1), (2, 0), (0, 0)]:
+            ctx.set_option("transport_start", staged)
+            ctx.set_option("transport_events", events)
+            out = ctx.transport_run(h, first, n, len(lattice[1]))
+            outs.append({key: host(v) for key, v in out.items()})
+        for o in outs[1:]:
+            for key in outs[0]:
+                assert np.array_equal(o[key], outs[0][key]), key
+        o = outs[0]
+        assert set(np.unique(o["status"])) <= {1, 2}
+        assert np.array_equal(o["events"][:k], r["events"])
+        assert np.array_equal(o["status"][:k], r["status"])
+        assert np.array_equal(o["hash"][:k].view(np.uint64), r["hash"])
+    finally:
+        ctx.transport_destroy(h)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_transport_event_rounds(ctx, lattice, prec, start_form):
     """Sorted one-event rounds before the event kernel (none, one, a few, and

@@ -272,8 +272,8 @@ class Context:
         self._check(lib().xdg_hip_set_launch(self.h, block, stack))
 
     def set_traversal(self, wide=True, defer=True, persistent=None):
-        """persistent: None for the default (ray_fire, both precisions),
-        True / False for every kernel, or the raw mask (bit 0 plain
+        """persistent: None for the default (PERSIST_DEFAULT, every
+        kernel), True / False for every kernel, or the raw mask (bit 0 plain
         ray_fire, bit 1 mixed ray_fire, bit 2 find_volume)."""
         if persistent is None:
             mask = PERSIST_DEFAULT
