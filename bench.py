@@ -300,6 +300,30 @@ def bench_tet(args, ctx, torch, stream, sh):
         roof.update(traffic=tr, traffic_source="profiles/tet_walk_kuhn64_counters.json "
                     "(rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-probe scaled)",
                     traffic_over_algorithmic=round(tr / (seg_bytes * ns_), 3))
+    # The walk's binding ceiling is the rate of random-address int64 atomics
+    # (one tally update per element step), which no published figure gives:
+    # measured here on an array of the tally's size with torch's index_add_
+    # (one no-return 64-bit atomic per index, uniform random indices).
+    na = 1 << 27
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    aidx = torch.randint(0, tally.numel(), (na,), device=dev, generator=gen)
+    asrc = torch.ones(na, dtype=torch.int64, device=dev)
+    atally = torch.zeros_like(tally)
+
+    def atomics():
+        with torch.cuda.stream(stream):
+            atally.index_add_(0, aidx, asrc)
+
+    per_a, _ = time_launches(torch, stream, atomics, 3, 1)
+    ceil = na / (sum(per_a) / len(per_a)) / 1e9
+    steps_s = ns_ * steps_per_seg / avg_w / 1e9
+    roof["atomics"] = {"achieved": round(steps_s, 2), "ceiling": round(ceil, 2), "unit": "G int64 atomics/s",
+                       "frac": round(steps_s / ceil, 4),
+                       "ceiling_source": "measured in this run: index_add_ of 2^27 ones at uniform random "
+                                         "indices into an int64 array of the tally's size"}
+    roof["binding"] = "atomics"
+    del aidx, asrc, atally
     tet["roofline"] = roof
     tet["_p"] = (p0, p1, cnt, hsh, mesh)
     log(f"tet walk {tet['msegs_s']:.2f} Msegs/s, {steps_per_seg:.1f} elements/segment")
