@@ -345,6 +345,45 @@ def test_persistent_find_volume_many_per_lane(ctx, torch_mod, fv_refill):
         ctx.set_option("fv_refill", 0)
 
 
+def test_tree_build_modes(ctx, torch_mod):
+    """Trees built by the binned-SAH hierarchy and by the Karras radix tree
+    answer ray_fire (both precisions, both widths) and point_in_volume
+    identically, and both equal the oracle; both trees stay within the 64
+    levels the traversal stacks allow."""
+    torch = torch_mod
+    m = G.nested_spheres(3)
+    vols = [v for v in m.volumes if v != m.ipc]
+    n = 30000
+    org = P.box_points(53, P.STREAM_POS, n, [-16] * 3, [16] * 3)
+    dirs = P.iso_dirs(53, P.STREAM_DIR, n)
+    o_d, d_d = dev(torch, org), dev(torch, dirs)
+    keep = ctx.get_option("build")
+    try:
+        for v in vols:
+            ot = O.Tree.from_model(m, v)
+            rt, rs, rf = ot.ray_fire(org, dirs)
+            rin = ot.point_in_volume(org, dirs)
+            got = {}
+            for mode in (1, 0):
+                ctx.set_option("build", mode)
+                tree = build_tree(ctx, m, v)
+                got[mode] = ctx.tree_info(tree)["depth"]
+                for wide in (True, False):
+                    ctx.set_traversal(wide=wide)
+                    for prec in (0, 1):
+                        t, s_, f = ctx.ray_fire(tree, o_d, d_d, precision=prec)
+                        assert np.array_equal(bits(host(t)), bits(rt))
+                        assert np.array_equal(host(s_), rs) and np.array_equal(host(f), rf)
+                        pin = host(ctx.point_in_volume(tree, o_d, d_d, precision=prec))
+                        assert np.array_equal(pin.astype(bool), rin.astype(bool))
+                ctx.set_traversal()
+                ctx.release_surface_tree(tree)
+            assert got[1] <= 64 and got[0] <= 64
+    finally:
+        ctx.set_option("build", keep)
+        ctx.set_traversal()
+
+
 @pytest.mark.parametrize("wide", [True, False])
 def test_nested_piv_occluded_variants(ctx, torch_mod, wide):
     """Binary and 4-wide traversal, plain and mixed, give identical

@@ -260,6 +260,13 @@ class Context:
             raise XdgHipError(rc, msg.decode() if msg else "")
         return rc
 
+    def _fence(self, stream):
+        """Without an explicit stream, wait for torch's current stream once
+        the wrapper's own allocations (a zero fill, an upload) are queued:
+        the context's stream is not ordered with it."""
+        if stream is None:
+            torch.cuda.current_stream(self.dev).synchronize()
+
     def _done(self, rc, stream):
         """Check a query's status; without an explicit stream the call ran on
         the context's own stream, so wait for it before torch reads."""
@@ -668,6 +675,7 @@ class Context:
                       out=None, per_history=True, stream=None):
         if out is None:
             out = self.transport_outputs(nvol, n, per_history)
+            self._fence(stream)
         self._transport_out(out, nvol, n)
         self._done(lib().xdg_hip_transport_run(
             self.h, handle, int(first), int(n), precision,
@@ -685,6 +693,7 @@ class Context:
             _ptr(out.get("status")), _ptr(out.get("hash")), C.byref(g)))
         return g.value
 
+    @_ordered
     def graph_launch(self, graph, stream=None):
         self._check(lib().xdg_hip_graph_launch(self.h, graph, stream))
 
@@ -728,6 +737,7 @@ class Context:
                               device=self.dev).contiguous()
         n = inp.shape[0]
         out = torch.zeros((n, width_out), dtype=torch.float64, device=self.dev)
+        self._fence(stream)
         self._check(lib().xdg_hip_eval_primitive(self.h, kind, n, _ptr(inp),
                                                  _ptr(out), stream))
         self.synchronize(stream)

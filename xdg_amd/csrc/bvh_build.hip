@@ -458,6 +458,296 @@ inline unsigned grid_for(int64_t n, int blk = BLK)
   return (unsigned)std::max<int64_t>(1, g);
 }
 
+// ---------------------------------------------------------------------------
+// Binned-SAH topology, the alternative to the Karras hierarchy: top down, one
+// level of tasks at a time.  A task is an internal node whose primitives
+// occupy the positions [start, start + cnt) of the order array; one
+// workgroup per task bins the centroids into SAH_BINS bins per axis (counts
+// and FP32 primitive bounds, LDS atomics, so the result does not depend on
+// the order of the atomics), takes the split of least
+// area(left) * n_left + area(right) * n_right (ties: lower axis, then lower
+// bin) and partitions its range stably in place.  Children of two or more
+// primitives are the next level's tasks, and node ids are handed out in task
+// order by a scan, so the tree is deterministic.  Tasks whose centroids no
+// bin boundary separates, and every task from SAH_MEDIAN_LEVEL on, split at
+// the object median of their current order (which keeps the depth within the
+// 64 levels the traversal stacks allow).  The result takes the Karras
+// kernel's form -- children as internal ids or ~position, ranges, parents --
+// so refit, emission, validation and the BVH4 collapse are shared.
+// ---------------------------------------------------------------------------
+constexpr int SAH_BINS = 32;
+constexpr int SAH_THREADS = 256;
+constexpr int SAH_MEDIAN_LEVEL = 36;
+
+struct SahTask {
+  int32_t node, start, cnt, pad;
+};
+
+// monotone map float -> uint32 (ordered-int trick)
+__device__ __forceinline__ unsigned int ordf(float x)
+{
+  const unsigned int b = __float_as_uint(x);
+  return (b >> 31) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float unordf(unsigned int u)
+{
+  const unsigned int b = (u >> 31) ? (u & 0x7fffffffu) : ~u;
+  return __uint_as_float(b);
+}
+
+__device__ __forceinline__ int sah_bin(double c, double lo, double scale)
+{
+  const int b = (int)((c - lo) * scale);
+  return b < 0 ? 0 : (b >= SAH_BINS ? SAH_BINS - 1 : b);
+}
+
+__device__ __forceinline__ float half_area(const float* b)
+{
+  const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+  return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ void k_sah_root(SahTask* tasks, int32_t* rfirst, int32_t* rlast, int32_t* pint, int64_t n)
+{
+  tasks[0] = SahTask{0, 0, (int32_t)n, 0};
+  rfirst[0] = 0;
+  rlast[0] = (int32_t)(n - 1);
+  pint[0] = -1;
+}
+
+__global__ __launch_bounds__(SAH_THREADS) void k_sah_split(const double* __restrict__ box6, int32_t* idx,
+                                                          int32_t* tmp, const SahTask* __restrict__ tasks,
+                                                          int32_t* nleft, int median)
+{
+  __shared__ double red[6][SAH_THREADS / 64];
+  __shared__ double cb[6];
+  __shared__ unsigned int lcnt[3][SAH_BINS];
+  __shared__ unsigned int lbox[3][SAH_BINS][6];
+  __shared__ float cost[SAH_THREADS];
+  __shared__ int cand[SAH_THREADS];
+  __shared__ unsigned int wsum[SAH_THREADS / 64];
+  __shared__ int s_axis, s_k, s_nl;
+  const SahTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (t.cnt <= 2 || median) {
+    if (tid == 0) nleft[blockIdx.x] = t.cnt / 2;
+    return;
+  }
+  // centroid bounds of the task
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int p = tid; p < t.cnt; p += SAH_THREADS) {
+    const double* b = box6 + 6 * (int64_t)idx[t.start + p];
+    for (int k = 0; k < 3; ++k) {
+      const double c = 0.5 * (b[k] + b[3 + k]);
+      mn[k] = fmin(mn[k], c);
+      mx[k] = fmax(mx[k], c);
+    }
+  }
+  for (int k = 0; k < 3; ++k)
+    for (int off = 32; off > 0; off >>= 1) {
+      mn[k] = fmin(mn[k], __shfl_xor(mn[k], off));
+      mx[k] = fmax(mx[k], __shfl_xor(mx[k], off));
+    }
+  if (lane == 0)
+    for (int k = 0; k < 3; ++k) {
+      red[k][wv] = mn[k];
+      red[3 + k][wv] = mx[k];
+    }
+  for (int i = tid; i < 3 * SAH_BINS; i += SAH_THREADS) {
+    lcnt[i / SAH_BINS][i % SAH_BINS] = 0u;
+    for (int j = 0; j < 3; ++j) {
+      lbox[i / SAH_BINS][i % SAH_BINS][j] = ordf(INFINITY);
+      lbox[i / SAH_BINS][i % SAH_BINS][3 + j] = ordf(-INFINITY);
+    }
+  }
+  __syncthreads();
+  if (tid < 6) {
+    double v = red[tid][0];
+    for (int w = 1; w < SAH_THREADS / 64; ++w) v = tid < 3 ? fmin(v, red[tid][w]) : fmax(v, red[tid][w]);
+    cb[tid] = v;
+  }
+  __syncthreads();
+  double lo[3], scale[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = cb[k];
+    const double ext = cb[3 + k] - cb[k];
+    scale[k] = ext > 0.0 ? SAH_BINS / ext : 0.0;
+  }
+  // bin counts and FP32 bounds (the cost needs no more)
+  for (int p = tid; p < t.cnt; p += SAH_THREADS) {
+    const double* b = box6 + 6 * (int64_t)idx[t.start + p];
+    unsigned int fb[6];
+    for (int j = 0; j < 6; ++j) fb[j] = ordf((float)b[j]);
+    for (int k = 0; k < 3; ++k) {
+      const int bin = sah_bin(0.5 * (b[k] + b[3 + k]), lo[k], scale[k]);
+      atomicAdd(&lcnt[k][bin], 1u);
+      for (int j = 0; j < 3; ++j) {
+        atomicMin(&lbox[k][bin][j], fb[j]);
+        atomicMax(&lbox[k][bin][3 + j], fb[3 + j]);
+      }
+    }
+  }
+  __syncthreads();
+  // one candidate per thread: axis ax, left = bins [0, k)
+  float cst = INFINITY;
+  if (tid < 3 * (SAH_BINS - 1)) {
+    const int ax = tid / (SAH_BINS - 1), k = tid % (SAH_BINS - 1) + 1;
+    if (scale[ax] > 0.0) {
+      unsigned int nl = 0, nr = 0;
+      float l[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      for (int b = 0; b < SAH_BINS; ++b) {
+        const unsigned int c = lcnt[ax][b];
+        if (!c) continue;
+        float* d = b < k ? l : r;
+        if (b < k)
+          nl += c;
+        else
+          nr += c;
+        for (int j = 0; j < 3; ++j) {
+          d[j] = fminf(d[j], unordf(lbox[ax][b][j]));
+          d[3 + j] = fmaxf(d[3 + j], unordf(lbox[ax][b][3 + j]));
+        }
+      }
+      if (nl && nr) cst = half_area(l) * (float)nl + half_area(r) * (float)nr;
+    }
+  }
+  cost[tid] = cst;
+  cand[tid] = tid;
+  __syncthreads();
+  for (int st = SAH_THREADS / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      const float a = cost[tid], b = cost[tid + st];
+      if (b < a || (b == a && cand[tid + st] < cand[tid])) {
+        cost[tid] = b;
+        cand[tid] = cand[tid + st];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (cost[0] < INFINITY) {
+      const int ax = cand[0] / (SAH_BINS - 1), k = cand[0] % (SAH_BINS - 1) + 1;
+      unsigned int nl = 0;
+      for (int b = 0; b < k; ++b) nl += lcnt[ax][b];
+      s_axis = ax;
+      s_k = k;
+      s_nl = (int)nl;
+    } else {
+      s_axis = -1;
+      s_k = 0;
+      s_nl = t.cnt / 2;
+    }
+    nleft[blockIdx.x] = s_nl;
+  }
+  __syncthreads();
+  const int ax = s_axis, ks = s_k, nl = s_nl;
+  if (ax < 0) return;  // no bin boundary separates the centroids: the object median, order kept
+  // stable partition of the range into tmp, then back
+  int lrun = 0;
+  for (int base = 0; base < t.cnt; base += SAH_THREADS) {
+    const int pr = base + tid;
+    int prim = 0, f = 0;
+    if (pr < t.cnt) {
+      prim = idx[t.start + pr];
+      const double* b = box6 + 6 * (int64_t)prim;
+      f = sah_bin(0.5 * (b[ax] + b[3 + ax]), lo[ax], scale[ax]) < ks ? 1 : 0;
+    }
+    const unsigned long long bal = __ballot(f);
+    const int wpre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wv] = (unsigned int)__popcll(bal);
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int w = 0; w < SAH_THREADS / 64; ++w) {
+      if (w < wv) before += (int)wsum[w];
+      tot += (int)wsum[w];
+    }
+    if (pr < t.cnt) {
+      const int lpre = lrun + before + wpre;
+      tmp[t.start + (f ? lpre : nl + (pr - lpre))] = prim;
+    }
+    lrun += tot;
+    __syncthreads();
+  }
+  for (int pr = tid; pr < t.cnt; pr += SAH_THREADS) idx[t.start + pr] = tmp[t.start + pr];
+}
+
+__global__ void k_sah_count(const SahTask* __restrict__ tasks, int64_t nt, const int32_t* __restrict__ nleft,
+                            uint32_t* ci)
+{
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const int32_t nl = nleft[i], nr = tasks[i].cnt - nl;
+  ci[i] = (nl >= 2 ? 1u : 0u) + (nr >= 2 ? 1u : 0u);
+}
+
+__global__ void k_sah_link(const SahTask* __restrict__ tasks, int64_t nt, const int32_t* __restrict__ nleft,
+                           const uint32_t* __restrict__ off, int32_t next_id, int32_t* left, int32_t* right,
+                           int32_t* rfirst, int32_t* rlast, int32_t* pint, int32_t* pleaf, SahTask* out)
+{
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const SahTask t = tasks[i];
+  const int32_t nl = nleft[i], nr = t.cnt - nl;
+  uint32_t o = off[i];
+  int32_t id = next_id + (int32_t)o;
+  if (nl >= 2) {
+    left[t.node] = id;
+    pint[id] = t.node;
+    rfirst[id] = t.start;
+    rlast[id] = t.start + nl - 1;
+    out[o++] = SahTask{id, t.start, nl, 0};
+    ++id;
+  } else {
+    left[t.node] = ~t.start;
+    pleaf[t.start] = t.node;
+  }
+  const int32_t rs = t.start + nl;
+  if (nr >= 2) {
+    right[t.node] = id;
+    pint[id] = t.node;
+    rfirst[id] = rs;
+    rlast[id] = rs + nr - 1;
+    out[o] = SahTask{id, rs, nr, 0};
+  } else {
+    right[t.node] = ~rs;
+    pleaf[rs] = t.node;
+  }
+}
+
+// Binned-SAH hierarchy over the n >= 2 primitives listed in order (Morton
+// order on entry, the leaf order on return), into the topology arrays t.
+void sah_topology(const double* box6, int64_t n, int32_t* order, const LbvhTopo& t, hipStream_t s,
+                  std::vector<DevBuf>& tmp)
+{
+  const int64_t cap = n / 2 + 1;  // tasks hold >= 2 primitives each, disjoint
+  auto* ta = (SahTask*)dev_alloc(sizeof(SahTask) * cap, tmp);
+  auto* tb = (SahTask*)dev_alloc(sizeof(SahTask) * cap, tmp);
+  auto* nl = (int32_t*)dev_alloc(sizeof(int32_t) * cap, tmp);
+  auto* ci = (uint32_t*)dev_alloc(sizeof(uint32_t) * cap, tmp);
+  auto* tot = (uint32_t*)dev_alloc(sizeof(uint32_t), tmp);
+  auto* scratch = (int32_t*)dev_alloc(sizeof(int32_t) * n, tmp);
+  hipLaunchKernelGGL(k_sah_root, dim3(1), dim3(1), 0, s, ta, t.rfirst, t.rlast, t.pint, n);
+  int64_t nt = 1;
+  int32_t next_id = 1;
+  for (int level = 0; nt > 0; ++level) {
+    hipLaunchKernelGGL(k_sah_split, dim3((unsigned)nt), dim3(SAH_THREADS), 0, s, box6, order, scratch, ta, nl,
+                       level >= SAH_MEDIAN_LEVEL ? 1 : 0);
+    hipLaunchKernelGGL(k_sah_count, dim3(grid_for(nt)), dim3(BLK), 0, s, ta, nt, nl, ci);
+    launch_scan_excl(ci, nt, tot, s);
+    hipLaunchKernelGGL(k_sah_link, dim3(grid_for(nt)), dim3(BLK), 0, s, ta, nt, nl, ci, next_id, t.left, t.right,
+                       t.rfirst, t.rlast, t.pint, t.pleaf, tb);
+    uint32_t h = 0;
+    XDG_HIP_CHECK(hipMemcpyAsync(&h, tot, sizeof(h), hipMemcpyDeviceToHost, s));
+    XDG_HIP_CHECK(hipStreamSynchronize(s));
+    if ((int64_t)h > cap || (int64_t)next_id + h > n - 1) throw HipError{"sah: task count out of range"};
+    next_id += (int32_t)h;
+    nt = (int64_t)h;
+    std::swap(ta, tb);
+  }
+  if (next_id != n - 1) throw HipError{"sah: internal node count differs from n - 1"};
+}
+
 
 // ---------------------------------------------------------------------------
 // Structural check of the emitted binary tree, breadth-first from the root:
@@ -588,7 +878,7 @@ void launch_scan_excl(uint32_t* a, int64_t m, uint32_t* total, hipStream_t s)
 }
 
 void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std::vector<DevBuf>& owned,
-                LbvhOut& out, bool keep_topology)
+                LbvhOut& out, bool keep_topology, bool sah)
 {
   if (leaf_max < 1) leaf_max = 1;
   if (leaf_max > 8) leaf_max = 8;
@@ -640,8 +930,11 @@ void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std:
       hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ni)), dim3(BLK), 0, s, t.flags, ni, 0u);
       hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ni)), dim3(BLK), 0, s, (unsigned int*)t.pint, ni, 0xffffffffu);
       hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(BLK), 0, s, (unsigned int*)t.pleaf, n, 0xffffffffu);
-      hipLaunchKernelGGL(k_karras, dim3(grid_for(ni)), dim3(BLK), 0, s, keys, n, t.left, t.right, t.rfirst, t.rlast,
-                         t.pint, t.pleaf);
+      if (sah)
+        sah_topology(box6, n, out.order, t, s, tmp);
+      else
+        hipLaunchKernelGGL(k_karras, dim3(grid_for(ni)), dim3(BLK), 0, s, keys, n, t.left, t.right, t.rfirst,
+                           t.rlast, t.pint, t.pleaf);
       hipLaunchKernelGGL(k_refit, dim3(grid_for(n)), dim3(BLK), 0, s, box6, out.order, n, t.left, t.right, t.pint,
                          t.pleaf, t.flags, t.ibox);
       hipLaunchKernelGGL(k_emit, dim3(grid_for(ni)), dim3(BLK), 0, s, box6, out.order, n, leaf_max, t.left, t.right,

@@ -134,6 +134,7 @@ struct xdg_hip_ctx {
   hipEvent_t scratch_ev = nullptr;
   LaunchCfg cfg;
   int leaf_max = 2;
+  int build_sah = 1;  // trees built next: binned-SAH hierarchy (1) or the Karras radix tree (0)
   int refit = 0;  // surface trees built next keep what a refit needs
   int max_surf_depth = 1;
   int max_surf_stack4 = 1;
@@ -506,6 +507,7 @@ int xdg_hip_create(xdg_hip_ctx** out, int device)
   c->cfg.block = env_int("XDG_HIP_BLOCK", c->cfg.block);
   c->cfg.stack = env_int("XDG_HIP_STACK", 1);
   c->leaf_max = env_int("XDG_HIP_LEAF", c->leaf_max);
+  c->build_sah = env_int("XDG_HIP_BUILD", c->build_sah);
   c->cfg.defer = env_int("XDG_HIP_DEFER", c->cfg.defer) != 0;
   c->cfg.wide = env_int("XDG_HIP_WIDE", c->cfg.wide) != 0;
   c->cfg.spec = env_int("XDG_HIP_SPEC", c->cfg.spec) != 0;
@@ -595,6 +597,7 @@ int* opt_spec(xdg_hip_ctx* c) { return &c->cfg.spec; }
 int* opt_wide(xdg_hip_ctx* c) { return &c->cfg.wide; }
 int* opt_defer(xdg_hip_ctx* c) { return &c->cfg.defer; }
 int* opt_leaf(xdg_hip_ctx* c) { return &c->leaf_max; }
+int* opt_build(xdg_hip_ctx* c) { return &c->build_sah; }
 int* opt_refill(xdg_hip_ctx* c) { return &c->cfg.refill; }
 int* opt_fv_refill(xdg_hip_ctx* c) { return &c->cfg.fv_refill; }
 int* opt_adj_sort(xdg_hip_ctx* c) { return &c->cfg.adj_sort; }
@@ -612,6 +615,7 @@ const OptionRef OPTIONS[] = {
     {"block", 64, 1024, opt_block},         {"stack", 1, 64, opt_stack},     {"persistent", 0, 7, opt_persistent},
     {"ray_order", 0, 2, opt_order},         {"speculative", 0, 1, opt_spec}, {"wide", 0, 1, opt_wide},
     {"defer", 0, 1, opt_defer},             {"leaf_size", 1, 8, opt_leaf},
+    {"build", 0, 1, opt_build},
     {"refill", 1, 64, opt_refill},          {"mw_wave", 0, 1, opt_mw_wave},
     {"mw_refill", 0, 64, opt_mw_refill},
     {"refit", 0, 1, opt_refit},            {"segment_order", 0, 2, opt_seg_order},
@@ -715,7 +719,7 @@ int xdg_hip_build_surface_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
       launch_tri_boxes(d_v9, n_tris, bump, d_box, s);
       const bool keep = c->refit != 0;
       LbvhOut lb;
-      lbvh_build(d_box, n_tris, c->leaf_max, s, t.bufs, lb, keep);
+      lbvh_build(d_box, n_tris, c->leaf_max, s, t.bufs, lb, keep, c->build_sah != 0);
       TriRec* d_tris = (TriRec*)dev_alloc(nt * sizeof(TriRec), t.bufs);
       launch_write_tris(d_v9, d_face, d_surf, d_flip, lb.order, n_tris, d_tris, s);
       double* d_nrm = (double*)dev_alloc(nt * 3 * sizeof(double), t.bufs);
@@ -832,7 +836,7 @@ int xdg_hip_build_element_tree(xdg_hip_ctx* c, int64_t n_verts, const double* ve
         XDG_HIP_CHECK(hipMemcpyAsync(d_elems, recs.data(), (size_t)n_tets * sizeof(ElemRec), hipMemcpyHostToDevice, s));
       launch_elem_boxes(d_elems, n_tets, d_box, s);
       LbvhOut lb;
-      lbvh_build(d_box, n_tets, c->leaf_max, s, t.bufs, lb);
+      lbvh_build(d_box, n_tets, c->leaf_max, s, t.bufs, lb, false, c->build_sah != 0);
       XDG_HIP_CHECK(hipGetLastError());
       XDG_HIP_CHECK(hipStreamSynchronize(s));
       t.dev.nodes_d = lb.nodes_d;

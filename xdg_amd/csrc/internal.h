@@ -73,10 +73,12 @@ void launch_scan_excl(uint32_t* a, int64_t m, uint32_t* total, hipStream_t s);
 
 // box6: device array of n boxes (lo.xyz, hi.xyz), already dilated.  The
 // whole build runs on the device: 63-bit Morton codes, a stable LSD radix
-// sort, the Karras hierarchy, a bottom-up refit and a structural check by
-// breadth-first levels; only counts and the max |coordinate| are read back.
+// sort, the hierarchy (the Karras radix tree over the Morton order, or with
+// sah a top-down binned-SAH split of that order, level by level), a
+// bottom-up refit and a structural check by breadth-first levels; only
+// counts and the max |coordinate| are read back.
 void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std::vector<DevBuf>& owned,
-                LbvhOut& out, bool keep_topology = false);
+                LbvhOut& out, bool keep_topology = false, bool sah = false);
 // New boxes for the same primitives (same order and connectivity): node
 // boxes recomputed bottom-up, topology unchanged (needs out.topo).
 void lbvh_refit(const double* box6, int64_t n, LbvhOut& out, hipStream_t s);
