@@ -118,7 +118,7 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *   "defer"       mixed ray_fire traces far-origin rays in an FP64 pass (1)
  *   "leaf_size"   max triangles per leaf for trees built afterwards (2)
  *   "refill"      persistent ray_fire: idle lanes a wave waits for before
- *                 it claims new work (1..64, default 40)
+ *                 it claims new work (1..64, default 44)
  *   "fv_refill"   the same for the persistent find_volume (1..64; 0, the
  *                 default: 56 mixed, 40 plain)
  *   "adjoint_sort" adjoint batches of >= 65536 rays with dldv: rays ordered

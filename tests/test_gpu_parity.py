@@ -274,6 +274,7 @@ def test_persistent_refill_thresholds(ctx, torch_mod, sphere5, refill):
     org = P.ball_points(31, P.STREAM_POS, n, 9.0)
     dirs = P.iso_dirs(31, P.STREAM_DIR, n)
     rt, rs, rf = ot.ray_fire(org, dirs)
+    keep = ctx.get_option("refill")
     ctx.set_traversal(persistent=True)
     ctx.set_option("refill", refill)
     try:
@@ -284,10 +285,10 @@ def test_persistent_refill_thresholds(ctx, torch_mod, sphere5, refill):
             assert np.array_equal(bits(host(t)), bits(rt))
     finally:
         ctx.set_traversal()
-        ctx.set_option("refill", 40)  # the default
+        ctx.set_option("refill", keep)
 
 
-@pytest.mark.parametrize("refill", [1, 40])
+@pytest.mark.parametrize("refill", [1, 44])
 def test_persistent_ray_fire_many_per_lane(ctx, torch_mod, sphere5, refill):
     """2^21 rays, several for each lane of the resident grid, so lanes claim
     new rays while others of their wave are still tracing: persistent
@@ -300,6 +301,7 @@ def test_persistent_ray_fire_many_per_lane(ctx, torch_mod, sphere5, refill):
     dirs = P.iso_dirs(37, P.STREAM_DIR, n)
     o_d, d_d = dev(torch, org), dev(torch, dirs)
     rt, rs, rf = ot.ray_fire(org[:k], dirs[:k])
+    keep = ctx.get_option("refill")
     try:
         for prec in (0, 1):
             ctx.set_traversal(persistent=False)
@@ -313,7 +315,7 @@ def test_persistent_ray_fire_many_per_lane(ctx, torch_mod, sphere5, refill):
             assert np.array_equal(got[1][:k], rs) and np.array_equal(got[2][:k], rf)
     finally:
         ctx.set_traversal()
-        ctx.set_option("refill", 40)  # the default
+        ctx.set_option("refill", keep)
 
 
 @pytest.mark.parametrize("fv_refill", [0, 1])

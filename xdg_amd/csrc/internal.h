@@ -159,7 +159,7 @@ struct LaunchCfg {
   int persistent = 7;
   int order = 0;  // ray_fire coherence ordering mode (0 = off)
   int spec = 1;   // ray_fire BVH4 node loop parks leaves until the wave holds one each
-  int refill = 40; // persistent ray_fire: idle lanes of a wave needed before a refill
+  int refill = 44; // persistent ray_fire: idle lanes of a wave needed before a refill
   int fv_refill = 0;  // persistent find_volume: the same threshold (0: 56 mixed, 40 plain)
   int adj_sort = 1;   // adjoint batches of >= 65536 rays: ordered by hit face, vertex sums per wave
   int mw_wave = 1;  // mesh walk: batched re-entry rays (wavefront) instead of inline ones
