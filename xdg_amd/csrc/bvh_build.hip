@@ -468,16 +468,22 @@ inline unsigned grid_for(int64_t n, int blk = BLK)
 // area(left) * n_left + area(right) * n_right (ties: lower axis, then lower
 // bin) and partitions its range stably in place.  Children of two or more
 // primitives are the next level's tasks, and node ids are handed out in task
-// order by a scan, so the tree is deterministic.  Tasks whose centroids no
-// bin boundary separates, and every task from SAH_MEDIAN_LEVEL on, split at
-// the object median of their current order (which keeps the depth within the
-// 64 levels the traversal stacks allow).  The result takes the Karras
+// order by a scan, so the tree is deterministic.  Tasks of more than
+// SAH_MORTON_MIN primitives split where the Karras tree would, at the first
+// key of their range with its highest differing Morton bit set: one workgroup
+// would otherwise bin a whole top-level range alone, and the SAH near the
+// root changes little.  Those ranges are never reordered, so they still
+// hold the sorted keys.  Tasks whose centroids no bin boundary separates, and
+// every task from SAH_MEDIAN_LEVEL on, split at the object median of their
+// current order (which keeps the depth within the 64 levels the traversal
+// stacks allow).  The result takes the Karras
 // kernel's form -- children as internal ids or ~position, ranges, parents --
 // so refit, emission, validation and the BVH4 collapse are shared.
 // ---------------------------------------------------------------------------
 constexpr int SAH_BINS = 32;
 constexpr int SAH_THREADS = 256;
 constexpr int SAH_MEDIAN_LEVEL = 36;
+constexpr int SAH_MORTON_MIN = 32768;
 
 struct SahTask {
   int32_t node, start, cnt, pad;
@@ -515,7 +521,27 @@ __global__ void k_sah_root(SahTask* tasks, int32_t* rfirst, int32_t* rlast, int3
   pint[0] = -1;
 }
 
-__global__ __launch_bounds__(SAH_THREADS) void k_sah_split(const double* __restrict__ box6, int32_t* idx,
+// Split position of the Morton-sorted range [a, b) (b - a >= 2): the first
+// key with the highest bit in which keys[a] and keys[b - 1] differ, or the
+// middle when they are equal.
+__device__ int32_t morton_split(const uint64_t* __restrict__ keys, int32_t a, int32_t b)
+{
+  const uint64_t x = keys[a] ^ keys[b - 1];
+  if (x == 0) return a + (b - a) / 2;
+  const uint64_t bit = 1ull << (63 - __clzll((long long)x));
+  int32_t lo = a + 1, hi = b - 1;  // keys[a] has the bit clear, keys[b - 1] set
+  while (lo < hi) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    if (keys[mid] & bit)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(SAH_THREADS) void k_sah_split(const double* __restrict__ box6,
+                                                          const uint64_t* __restrict__ keys, int32_t* idx,
                                                           int32_t* tmp, const SahTask* __restrict__ tasks,
                                                           int32_t* nleft, int median)
 {
@@ -531,6 +557,10 @@ __global__ __launch_bounds__(SAH_THREADS) void k_sah_split(const double* __restr
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (t.cnt <= 2 || median) {
     if (tid == 0) nleft[blockIdx.x] = t.cnt / 2;
+    return;
+  }
+  if (t.cnt > SAH_MORTON_MIN) {
+    if (tid == 0) nleft[blockIdx.x] = morton_split(keys, t.start, t.start + t.cnt) - t.start;
     return;
   }
   // centroid bounds of the task
@@ -716,9 +746,10 @@ __global__ void k_sah_link(const SahTask* __restrict__ tasks, int64_t nt, const 
 }
 
 // Binned-SAH hierarchy over the n >= 2 primitives listed in order (Morton
-// order on entry, the leaf order on return), into the topology arrays t.
-void sah_topology(const double* box6, int64_t n, int32_t* order, const LbvhTopo& t, hipStream_t s,
-                  std::vector<DevBuf>& tmp)
+// order on entry, with their sorted keys; the leaf order on return), into
+// the topology arrays t.
+void sah_topology(const double* box6, const uint64_t* keys, int64_t n, int32_t* order, const LbvhTopo& t,
+                  hipStream_t s, std::vector<DevBuf>& tmp)
 {
   const int64_t cap = n / 2 + 1;  // tasks hold >= 2 primitives each, disjoint
   auto* ta = (SahTask*)dev_alloc(sizeof(SahTask) * cap, tmp);
@@ -731,8 +762,8 @@ void sah_topology(const double* box6, int64_t n, int32_t* order, const LbvhTopo&
   int64_t nt = 1;
   int32_t next_id = 1;
   for (int level = 0; nt > 0; ++level) {
-    hipLaunchKernelGGL(k_sah_split, dim3((unsigned)nt), dim3(SAH_THREADS), 0, s, box6, order, scratch, ta, nl,
-                       level >= SAH_MEDIAN_LEVEL ? 1 : 0);
+    hipLaunchKernelGGL(k_sah_split, dim3((unsigned)nt), dim3(SAH_THREADS), 0, s, box6, keys, order, scratch, ta,
+                       nl, level >= SAH_MEDIAN_LEVEL ? 1 : 0);
     hipLaunchKernelGGL(k_sah_count, dim3(grid_for(nt)), dim3(BLK), 0, s, ta, nt, nl, ci);
     launch_scan_excl(ci, nt, tot, s);
     hipLaunchKernelGGL(k_sah_link, dim3(grid_for(nt)), dim3(BLK), 0, s, ta, nt, nl, ci, next_id, t.left, t.right,
@@ -931,7 +962,7 @@ void lbvh_build(const double* box6, int64_t n, int leaf_max, hipStream_t s, std:
       hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(ni)), dim3(BLK), 0, s, (unsigned int*)t.pint, ni, 0xffffffffu);
       hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(BLK), 0, s, (unsigned int*)t.pleaf, n, 0xffffffffu);
       if (sah)
-        sah_topology(box6, n, out.order, t, s, tmp);
+        sah_topology(box6, keys, n, out.order, t, s, tmp);
       else
         hipLaunchKernelGGL(k_karras, dim3(grid_for(ni)), dim3(BLK), 0, s, keys, n, t.left, t.right, t.rfirst,
                            t.rlast, t.pint, t.pleaf);
