@@ -96,7 +96,7 @@ const char* xdg_hip_last_error(const xdg_hip_ctx* ctx);
 int xdg_hip_set_launch(xdg_hip_ctx* ctx, int block, int stack);
 int xdg_hip_get_launch(xdg_hip_ctx* ctx, int* block, int* stack);
 /* Traversal variant: wide != 0 uses the 4-wide tree (default), else the
- * binary LBVH; defer != 0 (default) traces far-origin mixed ray_fire rays in
+ * binary tree it is collapsed from; defer != 0 (default) traces far-origin mixed ray_fire rays in
  * a separate FP64 pass; persistent selects resident-grid kernels whose
  * lanes claim new work from a counter as they finish (bit 0 plain
  * ray_fire, bit 1 mixed ray_fire, bit 2 find_volume; default 7).  Results
@@ -114,9 +114,13 @@ int xdg_hip_set_traversal(xdg_hip_ctx* ctx, int wide, int defer, int persistent)
  *                 2 as 1 plus one coarse direction bit per axis (0)
  *   "speculative" ray_fire BVH4 node loop keeps descending with a parked
  *                 leaf until every lane of the wave holds one (1)
- *   "wide"        BVH4 traversal (1) or the binary LBVH (0)
+ *   "wide"        BVH4 traversal (1) or the binary tree (0)
  *   "defer"       mixed ray_fire traces far-origin rays in an FP64 pass (1)
  *   "leaf_size"   max triangles per leaf for trees built afterwards (2)
+ *   "build"       hierarchy of trees built afterwards: top-down binned SAH
+ *                 (1, default; ranges above 32768 primitives split at the
+ *                 Morton bit) or the Karras radix tree over the Morton
+ *                 order (0).  Only speed differs, never results
  *   "refill"      persistent ray_fire: idle lanes a wave waits for before
  *                 it claims new work (1..64, default 44)
  *   "fv_refill"   the same for the persistent find_volume (1..64; 0, the
@@ -162,7 +166,7 @@ int xdg_hip_set_option(xdg_hip_ctx* ctx, const char* name, int value);
 int xdg_hip_get_option(xdg_hip_ctx* ctx, const char* name, int* value);
 int xdg_hip_synchronize(xdg_hip_ctx* ctx, void* stream);
 
-/* ---- tree construction (host arrays in, device LBVH out) ---------------- */
+/* ---- tree construction (host arrays in, device tree out) ---------------- */
 
 /* RayTracer::create_surface_tree (ray_tracing_interface.h:51; Embree:
  * ray_tracer.cpp:50-105).  One tree = all triangles of one volume, in the
