@@ -386,6 +386,49 @@ def test_tree_build_modes(ctx, torch_mod):
         ctx.set_traversal()
 
 
+@pytest.mark.parametrize("mode", [1, 0])
+def test_tree_build_degenerate_and_tiny(ctx, torch_mod, mode):
+    """Trees whose centroids no SAH bin boundary separates (600 copies of
+    one triangle beside a sphere; their order then splits at the median) and
+    trees of 1, 2 and 3 triangles, in both build modes: ray_fire (every
+    orientation, both precisions) equals the oracle's brute force, duplicate
+    faces resolved to the smallest face id."""
+    torch = torch_mod
+    m = G.single_sphere(2, 10.0)
+    v9, f, s_, fl = m.tree_arrays(1)
+    dup = 600
+    v9d = np.concatenate([v9, np.repeat(v9[:1], dup, axis=0)])
+    fd = np.concatenate([f, np.arange(dup, dtype=np.int32) + int(f.max()) + 1])
+    cases = [(v9d, fd, np.concatenate([s_, np.repeat(s_[:1], dup)]), np.concatenate([fl, np.repeat(fl[:1], dup)]))]
+    for k in (1, 2, 3):
+        cases.append((v9[:k], f[:k], s_[:k], fl[:k]))
+    n = 4000
+    org = P.ball_points(61, P.STREAM_POS, n, 9.0)
+    dirs = P.iso_dirs(61, P.STREAM_DIR, n)
+    # half of the rays aimed at the duplicated triangle's centroid
+    c = v9[0].reshape(3, 3).mean(0)
+    aim = c - org[: n // 2]
+    dirs[: n // 2] = aim / np.linalg.norm(aim, axis=1, keepdims=True)
+    o_d, d_d = dev(torch, org), dev(torch, dirs)
+    keep = ctx.get_option("build")
+    try:
+        ctx.set_option("build", mode)
+        for cv9, cf, cs, cfl in cases:
+            nt = cv9.shape[0]
+            tree = ctx.build_surface_tree(cv9.reshape(-1, 3), np.arange(3 * nt, dtype=np.int32).reshape(-1, 3),
+                                          cf, cs, cfl, m.bump(1))
+            ot = O.Tree(cv9, cf, cs, cfl, m.bump(1))
+            for orient in (0, 1, -1):
+                rt, rs, rf = ot.ray_fire(org, dirs, orientation=orient, brute=True)
+                for prec in (0, 1):
+                    t, s2, f2 = ctx.ray_fire(tree, o_d, d_d, orientation=orient, precision=prec)
+                    assert np.array_equal(bits(host(t)), bits(rt))
+                    assert np.array_equal(host(s2), rs) and np.array_equal(host(f2), rf)
+            ctx.release_surface_tree(tree)
+    finally:
+        ctx.set_option("build", keep)
+
+
 @pytest.mark.parametrize("wide", [True, False])
 def test_nested_piv_occluded_variants(ctx, torch_mod, wide):
     """Binary and 4-wide traversal, plain and mixed, give identical
