@@ -136,11 +136,18 @@ def test_mixed_origins_up_to_o_limit_nested(ctx, torch_mod, wide):
         ctx.set_traversal()
 
 
-def test_find_element_analytic_kuhn(ctx, torch_mod):
+@pytest.mark.parametrize("build", [1, 0])
+def test_find_element_analytic_kuhn(ctx, torch_mod, build):
+    """Both element-tree builds (binned SAH, Karras radix tree)."""
     torch = torch_mod
     n = 16
     mesh = G.kuhn_cube(n, 0.0)
-    et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
+    keep = ctx.get_option("build")
+    ctx.set_option("build", build)
+    try:
+        et = ctx.build_element_tree(mesh.verts, mesh.tets, mesh.adj, mesh.face_order)
+    finally:
+        ctx.set_option("build", keep)
     pts, exp = K.interior_points(n, 20000, seed=11)
     assert np.array_equal(host(ctx.find_element(et, dev(torch, pts))), exp)
     dy = K.dyadic_points(n, 3, 20000, seed=12)
