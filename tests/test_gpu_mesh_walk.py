@@ -123,6 +123,31 @@ def test_mesh_walk_resident_many_per_lane(ctx, torch_mod, walk_form, refill):
     assert out[0][1].mean() > 3
 
 
+@pytest.mark.parametrize("form", [(1, 0), (1, 32), (0, 0)])
+def test_mesh_walk_table_face_order(ctx, torch_mod, walk_form, form):
+    """A face order that is neither MOAB's nor MeshMock's (MOAB's faces in
+    the order 1, 2, 3, 0, each with its outward winding), so the walk reads
+    the order from the tree: every walk form equals the oracle, which walks
+    the same order."""
+    torch = torch_mod
+    base = G.kuhn_cube(12, 0.2, seed=4)
+    fo = np.ascontiguousarray(base.face_order[[1, 2, 3, 0]], dtype=np.int32)
+    adj, bf, be = G.tet_adjacency(base.tets, fo, base.verts.shape[0])
+    mesh = G.TetMesh(base.verts, base.tets, adj, fo, bf, be)
+    et = _setup(ctx, mesh)
+    om = O.TetMesh(mesh)
+    n, first = 12000, 77
+    rd, re, rs, rh, rt = om.mesh_walk(first, n, SEED, STREAM, [0.0] * 3, [1.0] * 3, 0.05, tally=True)
+    tally = torch.zeros(mesh.tets.shape[0], dtype=torch.int64, device="cuda")
+    walk_form(*form)
+    d, e, s, h = ctx.mesh_walk(et, first, n, SEED, STREAM, [0.0] * 3, [1.0] * 3, 0.05, tally=tally)
+    assert np.array_equal(host(s), rs) and np.array_equal(host(e), re)
+    assert np.array_equal(bits(host(d)), bits(rd))
+    assert np.array_equal(host(h).view(np.uint64), rh)
+    assert np.array_equal(host(tally), rt)
+    assert re.mean() > 3
+
+
 def test_mesh_walk_event_cap(ctx, torch_mod):
     mesh = G.kuhn_cube(8, 0.2, seed=4)
     et = _setup(ctx, mesh)
