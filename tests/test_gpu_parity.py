@@ -666,9 +666,17 @@ def _segments_setup(ctx, mesh):
     return et
 
 
-def test_kuhn_find_element_and_segments(ctx, torch_mod):
+@pytest.mark.parametrize("table_order", [False, True])
+def test_kuhn_find_element_and_segments(ctx, torch_mod, table_order):
+    """MOAB's face order, and the same faces in the order 1, 2, 3, 0 (read
+    from the tree by the walk): counts, hashes and tallies equal the
+    oracle's."""
     torch = torch_mod
     mesh = G.kuhn_cube(16, 0.2, seed=4)
+    if table_order:
+        fo = np.ascontiguousarray(mesh.face_order[[1, 2, 3, 0]], dtype=np.int32)
+        adj, bf, be = G.tet_adjacency(mesh.tets, fo, mesh.verts.shape[0])
+        mesh = G.TetMesh(mesh.verts, mesh.tets, adj, fo, bf, be)
     et = _segments_setup(ctx, mesh)
     om = O.TetMesh(mesh)
     n = 20000
