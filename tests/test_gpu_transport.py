@@ -210,6 +210,38 @@ def test_transport_forms_many_per_lane(ctx, lattice, start_form):
         ctx.transport_destroy(h)
 
 
+def test_transport_boundary_conditions_many_per_lane(ctx, lattice, start_form):
+    """Reflecting box and mixed vacuum / reflecting / transmitting sphere
+    surfaces with 2^19 histories, several per lane of the resident grids:
+    the traversal-level and whole-event kernels agree history by history
+    and on the tallies, and the first 3000 histories equal the oracle."""
+    m, vols, trees, otrees = lattice
+    box_sid = max(m.surfaces)
+    bc = {box_sid: 2}
+    for k, s_ in enumerate(sorted(x for x in m.surfaces if x != box_sid)):
+        bc[s_] = (0, 1, 2)[k % 3]
+    h = ctx.transport_create(vols, trees, m.ipc, m.senses, SEED, STREAM,
+                             [0.0] * 3, [6.0] * 3, 2.0, 0.3, 10000, bc=bc)
+    try:
+        n, first, k = 1 << 19, 5, 3000
+        r = O.transport(m, otrees, vols, SEED, STREAM, [0.0] * 3, [6.0] * 3,
+                        2.0, 0.3, 10000, first, k, bc=bc)
+        outs = []
+        for events in (1, 0):
+            ctx.set_option("transport_events", events)
+            out = ctx.transport_run(h, first, n, len(vols))
+            outs.append({key: host(v) for key, v in out.items()})
+        for key in outs[0]:
+            assert np.array_equal(outs[0][key], outs[1][key]), key
+        o = outs[0]
+        assert set(np.unique(o["status"])) <= {O.TR_ABSORBED, O.TR_VACUUM}
+        assert np.array_equal(o["events"][:k], r["events"])
+        assert np.array_equal(o["status"][:k], r["status"])
+        assert np.array_equal(o["hash"][:k].view(np.uint64), r["hash"])
+    finally:
+        ctx.transport_destroy(h)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_transport_event_rounds(ctx, lattice, prec, start_form):
     """Sorted one-event rounds before the event kernel (none, one, a few, and
